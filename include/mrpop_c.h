@@ -1,0 +1,173 @@
+/*
+ * mrpop_c.h -- C ABI of the MI355X MR.POP planner library (libmrpop.so).
+ *
+ * Plain C: pointers, sizes and POD structs only.  Every entry point returns
+ * an int status (MRPOP_OK == 0); the text of the last error of a context is
+ * available from mrpop_last_error().  Nothing here calls exit().
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   dRRT::solve / PlannerResult        src/planning/Planners.hh:14-32, 57-64
+ *   pop_settings                       src/planning/pop_settings.hh:3-32
+ *   Environment<float> and shapes      src/collision/environment.hh:11-40,
+ *                                      src/collision/shapes.hh:27-185
+ *   fkcc_single_buffer{,_mr,_drrt}     src/planning/utils.cuh:447-686
+ *   fk / fk_approx (sphere centres)    src/robots/panda_four.cuh:44-131, 289-373
+ * The reference has no C ABI of its own (its host API is the C++ template
+ * dRRT::solve); these are the entry points its C++ template binds to in this
+ * library -- see mr.pop_amd/src/planning/Planners.hh.
+ */
+#ifndef MRPOP_C_H
+#define MRPOP_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MRPOP_OK 0
+#define MRPOP_ERR_HIP 1        /* a HIP runtime call failed                 */
+#define MRPOP_ERR_INVALID 2    /* bad argument (sizes, arm count, ...)      */
+#define MRPOP_ERR_BUDGET 3     /* device-memory budget or capacity exceeded */
+#define MRPOP_ERR_TIMEOUT 4    /* wall-clock budget ran out                 */
+#define MRPOP_ERR_NO_DEVICE 5  /* no usable GPU                             */
+#define MRPOP_ERR_STATE 6      /* call not valid in the object's state      */
+
+/* ---- collision modes (which reference checker an edge check follows) ---- */
+/* Multi-arm, environment + self + arm-arm: fkcc_single_buffer (utils.cuh:447-534). */
+#define MRPOP_MODE_FULL 0
+/* Multi-arm, self + arm-arm only, no environment: fkcc_single_buffer_drrt (utils.cuh:626-686). */
+#define MRPOP_MODE_DRRT 1
+/* One arm with its base pose, environment + self: fkcc_single_buffer_mr (utils.cuh:536-624). */
+#define MRPOP_MODE_SINGLE 2
+
+/* ---- scene ------------------------------------------------------------- */
+/*
+ * Flattened Environment<float> (environment.hh:11-40).  Each list is an array
+ * of float records:
+ *   spheres    4 floats: x y z r
+ *   capsules   8 floats: x1 y1 z1 xv yv zv r rdv   (rdv = 1 / |v|^2)
+ *   cuboids   15 floats: x y z  a1x a1y a1z  a2x a2y a2z  a3x a3y a3z  r1 r2 r3
+ * z_capsules / z_cuboids are the reference's z-aligned variants (same record
+ * layout, tested with the z-aligned formulas of utils.cuh:165-228).
+ * Environment::cylinders is never tested by the reference checker
+ * (utils.cuh:231-260), so it has no slot here; cylinders from problem files
+ * are passed as capsules, as the reference's harness does
+ * (scripts/evaluate_mr.cu:38-47).
+ */
+typedef struct mrpop_scene {
+    int n_spheres;
+    int n_capsules;
+    int n_z_capsules;
+    int n_cuboids;
+    int n_z_cuboids;
+    const float *spheres;
+    const float *capsules;
+    const float *z_capsules;
+    const float *cuboids;
+    const float *z_cuboids;
+} mrpop_scene;
+
+#define MRPOP_SPHERE_FLOATS 4
+#define MRPOP_CAPSULE_FLOATS 8
+#define MRPOP_CUBOID_FLOATS 15
+#define MRPOP_MAX_OBSTACLES 4096
+#define MRPOP_MAX_ARMS 5
+
+/* ---- planner settings: pop_settings (pop_settings.hh:3-32), same names,
+ *      same defaults (mrpop_settings_default). ----------------------------- */
+typedef struct mrpop_settings {
+    int max_samples;
+    int max_iters;
+    int optimize_iters;
+    int num_new_configs;
+    int granularity;
+    float range;
+    int balance;
+    float tree_ratio;
+    int dynamic_domain;
+    float dd_alpha;
+    float dd_radius;
+    float dd_min_radius;
+    int roadmap_reassign;
+    int rrtc_iter;
+    int resample_iter;
+    int cost_bound_resample;
+    int path_simplify;
+    int phs;
+    int simplify_round;
+    int bspline_step;
+    float bspline_min_change;
+    int perturb_step;
+    int perturb_empty_step;
+} mrpop_settings;
+
+/* ---- library / context ------------------------------------------------- */
+typedef struct mrpop_ctx mrpop_ctx;
+
+/* Library version string. */
+const char *mrpop_version(void);
+
+/* Fill *s with the reference's pop_settings defaults (pop_settings.hh:4-31). */
+int mrpop_settings_default(mrpop_settings *s);
+
+/* Default world<-base poses (row-major 3x4 per arm) for 2..5 arms.
+ * 4 and 5 arms: the reference's tables (panda.cuh:16-73). */
+int mrpop_default_bases(int n_arms, float *out_bases /* n_arms*12 */);
+
+/* Create a context on HIP device `device`; owns a stream and device memory. */
+int mrpop_create(int device, mrpop_ctx **out);
+int mrpop_destroy(mrpop_ctx *ctx);
+/* Last error text of the context (never NULL). */
+const char *mrpop_last_error(const mrpop_ctx *ctx);
+/* The context's HIP stream (hipStream_t) as an opaque pointer. */
+void *mrpop_stream(mrpop_ctx *ctx);
+
+/* ---- collision world uploaded to the device ---------------------------- */
+typedef struct mrpop_world mrpop_world;
+
+/* Upload a scene plus the arm layout (n_arms in 1..5; bases NULL = default
+ * poses of mrpop_default_bases, n_arms*12 floats otherwise). */
+int mrpop_world_create(mrpop_ctx *ctx, const mrpop_scene *scene, int n_arms,
+                       const float *bases, mrpop_world **out);
+int mrpop_world_destroy(mrpop_world *w);
+
+/*
+ * Batched edge check (replaces fkcc_single_buffer{,_mr,_drrt}, utils.cuh:447-686,
+ * driven by the interpolation of pop_planner.cu:553-560).
+ * Edge e goes from from[e*D..] to to[e*D..] with D = 7*n_arms (mode FULL/DRRT)
+ * or D = 7 (mode SINGLE, arm `arm` of the world).  Sample k = 1..granularity
+ * is q = from + k * ((to - from) / granularity): the start is excluded and
+ * the end included, as in the reference.  out_edge[e] = 1 iff any sample
+ * collides under the reference's two-phase composite check.
+ * out_mask (optional) receives bit (k-1) set for each colliding sample.
+ * granularity in 1..64.
+ *
+ * mrpop_check_edges: host buffers, synchronous.
+ * mrpop_check_edges_device: device buffers, enqueued on `stream` (a
+ * hipStream_t; NULL = the context stream), no synchronisation, no allocation.
+ */
+int mrpop_check_edges(mrpop_ctx *ctx, const mrpop_world *w, int mode, int arm, int granularity,
+                      int n_edges, const float *from, const float *to, uint8_t *out_edge,
+                      uint64_t *out_mask);
+int mrpop_check_edges_device(mrpop_ctx *ctx, const mrpop_world *w, int mode, int arm,
+                             int granularity, int n_edges, const float *d_from,
+                             const float *d_to, uint32_t *d_out_edge, uint64_t *d_out_mask,
+                             void *stream);
+
+/*
+ * Sphere centres by forward kinematics (replaces fk / fk_approx,
+ * panda_four.cuh:44-131, 289-373, and fk_mr / fk_approx_mr, panda.cuh:598-829).
+ * For n_cfg configurations of the world's n_arms arms (D = 7*n_arms floats
+ * each): out_coarse[c][arm][11][3], out_fine[c][arm][59][3] (either may be NULL).
+ */
+int mrpop_sphere_centres(mrpop_ctx *ctx, const mrpop_world *w, int n_cfg, const float *cfgs,
+                         float *out_coarse, float *out_fine);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRPOP_C_H */

@@ -1,0 +1,773 @@
+// check.hip -- batched FK + two-phase collision check of interpolated edges (gfx950).
+//
+// Replaces the reference's fkcc_single_buffer / _mr / _drrt
+// (src/planning/utils.cuh:447-686) with the same per-sample verdict:
+//   coarse pass  11 spheres per arm: env test of spheres 1..10 sets the env bit
+//                of the sphere's joint (panda_four.cuh:232-276); the 21
+//                intra-arm pairs and, for several arms, all 11x11 pairs of each
+//                arm pair a < b set the self bit of arm a's joint
+//                (panda_four.cuh:134-229);
+//   fine pass    59 spheres per arm: spheres 1..58 on env-flagged joints against
+//                the environment (panda_four.cuh:456-489); intra-arm pairs
+//                gated on the self bit (multi-arm: range i gated by and centred
+//                on fine sphere i with the radius of sphere ranges[i][0],
+//                panda_four.cuh:390-416; one arm: gated by sphere ranges[i][0],
+//                panda.cuh:832-862); inter-arm: sphere i of arm a on a
+//                self-flagged joint against all 59 spheres of every arm b > a
+//                (panda_four.cuh:419-450).
+//   A sample collides iff some fine test does; an edge iff some sample does.
+//
+// MI355X mapping.
+//   check_kernel: one wave per block; lane = (sample, arm), 64/A samples per
+//   wave.  FK runs per lane in registers with the fixed joint transforms
+//   folded in; coarse centres go to LDS so that the arm-arm pairs split evenly
+//   over the A lanes of a sample (each lane takes A-1 half-pairs: rows 0-5 or
+//   6-10 of arm a against arm b).  Obstacle records are wave-uniform and come
+//   through the scalar cache.  The fine self and arm-arm tests run in the same
+//   kernel, gated exactly as above, but only where they can hit: each coarse
+//   sphere grown by a few tenths of a millimetre (kBlockInflate) contains the
+//   fine spheres of its block, so a fine pair is tested only if its bounding
+//   balls overlap.  The culling never changes a verdict; it removes ~98% of
+//   the fine pairs, including the joint-5 ranges that the coarse pair
+//   (sphere 5, spheres 7..10) switches on for every configuration.
+//   env_fine_kernel: samples with environment bits and no verdict yet are
+//   appended to a compacted queue (ballot + prefix count, one atomic per
+//   wave); a persistent grid takes one queued sample per wave, builds its
+//   frames in LDS and tests the gated fine spheres against every obstacle.
+//   The last wave to finish resets the queue counter, so back-to-back
+//   launches need no memset.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "device_geom.h"
+#include "internal.h"
+
+namespace mrpop {
+namespace dev {
+
+using u64 = unsigned long long;
+
+template <int N> struct IntC {
+    static constexpr int value = N;
+};
+template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(IntC<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+}  // namespace dev
+}  // namespace mrpop
+#include "check_groups.h"
+namespace mrpop {
+namespace dev {
+using EdgeBatch = EdgeGroup;
+
+// ---------------------------------------------------------------------------
+// Lane-indexed constant tables (offsets, radii, joints of the fine spheres).
+struct SphereTable {
+    float off[kFineSpheres][4];
+    float r[kFineSpheres];
+    unsigned char joint[kFineSpheres];
+    float coarse_r[kCoarseSpheres];
+    float block_r[kCoarseSpheres];  // coarse radius + inflation: ball holding the block's fine spheres
+    unsigned char block_first[kCoarseSpheres];
+    unsigned char block_count[kCoarseSpheres];
+};
+constexpr SphereTable build_sphere_table() {
+    SphereTable t{};
+    for (int i = 0; i < kFineSpheres; ++i) {
+        for (int c = 0; c < 4; ++c) t.off[i][c] = kFineSphere[i][c];
+        t.r[i] = kFineSphere[i][3];
+        t.joint[i] = (unsigned char)kFineJoint[i];
+    }
+    for (int i = 0; i < kCoarseSpheres; ++i) {
+        t.coarse_r[i] = kCoarseSphere[i][3];
+        t.block_r[i] = kCoarseSphere[i][3] + kBlockInflate[i];
+        t.block_first[i] = (unsigned char)kBlockFirst[i];
+        t.block_count[i] = (unsigned char)kBlockCount[i];
+    }
+    return t;
+}
+__constant__ SphereTable kTab = build_sphere_table();
+
+constexpr bool coarse_joint_is_min_i_7() {
+    for (int i = 0; i < kCoarseSpheres; ++i)
+        if (kCoarseJoint[i] != (i < 7 ? i : 7)) return false;
+    return true;
+}
+static_assert(coarse_joint_is_min_i_7(), "coarse sphere i belongs to joint min(i, 7)");
+constexpr bool blocks_follow_joints() {
+    for (int k = 0; k < kCoarseSpheres; ++k)
+        for (int i = 0; i < kBlockCount[k]; ++i)
+            if (kFineJoint[kBlockFirst[k] + i] != kCoarseJoint[k]) return false;
+    return kBlockFirst[5] == 17 && kBlockFirst[10] + kBlockCount[10] == kFineSpheres;
+}
+static_assert(blocks_follow_joints(), "fine-sphere blocks lie on their coarse sphere's joint");
+
+// Fine intra-arm ranges (src/robots/panda_four.cuh:388-417, panda.cuh:832-862):
+// the sphere whose centre is tested, the sphere whose radius is used, the
+// joint whose self bit gates the range.
+template <int MODE> constexpr int range_pos(int r) {
+    return MODE == MRPOP_MODE_SINGLE ? kFineSelfRange[r][0] : r;  // multi-arm: fine sphere r itself
+}
+constexpr int range_rad(int r) { return kFineSelfRange[r][0]; }
+template <int MODE> constexpr int range_gate(int r) { return kFineJoint[range_pos<MODE>(r)]; }
+
+// ------------------------------------------------------------- FK helpers --
+__device__ __forceinline__ void load_base(const WorldDesc &w, int arm, Frame &T) {
+    const float *b = w.obs + w.o_base + 12 * arm;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) T.m[i] = b[i];
+}
+
+// Interpolated joint angles of one arm: q = from + k * ((to - from) / G)
+// (pop_planner.cu:553-560).
+__device__ __forceinline__ void interp7(const EdgeBatch &eb, int e, int k, int off, float q[7]) {
+    const float *f = eb.from + (size_t)e * eb.dim + off;
+    const float *t = eb.to + (size_t)e * eb.dim + off;
+    const float kf = (float)k, gf = (float)eb.G;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        float d = (t[j] - f[j]) / gf;
+        q[j] = __builtin_fmaf(kf, d, f[j]);
+    }
+}
+
+// Joint transforms for a rolled (runtime-depth) FK walk.
+struct JointTable {
+    float tf[kJoints][12];
+};
+constexpr JointTable build_joint_table() {
+    JointTable t{};
+    for (int j = 0; j < kJoints; ++j)
+        for (int i = 0; i < 12; ++i) t.tf[j][i] = kJointTf[j][i];
+    return t;
+}
+__constant__ JointTable kJT = build_joint_table();
+
+// Frame of joint `jt` (runtime 0..7) of an arm at sample (e, k), walked with a
+// rolled loop.  Every operation matches joint_step on the same inputs (fma
+// with a 0 / +-1 factor is exact), so the frame equals the unrolled one.
+__device__ __noinline__ Frame frame_rt(const WorldDesc &w, const EdgeBatch &eb, int base_arm, int dof_off, int e,
+                                       int k, int jt) {
+    Frame T;
+    load_base(w, base_arm, T);
+    const float *f = eb.from + (size_t)e * eb.dim + dof_off;
+    const float *t = eb.to + (size_t)e * eb.dim + dof_off;
+    const float kf = (float)k, gf = (float)eb.G;
+#pragma unroll 1
+    for (int J = 1; J <= jt; ++J) {
+        const float qj = __builtin_fmaf(kf, (t[J - 1] - f[J - 1]) / gf, f[J - 1]);
+        float sq, cq;
+        sincosf(qj, &sq, &cq);
+        const float *F = kJT.tf[J];
+        float S[12];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            S[4 * r + 0] = __builtin_fmaf(F[4 * r + 1], sq, F[4 * r + 0] * cq);
+            S[4 * r + 1] = __builtin_fmaf(-F[4 * r + 0], sq, F[4 * r + 1] * cq);
+        }
+        Frame N;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                float v = T.m[4 * r + 0] * S[c];
+                v = __builtin_fmaf(T.m[4 * r + 1], S[4 + c], v);
+                v = __builtin_fmaf(T.m[4 * r + 2], S[8 + c], v);
+                N.m[4 * r + c] = v;
+            }
+            N.m[4 * r + 2] = __builtin_fmaf(F[10], T.m[4 * r + 2], __builtin_fmaf(F[6], T.m[4 * r + 1], F[2] * T.m[4 * r + 0]));
+            N.m[4 * r + 3] = __builtin_fmaf(F[11], T.m[4 * r + 2],
+                                            __builtin_fmaf(F[7], T.m[4 * r + 1], __builtin_fmaf(F[3], T.m[4 * r + 0], T.m[4 * r + 3])));
+        }
+        T = N;
+    }
+    return T;
+}
+
+__device__ __forceinline__ void centre_rt(const Frame &T, int s, float &x, float &y, float &z) {
+    centre(T, kTab.off[s][0], kTab.off[s][1], kTab.off[s][2], x, y, z);
+}
+
+// ------------------------------------------------------------ fused check --
+// Fine environment tests of block k's spheres (all but sphere 0) in frame T,
+// only against obstacles that its bounding ball (centre b, radius R) touches.
+__device__ __noinline__ bool block_env(const WorldDesc &w, const Frame &T, int k, float bx, float by, float bz,
+                                          float R) {
+    bool hit = false;
+    const int j0 = max((int)kTab.block_first[k], 1), j1 = kTab.block_first[k] + kTab.block_count[k];
+    const float Rsq = R * R;
+    for (int i = 0; i < w.n_sph; ++i) {
+        const float *o = w.obs + w.o_sph + 4 * i;
+        if (sphere_sphere(o[0], o[1], o[2], o[3], bx, by, bz, R) < 0.f)
+            for (int j = j0; j < j1; ++j) {
+                float x, y, z;
+                centre_rt(T, j, x, y, z);
+                hit |= sphere_sphere(o[0], o[1], o[2], o[3], x, y, z, kTab.r[j]) < 0.f;
+            }
+    }
+    for (int i = 0; i < w.n_cap; ++i) {
+        const float *o = w.obs + w.o_cap + 8 * i;
+        if (sphere_capsule(o, bx, by, bz, R) < 0.f)
+            for (int j = j0; j < j1; ++j) {
+                float x, y, z;
+                centre_rt(T, j, x, y, z);
+                hit |= sphere_capsule(o, x, y, z, kTab.r[j]) < 0.f;
+            }
+    }
+    for (int i = 0; i < w.n_zcap; ++i) {
+        const float *o = w.obs + w.o_zcap + 8 * i;
+        if (sphere_zcapsule(o, bx, by, bz, R) < 0.f)
+            for (int j = j0; j < j1; ++j) {
+                float x, y, z;
+                centre_rt(T, j, x, y, z);
+                hit |= sphere_zcapsule(o, x, y, z, kTab.r[j]) < 0.f;
+            }
+    }
+    for (int i = 0; i < w.n_aabb; ++i) {
+        const float *o = w.obs + w.o_aabb + 6 * i;
+        if (sphere_aabb(o, bx, by, bz, Rsq) < 0.f)
+            for (int j = j0; j < j1; ++j) {
+                float x, y, z;
+                centre_rt(T, j, x, y, z);
+                hit |= sphere_aabb(o, x, y, z, kTab.r[j] * kTab.r[j]) < 0.f;
+            }
+    }
+    for (int i = 0; i < w.n_obb; ++i) {
+        const float *o = w.obs + w.o_obb + 15 * i;
+        if (sphere_obb(o, bx, by, bz, Rsq) < 0.f)
+            for (int j = j0; j < j1; ++j) {
+                float x, y, z;
+                centre_rt(T, j, x, y, z);
+                hit |= sphere_obb(o, x, y, z, kTab.r[j] * kTab.r[j]) < 0.f;
+            }
+    }
+    for (int i = 0; i < w.n_zcub; ++i) {
+        const float *o = w.obs + w.o_zcub + 15 * i;
+        if (sphere_zcuboid(o, bx, by, bz, Rsq) < 0.f)
+            for (int j = j0; j < j1; ++j) {
+                float x, y, z;
+                centre_rt(T, j, x, y, z);
+                hit |= sphere_zcuboid(o, x, y, z, kTab.r[j] * kTab.r[j]) < 0.f;
+            }
+    }
+    return hit;
+}
+
+// One FK walk of this lane's arm doing, exactly as the reference gates them:
+//   * the fine intra-arm ranges whose gate joint lies in [G0, G1], with
+//     bounding-ball culling of the target blocks;
+//   * if WITH_ENV, the fine environment tests of every env-flagged joint,
+//     with bounding-ball culling of the obstacles.
+// Run once for gates on joints 0..4 and once for joint 5 plus the
+// environment, so that at most 17 tested centres are live at a time.
+template <int MODE, int G0, int G1, bool WITH_ENV>
+__device__ __forceinline__ bool fine_walk(const WorldDesc &w, const Frame &base, const float q[7], unsigned selfbits,
+                                          unsigned envbits, const float (*cs)[3][64], int lane) {
+    bool hit = false;
+    float px[kFineSelfRanges], py[kFineSelfRanges], pz[kFineSelfRanges];
+    Frame T = base;
+    static_for<0, kJoints>([&](auto jc) {
+        constexpr int J = decltype(jc)::value;
+        if constexpr (J > 0) {
+            Frame N;
+            joint_frame<J>(T, q[J - 1], N);
+            T = N;
+        }
+        // centres of the ranges' tested spheres lying on joint J
+        if constexpr (J >= G0 && J <= G1) {
+            if (selfbits & (1u << J)) {
+                static_for<0, kFineSelfRanges>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    constexpr int ps = range_pos<MODE>(r);
+                    if constexpr (kFineJoint[ps] == J)
+                        centre(T, kFineSphere[ps][0], kFineSphere[ps][1], kFineSphere[ps][2], px[r], py[r], pz[r]);
+                });
+            }
+        }
+        // fine environment tests of joint J (sphere 0 is never tested: panda_four.cuh:468)
+        if constexpr (WITH_ENV && J > 0) {
+            if (envbits & (1u << J)) {
+                static_for<1, kCoarseSpheres>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    if constexpr (kCoarseJoint[k] == J)
+                        hit |= block_env(w, T, k, cs[k][0][lane], cs[k][1][lane], cs[k][2][lane],
+                                         kCoarseSphere[k][3] + kBlockInflate[k]);
+                });
+            }
+        }
+        // target blocks on joint J (targets are fine spheres 17..58: blocks 5..10)
+        static_for<5, kCoarseSpheres>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            if constexpr (kCoarseJoint[k] == J) {
+                const float bx = cs[k][0][lane], by = cs[k][1][lane], bz = cs[k][2][lane];
+                constexpr int b0 = kBlockFirst[k], b1 = kBlockFirst[k] + kBlockCount[k] - 1;
+                constexpr float R = kCoarseSphere[k][3] + kBlockInflate[k];
+                static_for<0, kFineSelfRanges>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    constexpr int lo = kFineSelfRange[r][1] > b0 ? kFineSelfRange[r][1] : b0;
+                    constexpr int hi = kFineSelfRange[r][2] < b1 ? kFineSelfRange[r][2] : b1;
+                    constexpr int gj = range_gate<MODE>(r);
+                    if constexpr (lo <= hi && gj >= G0 && gj <= G1) {
+                        constexpr float rad = kFineSphere[range_rad(r)][3];
+                        if (selfbits & (1u << gj)) {
+                            const float dx = px[r] - bx, dy = py[r] - by, dz = pz[r] - bz;
+                            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                            if (d2 < (rad + R) * (rad + R)) {
+                                for (int t = lo; t <= hi; ++t) {
+                                    float x, y, z;
+                                    centre_rt(T, t, x, y, z);
+                                    hit |= sphere_sphere(px[r], py[r], pz[r], rad, x, y, z, kTab.r[t]) < 0.f;
+                                }
+                            }
+                        }
+                    }
+                });
+            }
+        });
+    });
+    return hit;
+}
+
+// Decode half-pair h (0 .. A(A-1)-1) into arm pair (a, b), a < b, and the
+// first row i0 (0 or 6) of arm a's coarse spheres it covers.
+template <int A> __device__ __forceinline__ void half_pair(int h, int &a, int &b, int &i0) {
+    const int p = h >> 1;
+    int aa = 0, rem = p;
+#pragma unroll
+    for (int t = 0; t < A - 1; ++t) {
+        const int row = A - 1 - t;
+        if (aa == t && rem >= row) {
+            rem -= row;
+            aa = t + 1;
+        }
+    }
+    a = aa;
+    b = aa + 1 + rem;
+    i0 = (h & 1) ? 6 : 0;
+}
+
+template <int A, int MODE>
+__global__ __launch_bounds__(64) void check_kernel(const WorldDesc *worlds, const EdgeGroup *groups, EdgeGroup g0,
+                                                   int n_groups, const unsigned *wave_start, FineQueue fq) {
+    constexpr int S = 64 / A;  // samples per wave
+    constexpr bool kEnv = MODE != MRPOP_MODE_DRRT;
+    constexpr bool kInter = A > 1;
+    constexpr int kHalf = kInter ? A - 1 : 1;
+    __shared__ float cs[kCoarseSpheres][3][64];
+    __shared__ unsigned inter_flags[64];
+    __shared__ unsigned result[64];
+    __shared__ unsigned long long cand_lo[kHalf][64];  // candidate blocks, rows 0..4: bit t*11 + k
+    __shared__ unsigned cand_hi[kHalf][64];            // row 5: bit k
+
+    const int lane = threadIdx.x;
+    const int slot = lane / A;
+    const int arm = lane - slot * A;
+    const int sbase = slot < S ? slot * A : 0;  // spare lanes (64 % A) shadow slot 0 and discard
+    const unsigned total_waves =
+        groups ? wave_start[n_groups] : (unsigned)(((long)g0.n_edges * g0.G + S - 1) / S);
+
+    for (unsigned wv = blockIdx.x; wv < total_waves; wv += gridDim.x) {
+    int gi = 0;
+    if (groups) {  // last group whose first wave is <= wv
+        int lo = 0, hi = n_groups - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (wave_start[mid] <= wv) lo = mid;
+            else hi = mid - 1;
+        }
+        gi = lo;
+    }
+    const EdgeGroup eb = groups ? groups[gi] : g0;
+    const WorldDesc w = worlds[eb.world];
+    const unsigned lw = groups ? wv - wave_start[gi] : wv;
+    const long total = (long)(eb.count ? *eb.count : eb.n_edges) * eb.G;
+    const long n = (long)lw * S + slot;
+    const bool active = slot < S && n < total;
+    const long nn = active ? n : 0;
+    const int e = (int)(nn / eb.G);
+    const int k = (int)(nn - (long)e * eb.G) + 1;
+    const int base_arm = MODE == MRPOP_MODE_SINGLE ? eb.arm : arm;
+    const int dof_off = MODE == MRPOP_MODE_SINGLE ? 0 : 7 * arm;
+
+    // ---- A. coarse FK of this lane's arm
+    float cx[kCoarseSpheres], cy[kCoarseSpheres], cz[kCoarseSpheres];
+    {
+        float q[7];
+        interp7(eb, e, k, dof_off, q);
+        Frame T;
+        load_base(w, base_arm, T);
+        centre(T, kCoarseSphere[0][0], kCoarseSphere[0][1], kCoarseSphere[0][2], cx[0], cy[0], cz[0]);
+        static_for<1, kJoints>([&](auto jc) {
+            constexpr int J = decltype(jc)::value;
+            Frame N;
+            joint_frame<J>(T, q[J - 1], N);
+            T = N;
+            static_for<kCoarseJointStart[J], kCoarseJointStart[J] + kCoarseJointCount[J]>([&](auto sc) {
+                constexpr int s = decltype(sc)::value;
+                centre(T, kCoarseSphere[s][0], kCoarseSphere[s][1], kCoarseSphere[s][2], cx[s], cy[s], cz[s]);
+            });
+        });
+    }
+#pragma unroll
+    for (int s = 0; s < kCoarseSpheres; ++s) {
+        cs[s][0][lane] = cx[s];
+        cs[s][1][lane] = cy[s];
+        cs[s][2][lane] = cz[s];
+    }
+    inter_flags[lane] = 0u;
+
+    // coarse environment: spheres 1..10 -> env bit of their joint (panda_four.cuh:241-261)
+    unsigned hitmask = 0;
+    if constexpr (kEnv) {
+        auto test_all = [&](auto &&fn) {
+#pragma unroll
+            for (int s = 1; s < kCoarseSpheres; ++s)
+                if (fn(cx[s], cy[s], cz[s], kCoarseSphere[s][3]) < 0.f) hitmask |= 1u << s;
+        };
+        for (int i = 0; i < w.n_sph; ++i) {
+            const float *o = w.obs + w.o_sph + 4 * i;
+            const float ox = o[0], oy = o[1], oz = o[2], orr = o[3];
+            test_all([&](float x, float y, float z, float r) { return sphere_sphere(ox, oy, oz, orr, x, y, z, r); });
+        }
+        for (int i = 0; i < w.n_cap; ++i) {
+            const float *o = w.obs + w.o_cap + 8 * i;
+            test_all([&](float x, float y, float z, float r) { return sphere_capsule(o, x, y, z, r); });
+        }
+        for (int i = 0; i < w.n_zcap; ++i) {
+            const float *o = w.obs + w.o_zcap + 8 * i;
+            test_all([&](float x, float y, float z, float r) { return sphere_zcapsule(o, x, y, z, r); });
+        }
+        for (int i = 0; i < w.n_aabb; ++i) {
+            const float *o = w.obs + w.o_aabb + 6 * i;
+            test_all([&](float x, float y, float z, float r) { return sphere_aabb(o, x, y, z, r * r); });
+        }
+        for (int i = 0; i < w.n_obb; ++i) {
+            const float *o = w.obs + w.o_obb + 15 * i;
+            test_all([&](float x, float y, float z, float r) { return sphere_obb(o, x, y, z, r * r); });
+        }
+        for (int i = 0; i < w.n_zcub; ++i) {
+            const float *o = w.obs + w.o_zcub + 15 * i;
+            test_all([&](float x, float y, float z, float r) { return sphere_zcuboid(o, x, y, z, r * r); });
+        }
+    }
+    unsigned envbits = 0;
+#pragma unroll
+    for (int s = 1; s < kCoarseSpheres; ++s)
+        if (hitmask & (1u << s)) envbits |= 1u << kCoarseJoint[s];
+
+    // coarse intra-arm pairs -> self bit of the first sphere's joint (panda_four.cuh:144-176)
+    unsigned selfbits = 0;
+#pragma unroll
+    for (int r = 0; r < kCoarseSelfRanges; ++r) {
+        const int s1 = kCoarseSelfRange[r][0];
+#pragma unroll
+        for (int j = kCoarseSelfRange[r][1]; j <= kCoarseSelfRange[r][2]; ++j)
+            if (sphere_sphere(cx[s1], cy[s1], cz[s1], kCoarseSphere[s1][3], cx[j], cy[j], cz[j], kCoarseSphere[j][3]) <
+                0.f)
+                selfbits |= 1u << kCoarseJoint[s1];
+    }
+    __syncthreads();
+
+    // coarse arm-arm pairs (panda_four.cuh:178-214), A-1 half-pairs per lane.
+    // The same distances give the candidate blocks of the fine arm-arm check:
+    // blocks p of arm a and k of arm b whose bounding balls overlap.
+    if constexpr (kInter) {
+#pragma unroll
+        for (int m = 0; m < A - 1; ++m) {
+            int a, b, i0;
+            half_pair<A>(arm + m * A, a, b, i0);
+            float bx[kCoarseSpheres], by[kCoarseSpheres], bz[kCoarseSpheres];
+#pragma unroll
+            for (int j = 0; j < kCoarseSpheres; ++j) {
+                bx[j] = cs[j][0][sbase + b];
+                by[j] = cs[j][1][sbase + b];
+                bz[j] = cs[j][2][sbase + b];
+            }
+            unsigned mask = 0;
+            unsigned long long clo = 0;
+            unsigned chi = 0;
+#pragma unroll 1
+            for (int t = 0; t < 6; ++t) {
+                const int i = min(i0 + t, kCoarseSpheres - 1);  // rows 6..10 are 5: repeat row 10
+                const float ax = cs[i][0][sbase + a], ay = cs[i][1][sbase + a], az = cs[i][2][sbase + a];
+                const float rad = kTab.coarse_r[i], brad = kTab.block_r[i];
+                bool hit = false;
+                unsigned cand = 0;
+#pragma unroll
+                for (int j = 0; j < kCoarseSpheres; ++j) {
+                    const float dx = ax - bx[j], dy = ay - by[j], dz = az - bz[j];
+                    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                    const float rs = rad + kCoarseSphere[j][3];
+                    const float Rs = brad + (kCoarseSphere[j][3] + kBlockInflate[j]);
+                    hit |= d2 - rs * rs < 0.f;
+                    if (d2 < Rs * Rs) cand |= 1u << j;
+                }
+                if (hit) mask |= 1u << (i < 7 ? i : 7);
+                if (t < 5)
+                    clo |= (unsigned long long)cand << (11 * t);
+                else if (i0 == 0)
+                    chi = cand;
+            }
+            cand_lo[m][lane] = active ? clo : 0ull;
+            cand_hi[m][lane] = active ? chi : 0u;
+            if (mask && slot < S) atomicOr(&inter_flags[sbase + a], mask);
+        }
+    }
+    __syncthreads();
+    if constexpr (kInter) selfbits |= inter_flags[lane];
+    // publish the final self bits of every arm for the arm-arm fine stage
+    result[lane] = selfbits;
+    __syncthreads();
+
+    // ---- B. fine intra-arm pairs of this lane's arm
+    bool finehit = false;
+    if (active && (selfbits | envbits)) {
+        float q[7];
+        interp7(eb, e, k, dof_off, q);
+        Frame base;
+        load_base(w, base_arm, base);
+        if (selfbits & 0x1fu) finehit = fine_walk<MODE, 0, 4, false>(w, base, q, selfbits, 0u, cs, lane);
+        if ((selfbits & 0x20u) || envbits) finehit |= fine_walk<MODE, 5, 5, kEnv>(w, base, q, selfbits, envbits, cs, lane);
+    }
+
+    // ---- C. fine arm-arm pairs for candidate blocks of this lane's half-pairs
+    if constexpr (kInter) {
+#pragma unroll 1
+        for (int m = 0; m < A - 1; ++m) {
+            int a, b, i0;
+            half_pair<A>(arm + m * A, a, b, i0);
+            const unsigned gate = result[sbase + a];  // arm a's self bits
+            unsigned long long clo = cand_lo[m][lane];
+            unsigned chi = cand_hi[m][lane];
+            // keep rows whose block's joint is self-flagged on arm a
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                const int p = i0 + t;
+                if (!((gate >> (p < 7 ? p : 7)) & 1u)) clo &= ~(0x7ffull << (11 * t));
+            }
+            if (!((gate >> 5) & 1u)) chi = 0;  // row 5 exists only for i0 == 0 (block 5)
+            while (clo | chi) {
+                int t, kb;
+                if (clo) {
+                    const int bit = __builtin_ctzll(clo);
+                    clo &= clo - 1;
+                    t = bit / 11;
+                    kb = bit - 11 * t;
+                } else {
+                    kb = __builtin_ctz(chi);
+                    chi &= chi - 1;
+                    t = 5;
+                }
+                const int p = i0 + t;
+                const Frame Ta = frame_rt(w, eb, a, 7 * a, e, k, p < 7 ? p : 7);
+                const Frame Tb = frame_rt(w, eb, b, 7 * b, e, k, kb < 7 ? kb : 7);
+                const int p0 = kTab.block_first[p], pn = kTab.block_count[p];
+                const int k0 = kTab.block_first[kb], kn = kTab.block_count[kb];
+                for (int i = p0; i < p0 + pn; ++i) {
+                    float ax, ay, az;
+                    centre_rt(Ta, i, ax, ay, az);
+                    const float ar = kTab.r[i];
+                    for (int j = k0; j < k0 + kn; ++j) {
+                        float x, y, z;
+                        centre_rt(Tb, j, x, y, z);
+                        finehit |= sphere_sphere(ax, ay, az, ar, x, y, z, kTab.r[j]) < 0.f;
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- D. per-sample verdict: OR over the sample's lanes
+    if (fq.diag && active)
+        fq.diag[(size_t)n * 8 + arm] = envbits | (selfbits << 8) | (finehit ? 1u << 16 : 0u) | (1u << 17);
+    __syncthreads();
+    result[lane] = finehit ? 1u : 0u;
+    __syncthreads();
+    if (active && arm == 0) {
+        bool coll = false;
+#pragma unroll
+        for (int a = 0; a < A; ++a) coll |= result[slot * A + a] != 0u;
+        if (coll) {
+            atomicOr(&eb.out_edge[e], 1u);
+            if (eb.out_mask) atomicOr(&eb.out_mask[e], 1ull << (k - 1));
+        }
+        if (fq.diag) fq.diag[(size_t)n * 8 + 7] = (coll ? 1u : 0u) | (1u << 17);
+    }
+    __syncthreads();  // LDS is reused by the next wave-tile
+    }
+}
+
+// ---------------------------------------------------------- centres kernel --
+// One thread per (configuration, arm): all coarse and fine centres.
+__global__ void centres_kernel(WorldDesc w, int n_cfg, const float *cfg, float *out_coarse, float *out_fine) {
+    const int A = w.n_arms;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_cfg * A) return;
+    const int c = t / A, arm = t - c * A;
+    const float *q = cfg + (size_t)c * 7 * A + 7 * arm;
+    Frame T;
+    load_base(w, arm, T);
+    auto emit = [&](const Frame &F, int joint) {
+        if (out_coarse)
+            for (int s = kCoarseJointStart[joint]; s < kCoarseJointStart[joint] + kCoarseJointCount[joint]; ++s) {
+                float x, y, z;
+                centre(F, kCoarseSphere[s][0], kCoarseSphere[s][1], kCoarseSphere[s][2], x, y, z);
+                float *o = out_coarse + (((size_t)c * A + arm) * kCoarseSpheres + s) * 3;
+                o[0] = x;
+                o[1] = y;
+                o[2] = z;
+            }
+        if (out_fine)
+            for (int s = kFineJointStart[joint]; s < kFineJointStart[joint] + kFineJointCount[joint]; ++s) {
+                float x, y, z;
+                centre(F, kTab.off[s][0], kTab.off[s][1], kTab.off[s][2], x, y, z);
+                float *o = out_fine + (((size_t)c * A + arm) * kFineSpheres + s) * 3;
+                o[0] = x;
+                o[1] = y;
+                o[2] = z;
+            }
+    };
+    emit(T, 0);
+    static_for<1, kJoints>([&](auto jc) {
+        constexpr int J = decltype(jc)::value;
+        Frame N;
+        joint_frame<J>(T, q[J - 1], N);
+        T = N;
+        emit(T, J);
+    });
+}
+
+}  // namespace dev
+}  // namespace mrpop
+
+// ------------------------------------------------------------- launchers --
+namespace mrpop {
+
+namespace {
+
+// Clears the verdict words of every edge before a check (the fused kernel
+// only ORs into them, from any wave).
+__global__ void zero_outputs_kernel(const dev::EdgeGroup *groups, dev::EdgeGroup g0, int n_groups) {
+    for (int gi = blockIdx.y; gi < n_groups; gi += gridDim.y) {
+        const dev::EdgeGroup &g = groups ? groups[gi] : g0;
+        const int n = g.count ? *g.count : g.n_edges;
+        for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+            g.out_edge[e] = 0u;
+            if (g.out_mask) g.out_mask[e] = 0ull;
+        }
+    }
+}
+
+// Waves of each group (ceil(samples / S)) -> exclusive prefix; wave_start[n] = total.
+__global__ void wave_prefix_kernel(const dev::EdgeGroup *groups, int n_groups, int S, unsigned *wave_start) {
+    __shared__ unsigned sh[256];
+    const int tid = threadIdx.x;
+    unsigned carry = 0;
+    for (int base = 0; base < n_groups; base += 256) {
+        const int i = base + tid;
+        unsigned v = 0;
+        if (i < n_groups) {
+            const dev::EdgeGroup &g = groups[i];
+            const unsigned long long samples = (unsigned long long)(g.count ? *g.count : g.n_edges) * g.G;
+            v = (unsigned)((samples + S - 1) / S);
+        }
+        sh[tid] = v;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const unsigned add = tid >= off ? sh[tid - off] : 0u;
+            __syncthreads();
+            sh[tid] += add;
+            __syncthreads();
+        }
+        if (i < n_groups) wave_start[i] = carry + sh[tid] - v;
+        carry += sh[255];
+        __syncthreads();
+    }
+    if (tid == 0) wave_start[n_groups] = carry;
+}
+
+template <int A, int MODE>
+void enqueue_check(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups, const dev::EdgeGroup &g0,
+                   int n_groups, unsigned *d_wave_start, const FineQueue &aux, int check_blocks,
+                   hipStream_t stream) {
+    constexpr int S = 64 / A;
+    hipLaunchKernelGGL(zero_outputs_kernel, dim3(d_groups ? 4 : std::max(1, std::min(1024, (g0.n_edges + 255) / 256)),
+                                                 std::min(n_groups, 1024)),
+                       dim3(256), 0, stream, d_groups, g0, n_groups);
+    if (d_groups)
+        hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups, S, d_wave_start);
+    hipLaunchKernelGGL((dev::check_kernel<A, MODE>), dim3(check_blocks), dim3(64), 0, stream, d_worlds, d_groups, g0,
+                       n_groups, d_wave_start, aux);
+}
+
+int dispatch(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
+             const dev::EdgeGroup &g0, int n_groups, unsigned *d_wave_start, const FineQueue &aux, int check_blocks,
+             hipStream_t stream) {
+#define MRPOP_CASE(M, AA)                                                                                         \
+    case M * 8 + AA:                                                                                              \
+        enqueue_check<AA, M>(d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, stream);          \
+        break;
+    switch (mode * 8 + A) {
+        MRPOP_CASE(MRPOP_MODE_SINGLE, 1)
+        MRPOP_CASE(MRPOP_MODE_FULL, 2)
+        MRPOP_CASE(MRPOP_MODE_FULL, 3)
+        MRPOP_CASE(MRPOP_MODE_FULL, 4)
+        MRPOP_CASE(MRPOP_MODE_FULL, 5)
+        MRPOP_CASE(MRPOP_MODE_DRRT, 2)
+        MRPOP_CASE(MRPOP_MODE_DRRT, 3)
+        MRPOP_CASE(MRPOP_MODE_DRRT, 4)
+        MRPOP_CASE(MRPOP_MODE_DRRT, 5)
+        default: return c->fail(MRPOP_ERR_INVALID, "unsupported mode %d with %d arms", mode, A);
+    }
+#undef MRPOP_CASE
+    MRPOP_HIP(c, hipGetLastError());
+    return MRPOP_OK;
+}
+
+}  // namespace
+
+int launch_check(Context *c, const World *w, int mode, int arm, int granularity, int n_edges, const float *d_from,
+                 const float *d_to, uint32_t *d_out_edge, uint64_t *d_out_mask, hipStream_t stream) {
+    if (n_edges <= 0) return MRPOP_OK;
+    if (granularity < 1 || granularity > 64) return c->fail(MRPOP_ERR_INVALID, "granularity %d outside 1..64", granularity);
+    const int A = mode == MRPOP_MODE_SINGLE ? 1 : w->n_arms;
+    if (mode == MRPOP_MODE_SINGLE && (arm < 0 || arm >= w->n_arms))
+        return c->fail(MRPOP_ERR_INVALID, "arm %d outside the world's %d arms", arm, w->n_arms);
+    if (mode != MRPOP_MODE_SINGLE && (A < 2 || A > 5))
+        return c->fail(MRPOP_ERR_INVALID, "multi-arm modes need 2..5 arms (world has %d)", A);
+    const size_t total = (size_t)n_edges * granularity;
+    if (total > 0x7fffffffull) return c->fail(MRPOP_ERR_INVALID, "too many samples in one batch");
+    const FineQueue aux{(unsigned *)c->diag};
+    dev::EdgeGroup g0{d_from, d_to, mode == MRPOP_MODE_SINGLE ? 7 : 7 * A, granularity, n_edges, arm, nullptr,
+                      d_out_edge, (unsigned long long *)d_out_mask, 0, 0};
+    const int S = 64 / A;
+    const size_t waves = (total + S - 1) / S;
+    const int check_blocks = (int)std::min<size_t>(waves, (size_t)c->num_cus * 32);
+    return dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux, check_blocks, stream);
+}
+
+int launch_check_groups(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
+                        int n_groups, unsigned *d_wave_start, hipStream_t stream) {
+    if (n_groups <= 0) return MRPOP_OK;
+    const FineQueue aux{nullptr};
+    dev::EdgeGroup g0{};
+    return dispatch(c, mode, A, d_worlds, d_groups, g0, n_groups, d_wave_start, aux, c->num_cus * 32, stream);
+}
+
+int launch_centres(Context *c, const World *w, int n_cfg, const float *d_cfg, float *d_coarse, float *d_fine,
+                   hipStream_t stream) {
+    if (n_cfg <= 0) return MRPOP_OK;
+    const int threads = n_cfg * w->n_arms;
+    hipLaunchKernelGGL(dev::centres_kernel, dim3((threads + 63) / 64), dim3(64), 0, stream, w->desc, n_cfg, d_cfg,
+                       d_coarse, d_fine);
+    MRPOP_HIP(c, hipGetLastError());
+    return MRPOP_OK;
+}
+
+}  // namespace mrpop

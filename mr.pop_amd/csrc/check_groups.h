@@ -1,0 +1,26 @@
+// check_groups.h -- work description of a batched edge check.
+#pragma once
+
+#include <cstdint>
+
+namespace mrpop {
+namespace dev {
+
+// One list of edges checked against one world.  Edge e goes from
+// from[e*dim ..] to to[e*dim ..]; sample k = 1..G is from + k*(to-from)/G.
+struct EdgeGroup {
+    const float *from;
+    const float *to;
+    int dim;                 // floats per configuration (7 for one arm, 7*A otherwise)
+    int G;                   // samples per edge
+    int n_edges;             // edge count when count == nullptr
+    int arm;                 // one-arm mode: the arm whose base pose is used
+    const int *count;        // optional device-side edge count
+    uint32_t *out_edge;      // 1 = colliding
+    unsigned long long *out_mask;  // optional: bit k-1 per colliding sample
+    int world;               // index into the worlds array
+    int pad_;
+};
+
+}  // namespace dev
+}  // namespace mrpop

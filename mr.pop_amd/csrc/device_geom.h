@@ -1,0 +1,162 @@
+// device_geom.h -- forward kinematics and sphere/obstacle tests for gfx950.
+//
+// Forward kinematics (reference: panda_four.cuh:44-131, 289-373; utils.cuh:400-445):
+//   T_0 = base, T_j = T_{j-1} * FT_j * Rz(q_{j-1}), sphere centre = T_joint * p.
+// Written per joint with the fixed transform FT_j known at compile time, so
+// the products with its 0 / +-1 entries fold away; frames are rigid 3x4.
+//
+// Primitive tests follow utils.cuh:100-260: each returns the reference's
+// value (squared clearance form); collision iff value < 0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "panda_geometry.h"
+
+namespace mrpop {
+namespace dev {
+
+using namespace mrpop::geom;
+
+struct Frame {
+    float m[12];  // row-major [R | t]
+};
+
+// k * x for a compile-time constant k: exact for k in {0, 1, -1}.
+__device__ __forceinline__ constexpr float cmul(float k, float x) {
+    return k == 0.f ? 0.f : (k == 1.f ? x : (k == -1.f ? -x : k * x));
+}
+__device__ __forceinline__ constexpr float cfma(float k, float x, float acc) {
+    return k == 0.f ? acc : (k == 1.f ? acc + x : (k == -1.f ? acc - x : __builtin_fmaf(k, x, acc)));
+}
+
+// out = P * FT_J * Rz(q) with cq = cos(q), sq = sin(q).
+template <int J>
+__device__ __forceinline__ void joint_step(const Frame &P, float cq, float sq, Frame &out) {
+    constexpr const float *F = kJointTf[J];
+    // S = FT * Rz: S[r][0] = F[r][0] c + F[r][1] s; S[r][1] = F[r][1] c - F[r][0] s;
+    //              S[r][2] = F[r][2]; S[r][3] = F[r][3].
+    float S[12];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        S[4 * r + 0] = cfma(F[4 * r + 1], sq, cmul(F[4 * r + 0], cq));
+        S[4 * r + 1] = cfma(-F[4 * r + 0], sq, cmul(F[4 * r + 1], cq));
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            float v = P.m[4 * r + 0] * S[k];
+            v = __builtin_fmaf(P.m[4 * r + 1], S[4 + k], v);
+            v = __builtin_fmaf(P.m[4 * r + 2], S[8 + k], v);
+            out.m[4 * r + k] = v;
+        }
+        // column 2: P * F[:,2] (constant)
+        out.m[4 * r + 2] = cfma(F[10], P.m[4 * r + 2], cfma(F[6], P.m[4 * r + 1], cmul(F[2], P.m[4 * r + 0])));
+        // translation: P * F[:,3] + P_t
+        out.m[4 * r + 3] =
+            cfma(F[11], P.m[4 * r + 2], cfma(F[7], P.m[4 * r + 1], cfma(F[3], P.m[4 * r + 0], P.m[4 * r + 3])));
+    }
+}
+
+// T_J from T_{J-1} and the joint angle q_{J-1}.
+template <int J>
+__device__ __forceinline__ void joint_frame(const Frame &P, float q, Frame &out) {
+    float s, c;
+    sincosf(q, &s, &c);
+    joint_step<J>(P, c, s, out);
+}
+
+// Centre of a sphere with compile-time offset p in frame T.
+__device__ __forceinline__ void centre(const Frame &T, float px, float py, float pz, float &x, float &y, float &z) {
+    x = __builtin_fmaf(T.m[0], px, __builtin_fmaf(T.m[1], py, __builtin_fmaf(T.m[2], pz, T.m[3])));
+    y = __builtin_fmaf(T.m[4], px, __builtin_fmaf(T.m[5], py, __builtin_fmaf(T.m[6], pz, T.m[7])));
+    z = __builtin_fmaf(T.m[8], px, __builtin_fmaf(T.m[9], py, __builtin_fmaf(T.m[10], pz, T.m[11])));
+}
+
+// ---------------------------------------------------------- obstacles ----
+// Device obstacle table: one float buffer, sections in this order.
+// Records: sphere 4 (x y z r); capsule 8 (x1 y1 z1 xv yv zv r rdv);
+// aabb 6 (x y z hx hy hz) -- cuboids whose axes are exactly the world axes;
+// obb 15 (reference cuboid record); zcap 8; zcub 15.
+// The arm base poses (n_arms x 12, row-major 3x4) follow at o_base.
+struct WorldDesc {
+    const float *obs;
+    int n_sph, n_cap, n_zcap, n_aabb, n_obb, n_zcub;
+    int o_sph, o_cap, o_zcap, o_aabb, o_obb, o_zcub, o_base;  // float offsets into obs
+    int n_arms;
+};
+
+// utils.cuh:100-107
+__device__ __forceinline__ float sphere_sphere(float ax, float ay, float az, float ar, float bx, float by, float bz,
+                                               float br) {
+    float dx = ax - bx, dy = ay - by, dz = az - bz;
+    float rs = ar + br;
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) - rs * rs;
+}
+
+// utils.cuh:148-158
+__device__ __forceinline__ float sphere_capsule(const float *c, float x, float y, float z, float r) {
+    float dot = (x - c[0]) * c[3] + (y - c[1]) * c[4] + (z - c[2]) * c[5];
+    float t = fmaxf(fminf(dot * c[7], 1.f), 0.f);
+    float dx = x - __builtin_fmaf(c[3], t, c[0]);
+    float dy = y - __builtin_fmaf(c[4], t, c[1]);
+    float dz = z - __builtin_fmaf(c[5], t, c[2]);
+    float rs = r + c[6];
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) - rs * rs;
+}
+
+// utils.cuh:165-174
+__device__ __forceinline__ float sphere_zcapsule(const float *c, float x, float y, float z, float r) {
+    float dot = (z - c[2]) * c[5];
+    float t = fmaxf(fminf(dot * c[7], 1.f), 0.f);
+    float dx = x - c[0], dy = y - c[1], dz = z - __builtin_fmaf(c[5], t, c[2]);
+    float rs = r + c[6];
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) - rs * rs;
+}
+
+// utils.cuh:185-201 for a cuboid whose axes are the world axes: the axis dot
+// products are then exactly the coordinate differences.
+__device__ __forceinline__ float sphere_aabb(const float *b, float x, float y, float z, float rsq) {
+    float a1 = fmaxf(fabsf(x - b[0]) - b[3], 0.f);
+    float a2 = fmaxf(fabsf(y - b[1]) - b[4], 0.f);
+    float a3 = fmaxf(fabsf(z - b[2]) - b[5], 0.f);
+    return __builtin_fmaf(a1, a1, __builtin_fmaf(a2, a2, a3 * a3)) - rsq;
+}
+
+// utils.cuh:185-201
+__device__ __forceinline__ float sphere_obb(const float *b, float x, float y, float z, float rsq) {
+    float xs = x - b[0], ys = y - b[1], zs = z - b[2];
+    float a1 = fmaxf(fabsf(__builtin_fmaf(b[3], xs, __builtin_fmaf(b[4], ys, b[5] * zs))) - b[12], 0.f);
+    float a2 = fmaxf(fabsf(__builtin_fmaf(b[6], xs, __builtin_fmaf(b[7], ys, b[8] * zs))) - b[13], 0.f);
+    float a3 = fmaxf(fabsf(__builtin_fmaf(b[9], xs, __builtin_fmaf(b[10], ys, b[11] * zs))) - b[14], 0.f);
+    return __builtin_fmaf(a1, a1, __builtin_fmaf(a2, a2, a3 * a3)) - rsq;
+}
+
+// utils.cuh:208-221
+__device__ __forceinline__ float sphere_zcuboid(const float *b, float x, float y, float z, float rsq) {
+    float xs = x - b[0], ys = y - b[1], zs = z - b[2];
+    float a1 = fmaxf(fabsf(__builtin_fmaf(b[3], xs, b[4] * ys)) - b[12], 0.f);
+    float a2 = fmaxf(fabsf(__builtin_fmaf(b[6], xs, b[7] * ys)) - b[13], 0.f);
+    float a3 = fmaxf(fabsf(zs) - b[14], 0.f);
+    return __builtin_fmaf(a1, a1, __builtin_fmaf(a2, a2, a3 * a3)) - rsq;
+}
+
+// True iff the sphere touches any obstacle (utils.cuh:231-260).
+__device__ __forceinline__ bool env_hit(const WorldDesc &w, float x, float y, float z, float r) {
+    const float rsq = r * r;
+    bool hit = false;
+    for (int i = 0; i < w.n_sph; ++i) {
+        const float *o = w.obs + w.o_sph + 4 * i;
+        hit |= sphere_sphere(o[0], o[1], o[2], o[3], x, y, z, r) < 0.f;
+    }
+    for (int i = 0; i < w.n_cap; ++i) hit |= sphere_capsule(w.obs + w.o_cap + 8 * i, x, y, z, r) < 0.f;
+    for (int i = 0; i < w.n_zcap; ++i) hit |= sphere_zcapsule(w.obs + w.o_zcap + 8 * i, x, y, z, r) < 0.f;
+    for (int i = 0; i < w.n_aabb; ++i) hit |= sphere_aabb(w.obs + w.o_aabb + 6 * i, x, y, z, rsq) < 0.f;
+    for (int i = 0; i < w.n_obb; ++i) hit |= sphere_obb(w.obs + w.o_obb + 15 * i, x, y, z, rsq) < 0.f;
+    for (int i = 0; i < w.n_zcub; ++i) hit |= sphere_zcuboid(w.obs + w.o_zcub + 15 * i, x, y, z, rsq) < 0.f;
+    return hit;
+}
+
+}  // namespace dev
+}  // namespace mrpop

@@ -1,0 +1,88 @@
+// internal.h -- host-side internals shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mrpop_c.h"
+#include "device_geom.h"
+
+namespace mrpop {
+
+// Grow-only device buffer owned by a context.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// Optional outputs of the fused edge-check kernel.
+struct FineQueue {
+    unsigned *diag;   // per (sample, arm) phase results, 8 words per sample (diagnostics only)
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    std::string err;
+    DevBuf counters;
+    // staging for host-buffer entry points
+    DevBuf from, to, out_edge, out_mask, cfg, out_a, out_b;
+    std::vector<DevBuf *> owned;
+    void *diag = nullptr;  // when set, edge checks record per-sample phase results (diagnostics)
+
+    int fail(int code, const char *fmt, ...) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+};
+
+// Uploaded scene + arm layout.
+struct World {
+    Context *ctx = nullptr;
+    dev::WorldDesc desc{};  // desc.obs points at `buf`
+    dev::WorldDesc *d_desc = nullptr;  // device copy of desc
+    float *buf = nullptr;
+    int n_arms = 0;
+    float bases[geom::kMaxArms][12];
+    int n_obstacles = 0;
+    // algorithmic cost of one coarse env test set: sum of per-obstacle flop weights
+    double f_obstacles = 0.0;
+};
+
+int ensure(Context *c, DevBuf &b, size_t bytes);
+
+// Enqueue the coarse + fine edge-check kernels (check.hip).
+int launch_check(Context *c, const World *w, int mode, int arm, int granularity, int n_edges, const float *d_from,
+                 const float *d_to, uint32_t *d_out_edge, uint64_t *d_out_mask, hipStream_t stream);
+
+// Batched check over edge groups in device memory (check.hip).  Every group
+// of one launch uses the same mode and arm count A; d_wave_start holds
+// n_groups + 1 entries of workspace.
+namespace dev {
+struct EdgeGroup;
+}
+int launch_check_groups(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
+                        int n_groups, unsigned *d_wave_start, hipStream_t stream);
+
+// Sphere centres kernel (check.hip).
+int launch_centres(Context *c, const World *w, int n_cfg, const float *d_cfg, float *d_coarse, float *d_fine,
+                   hipStream_t stream);
+
+}  // namespace mrpop
+
+#define MRPOP_HIP(ctx, call)                                                                               \
+    do {                                                                                                   \
+        hipError_t e_ = (call);                                                                            \
+        if (e_ != hipSuccess) return (ctx)->fail(MRPOP_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)

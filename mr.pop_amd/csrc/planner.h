@@ -1,0 +1,142 @@
+// planner.h -- device-side state of one planning problem ("slot") and the
+// round kernels of the dRRT / AO-x planner (planner.hip).
+//
+// The reference grows its roadmaps and trees with one thread block per
+// expansion stream and inter-block races (src/planning/pop_planner.cu:173-737,
+// roadmap_interior.cuh:94-339).  Here every step is round-synchronous: all
+// expansions of a round read the trees as they stood at the round's start,
+// their edges are checked in one batched launch, and the results are
+// committed in expansion order.  Randomness is a counter-based generator
+// keyed by (problem seed, stream, round, item), so a problem's result does
+// not depend on which other problems share the GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "check_groups.h"
+#include "device_geom.h"
+
+namespace mrpop {
+
+constexpr int kMaxGoals = 8;
+constexpr float kUnwritten = -9999.0f;  // pop_planner_state.cuh:10
+
+// Slot status codes (device int).
+constexpr int kRunning = 0;
+// > 0: solved, value = 1 + winning expansion
+constexpr int kFailedOverflow = -1;  // tree capacity reached (pop_planner.cu:587-589)
+
+struct SlotDev {
+    // problem
+    int A, D, n_goals, world;
+    unsigned long long seed;
+    // settings (pop_settings.hh)
+    int G, B, M;  // granularity, num_new_configs, roadmap samples per arm per round
+    float range, tree_ratio;
+    int balance, phs;
+    // per-pass / per-round state written by the host before each launch
+    int optimize, round, rm_round, pass;
+    int rng_round;            // dRRT rounds since the solve began (random stream position)
+    float phs_center[35];
+    float *sg;                // [1 + n_goals][D]: start, then goals
+    // roadmaps (arm-major)
+    int rm_cap, rm_K, rm_Kc;  // node capacity, neighbour-list capacity, candidates per new node
+    float *rm_q;              // [A][rm_cap][8]
+    int *rm_nbr;              // [A][rm_cap][rm_K]
+    int *rm_deg;              // [A][rm_cap]
+    int *rm_uf;               // [A][rm_cap]   union-find parent
+    int *rm_n;                // [A]
+    int *rm_connected;        // [A]
+    // roadmap round staging
+    float *rs_q;              // [A][M][7]
+    uint32_t *rs_coll;        // [A][M]
+    int *rs_id;               // [A][M]
+    int *re_count;            // [A]
+    float *re_from, *re_to;   // [A][M*Kc][7]
+    int *re_pair;             // [A][M*Kc][2]  (neighbour, new node)
+    uint32_t *re_coll;        // [A][M*Kc]
+    // trees
+    int tr_cap;
+    float *tr_q;              // [2][tr_cap][D]
+    int *tr_parent;           // [2][tr_cap]
+    float *tr_cost;           // [2][tr_cap]
+    int *tr_rid;              // [2][tr_cap][A]  roadmap node of each arm
+    int *tr_n;                // [2]
+    // expansions
+    int *ex_tree;             // [B] tree extended by expansion b (kept across rounds)
+    int *ex_state;            // [B] 0 idle, 1 insert, 2 rewire
+    int *ex_near;             // [B]
+    int *ex_exist;            // [B]
+    float *ex_cost;           // [B]
+    float *ex_q;              // [B][D]
+    int *ex_rid;              // [B][A]
+    int *ex_index;            // [B] node index after insertion (-1 none)
+    unsigned long long *rw_key;  // [tr_cap*2] rewire arbitration (cost bits << 32 | b)
+    // extension edges (DRRT check)
+    int *ed_count;
+    float *ed_from, *ed_to;   // [B][D]
+    int *ed_b;                // [B]
+    uint32_t *ed_coll;        // [B]
+    // connection
+    int *cn_near;             // [B]
+    int *cn_next;             // [B]
+    float *cn_d;              // [B]
+    uint32_t *cn_fail;        // [B]
+    int ce_cap;
+    int *ce_count;
+    float *ce_from, *ce_to;   // [ce_cap][D]
+    int *ce_b;                // [ce_cap]
+    uint32_t *ce_coll;        // [ce_cap]
+    // status and result
+    int *status;              // kRunning, >0 solved, <0 failed
+    float *best_cost;
+    float *path_cost;         // cost of the recorded path
+    int path_cap;
+    float *path;              // [path_cap][D]  goal -> start
+    int *path_n;
+    int *path_meta;           // [0] start-tree nodes, [1] goal-tree nodes, [2] round of solution
+    // PHS transform (row-major D x D)
+    float *phs_tf;
+    // simplification
+    int sp_cap;               // edges
+    int *sp_count;
+    float *sp_from, *sp_to;   // [sp_cap][D]
+    int *sp_item;             // [sp_cap] (candidate index)
+    uint32_t *sp_coll;        // [sp_cap]
+    uint32_t *sp_fail;        // [path_cap * 16]
+    float *sp_mid;            // [path_cap][D]
+    float *sp_cost;           // [path_cap]
+    int *sp_parent;           // [path_cap]
+};
+
+// ----------------------------------------------------------------- RNG ----
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11).
+__host__ __device__ inline uint4 philox4x32_10(uint4 c, uint2 k) {
+    for (int r = 0; r < 10; ++r) {
+        const unsigned long long p0 = (unsigned long long)0xD2511F53u * c.x;
+        const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c.z;
+        const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
+        const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// Uniform in (0, 1] with 24 random bits, as curand_uniform's range.
+__host__ __device__ inline float u01(unsigned x) { return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f); }
+
+// Random streams.
+enum : unsigned { kStreamRoadmap = 1, kStreamTree = 2, kStreamCostBound = 3, kStreamPhsRadius = 4 };
+
+// Four uniforms for (stream, a, b, chunk) of a slot.
+__host__ __device__ inline uint4 slot_random(unsigned long long seed, unsigned stream, unsigned a, unsigned b,
+                                             unsigned chunk) {
+    return philox4x32_10(make_uint4(a, b, chunk, stream), make_uint2((unsigned)seed, (unsigned)(seed >> 32)));
+}
+
+}  // namespace mrpop

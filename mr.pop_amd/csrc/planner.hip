@@ -1,0 +1,714 @@
+// planner.hip -- round kernels of the dRRT / AO-x planner on gfx950.
+//
+// Each kernel handles every active slot (problem) of the engine at once:
+// blockIdx.y indexes the slot list.  The reference semantics followed are
+// cited per kernel; the reference's inter-block races (first CAS wins,
+// unsynchronised tree growth) are replaced by round-synchronous steps whose
+// results are committed in expansion order.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "planner.h"
+
+namespace mrpop {
+namespace plan {
+
+using namespace mrpop::geom;
+
+__device__ __forceinline__ void wave_argmin(float &d, int &i) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float od = __shfl_xor(d, off);
+        const int oi = __shfl_xor(i, off);
+        if (od < d || (od == d && oi < i)) {
+            d = od;
+            i = oi;
+        }
+    }
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ int wave_min_int(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return v;
+}
+
+// Squared distance of a D-vector in global memory to one in LDS (fixed order).
+__device__ __forceinline__ float sqdist(const float *a, const float *b, int D) {
+    float s = 0.f;
+    for (int d = 0; d < D; ++d) {
+        const float x = a[d] - b[d];
+        s = __builtin_fmaf(x, x, s);
+    }
+    return s;
+}
+
+__device__ __forceinline__ float *tree_q(const SlotDev &s, int t, int i) {
+    return s.tr_q + ((size_t)t * s.tr_cap + i) * s.D;
+}
+
+// ============================================================ roadmaps ====
+// Roadmap of each arm: node 0 = the arm's start slice, nodes 1..n_goals its
+// goal slices (roadmap_interior.cuh:69-88).
+__global__ void k_roadmap_init(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const float *sg = s.sg;
+    for (int t = threadIdx.x; t < s.A * (1 + s.n_goals) * 7; t += blockDim.x) {
+        const int a = t / ((1 + s.n_goals) * 7);
+        const int rem = t - a * (1 + s.n_goals) * 7;
+        const int node = rem / 7, j = rem - node * 7;
+        s.rm_q[((size_t)a * s.rm_cap + node) * 8 + j] = sg[(size_t)node * s.D + 7 * a + j];
+    }
+    for (int t = threadIdx.x; t < s.A * (1 + s.n_goals); t += blockDim.x) {
+        const int a = t / (1 + s.n_goals), node = t - a * (1 + s.n_goals);
+        s.rm_uf[(size_t)a * s.rm_cap + node] = node;
+        s.rm_deg[(size_t)a * s.rm_cap + node] = 0;
+    }
+    if (threadIdx.x < s.A) {
+        s.rm_n[threadIdx.x] = 1 + s.n_goals;
+        s.rm_connected[threadIdx.x] = 0;
+        s.re_count[threadIdx.x] = 0;
+    }
+}
+
+// M uniform samples per arm in the joint limits (roadmap_interior.cuh:105-112).
+// They are validated as G = 1 edges q -> q by the edge checker.
+__global__ void k_rm_sample(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= s.A * s.M) return;
+    const int a = t / s.M, i = t - a * s.M;
+    const uint4 r0 = slot_random(s.seed, kStreamRoadmap, (unsigned)(s.rm_round * kMaxArms + a), (unsigned)i, 0);
+    const uint4 r1 = slot_random(s.seed, kStreamRoadmap, (unsigned)(s.rm_round * kMaxArms + a), (unsigned)i, 1);
+    const unsigned bits[7] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
+    float *q = s.rs_q + ((size_t)a * s.M + i) * 7;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) q[j] = __builtin_fmaf(u01(bits[j]), kScale[j], kLow[j]);
+}
+
+// Valid samples become nodes n, n+1, ... in sample order.  One block per (slot, arm).
+__global__ void k_rm_assign(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int a = blockIdx.x;
+    if (a >= s.A) return;
+    __shared__ int sh[256];
+    __shared__ int base;
+    if (threadIdx.x == 0) {
+        base = s.rm_n[a];
+        s.re_count[a] = 0;
+    }
+    __syncthreads();
+    const int per = (s.M + 255) / 256;
+    const int i0 = threadIdx.x * per;
+    int cnt = 0;
+    for (int i = i0; i < min(i0 + per, s.M); ++i) cnt += s.rs_coll[(size_t)a * s.M + i] == 0u;
+    sh[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int id = base + sh[threadIdx.x] - cnt;
+    for (int i = i0; i < min(i0 + per, s.M); ++i) {
+        int out = -1;
+        if (s.rs_coll[(size_t)a * s.M + i] == 0u && id < s.rm_cap) {
+            out = id++;
+            const float *q = s.rs_q + ((size_t)a * s.M + i) * 7;
+            float *dst = s.rm_q + ((size_t)a * s.rm_cap + out) * 8;
+            for (int j = 0; j < 7; ++j) dst[j] = q[j];
+            s.rm_uf[(size_t)a * s.rm_cap + out] = out;
+            s.rm_deg[(size_t)a * s.rm_cap + out] = 0;
+        }
+        s.rs_id[(size_t)a * s.M + i] = out;
+    }
+    __syncthreads();
+    if (threadIdx.x == 255) s.rm_n[a] = min(base + sh[255], s.rm_cap);
+}
+
+// Radius neighbours of each new node among all nodes with a smaller id
+// (existing ones and earlier samples of this round), first rm_Kc in id order
+// (roadmap_interior.cuh:183-197).  Candidate edges go neighbour -> new node
+// (roadmap_interior.cuh:214-234).  One wave per (slot, arm, sample).
+__global__ __launch_bounds__(64) void k_rm_radius(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int t = blockIdx.x;
+    if (t >= s.A * s.M) return;
+    const int a = t / s.M, i = t - a * s.M;
+    const int id = s.rs_id[(size_t)a * s.M + i];
+    if (id < 0) return;
+    const int lane = threadIdx.x;
+    const float *rq = s.rm_q + (size_t)a * s.rm_cap * 8;
+    const float *qn = rq + (size_t)id * 8;
+    float q[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) q[j] = qn[j];
+    const float r2 = s.range * s.range;
+    int found = 0;
+    for (int base = 0; base < id && found < s.rm_Kc; base += 64) {
+        const int n = base + lane;
+        bool near = false;
+        if (n < id) {
+            const float *p = rq + (size_t)n * 8;
+            float d = 0.f;
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const float x = p[j] - q[j];
+                d = __builtin_fmaf(x, x, d);
+            }
+            near = d < r2;  // l2 distance < range
+        }
+        const unsigned long long bal = __ballot(near);
+        if (near) {
+            const int pos = found + __popcll(bal & ((1ull << lane) - 1ull));
+            if (pos < s.rm_Kc) {
+                const int slot = atomicAdd(&s.re_count[a], 1);
+                const size_t e = (size_t)a * s.M * s.rm_Kc + slot;
+                const float *p = rq + (size_t)n * 8;
+                for (int j = 0; j < 7; ++j) {
+                    s.re_from[e * 7 + j] = p[j];
+                    s.re_to[e * 7 + j] = q[j];
+                }
+                s.re_pair[e * 2 + 0] = n;
+                s.re_pair[e * 2 + 1] = id;
+            }
+        }
+        found += __popcll(bal);
+    }
+}
+
+__device__ __forceinline__ int uf_find(int *uf, int x) {
+    while (true) {
+        const int p = uf[x];
+        if (p == x) return x;
+        const int gp = uf[p];
+        if (gp != p) atomicCAS(&uf[x], p, gp);  // path halving
+        x = p;
+    }
+}
+
+// Lock-free union: the larger root is hooked under the smaller one, so each
+// component's root is its smallest node and the partition is deterministic.
+__device__ __forceinline__ void uf_union(int *uf, int a, int b) {
+    while (true) {
+        a = uf_find(uf, a);
+        b = uf_find(uf, b);
+        if (a == b) return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(&uf[a], a, b) == a) return;
+    }
+}
+
+// Collision-free candidate edges join both adjacency lists (the reference's
+// symmetric bitset, roadmap_interior.cuh:245-258) and the union-find.
+__global__ void k_rm_link(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    for (int a = 0; a < s.A; ++a) {
+        const int n = s.re_count[a];
+        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+            const size_t e = (size_t)a * s.M * s.rm_Kc + t;
+            if (s.re_coll[e]) continue;
+            const int u = s.re_pair[e * 2], v = s.re_pair[e * 2 + 1];
+            int *deg = s.rm_deg + (size_t)a * s.rm_cap;
+            int *nbr = s.rm_nbr + (size_t)a * s.rm_cap * s.rm_K;
+            const int pu = atomicAdd(&deg[u], 1);
+            if (pu < s.rm_K) nbr[(size_t)u * s.rm_K + pu] = v;
+            const int pv = atomicAdd(&deg[v], 1);
+            if (pv < s.rm_K) nbr[(size_t)v * s.rm_K + pv] = u;
+            uf_union(s.rm_uf + (size_t)a * s.rm_cap, u, v);
+        }
+    }
+}
+
+// Start and some goal in one component (roadmap_interior.cuh:276-314).
+__global__ void k_rm_connected(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int a = threadIdx.x;
+    if (a >= s.A) return;
+    int *uf = s.rm_uf + (size_t)a * s.rm_cap;
+    const int r0 = uf_find(uf, 0);
+    int c = 0;
+    for (int g = 0; g < s.n_goals; ++g) c |= uf_find(uf, 1 + g) == r0;
+    s.rm_connected[a] = c;
+    // adjacency lists may have overflowed their capacity: clamp the degrees
+    const int n = s.rm_n[a];
+    int *deg = s.rm_deg + (size_t)a * s.rm_cap;
+    for (int i = 0; i < n; ++i)
+        if (deg[i] > s.rm_K) deg[i] = s.rm_K;
+}
+
+// ====================================================== tree search ======
+// Trees reset at the start of every AO-x pass (pop_planner.cu:933-950):
+// tree 0 = the start, tree 1 = the goals, each its own parent with cost 0.
+__global__ void k_tree_reset(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const float *sg = s.sg;
+    for (int t = threadIdx.x; t < (1 + s.n_goals) * s.D; t += blockDim.x) {
+        const int node = t / s.D, d = t - node * s.D;
+        if (node == 0) tree_q(s, 0, 0)[d] = sg[d];
+        else tree_q(s, 1, node - 1)[d] = sg[(size_t)node * s.D + d];
+    }
+    for (int t = threadIdx.x; t < 1 + s.n_goals; t += blockDim.x) {
+        const int tr = t == 0 ? 0 : 1, i = t == 0 ? 0 : t - 1;
+        s.tr_parent[(size_t)tr * s.tr_cap + i] = i;
+        s.tr_cost[(size_t)tr * s.tr_cap + i] = 0.f;
+        for (int a = 0; a < s.A; ++a) s.tr_rid[((size_t)tr * s.tr_cap + i) * s.A + a] = t;  // roadmap node t
+    }
+    for (int b = threadIdx.x; b < s.B; b += blockDim.x) s.ex_tree[b] = b < s.B / 2 ? 0 : 1;
+    if (threadIdx.x == 0) {
+        s.tr_n[0] = 1;
+        s.tr_n[1] = s.n_goals;
+        *s.status = kRunning;
+        *s.ed_count = 0;
+        *s.ce_count = 0;
+    }
+}
+
+// Angle at v between (q - v) and (r - v) (utils.cuh:62-95).
+__device__ __forceinline__ float edge_angle(const float *v, const float *q, const float *r) {
+    float dot = 0.f, vn = 0.f, wn = 0.f;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const float a = q[j] - v[j], b = r[j] - v[j];
+        dot += a * b;
+        vn += a * a;
+        wn += b * b;
+    }
+    vn = sqrtf(vn);
+    wn = sqrtf(wn);
+    if (vn < 1e-8f || wn < 1e-8f) return 0.f;
+    const float c = fminf(1.f, fmaxf(-1.f, dot / (vn * wn)));
+    return acosf(c);
+}
+
+// One dRRT expansion per wave (pop_planner.cu:350-622 up to the edge check):
+// pick the tree, sample (uniform or PHS), nearest tree node, per-arm roadmap
+// neighbour of minimum angle, duplicate test, and the arm-arm edge to check.
+__global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int b = blockIdx.x;
+    if (b >= s.B || *s.status != kRunning) return;
+    const int lane = threadIdx.x;
+    const int D = s.D, A = s.A;
+    __shared__ float q[35];
+    __shared__ float vn[35];
+    __shared__ float nq[35];
+    __shared__ int nrid[kMaxArms];
+    __shared__ float red[2];
+
+    // ---- tree choice (pop_planner.cu:366-376)
+    int t = s.ex_tree[b];
+    if (s.round > 0 && s.balance == 2) {
+        const int nt = s.tr_n[t], no = s.tr_n[1 - t];
+        const float ratio = (float)abs(nt - no) / (float)nt;
+        if (ratio < s.tree_ratio) t = 1 - t;
+    }
+    if (lane == 0) s.ex_tree[b] = t;
+    if (lane == 0) {
+        s.ex_state[b] = 0;
+        s.ex_index[b] = -1;
+    }
+    const int o = 1 - t;
+    const float best = *s.best_cost;
+
+    // ---- sample (pop_planner.cu:389-447)
+    const uint4 rr = slot_random(s.seed, kStreamTree, (unsigned)s.rng_round, (unsigned)b, (unsigned)(lane >> 2));
+    const unsigned bits = (lane & 3) == 0 ? rr.x : (lane & 3) == 1 ? rr.y : (lane & 3) == 2 ? rr.z : rr.w;
+    const float u = u01(bits);  // (0, 1]
+    if (!s.phs || !s.optimize) {
+        if (lane < D) q[lane] = __builtin_fmaf(u, kScale[lane % 7], kLow[lane % 7]);
+    } else {
+        // logistic quantile as an approximate normal, then uniform in the unit ball,
+        // then the prolate hyperspheroid transform (pop_planner.cu:398-442)
+        const float uc = fminf(u, 1.0f - 1.0f / 16777216.0f);  // keep the logit finite
+        float z = lane < D ? logf(uc * __frcp_rn(1.0f - uc)) * sqrtf((float)M_PI / 8.0f) : 0.f;
+        const float nrm = sqrtf(wave_sum(z * z));
+        z = z / nrm;
+        const uint4 rad = slot_random(s.seed, kStreamPhsRadius, (unsigned)s.rng_round, (unsigned)b, 0);
+        const float scale = powf(u01(rad.x), 1.0f / (float)D);
+        if (lane < D) nq[lane] = z * scale;
+        __syncthreads();
+        if (lane < D) {
+            float v = 0.f;
+            for (int c = 0; c < D; ++c) v += s.phs_tf[lane * D + c] * nq[c];
+            v += s.phs_center[lane];
+            const int j = lane % 7;
+            float un = (v - kLow[j]) / kScale[j];
+            un = fmaxf(fminf(1.0f, un), 0.0f);
+            q[lane] = __builtin_fmaf(un, kScale[j], kLow[j]);
+        }
+    }
+    __syncthreads();
+
+    // ---- informed rejection (pop_planner.cu:449-456)
+    {
+        const float *r0 = tree_q(s, 0, 0), *r1 = tree_q(s, 1, 0);
+        float d0 = 0.f, d1 = 0.f;
+        if (lane < D) {
+            const float x0 = r0[lane] - q[lane], x1 = r1[lane] - q[lane];
+            d0 = x0 * x0;
+            d1 = x1 * x1;
+        }
+        const float g0 = sqrtf(wave_sum(d0)), g1 = sqrtf(wave_sum(d1));
+        if (g0 + g1 >= best) return;
+    }
+    // ---- cost bound of the sample (pop_planner.cu:459-469)
+    float c_rand = FLT_MAX;
+    if (s.optimize) {
+        const float *rt = tree_q(s, t, 0), *ro = tree_q(s, o, 0);
+        float dt = 0.f, dd = 0.f;
+        if (lane < D) {
+            const float x = rt[lane] - q[lane], y = ro[lane] - q[lane];
+            dt = x * x;
+            dd = y * y;
+        }
+        const float g_hat = sqrtf(wave_sum(dt)), h_hat = sqrtf(wave_sum(dd));
+        const float c_range = fmaxf(best - (g_hat + h_hat), 0.0f);
+        const uint4 cr = slot_random(s.seed, kStreamCostBound, (unsigned)s.rng_round, (unsigned)b, 0);
+        c_rand = u01(cr.x) * c_range + g_hat;
+    }
+
+    // ---- nearest node of tree t (pop_planner.cu:472-508)
+    const int nt = s.tr_n[t];
+    float bd = FLT_MAX;
+    int bi = 0;
+    for (int i = lane; i < nt; i += 64) {
+        if (s.optimize && !(s.tr_cost[(size_t)t * s.tr_cap + i] < c_rand)) continue;
+        const float d = sqdist(tree_q(s, t, i), q, D);
+        if (d < bd) {
+            bd = d;
+            bi = i;
+        }
+    }
+    wave_argmin(bd, bi);
+    const int near = bi;
+    const float dnear = sqrtf(bd);  // distance from the nearest node to the sample
+    if (lane < D) vn[lane] = tree_q(s, t, near)[lane];
+    __syncthreads();
+
+    // ---- per-arm roadmap step of minimum angle (utils.cuh:688-759)
+    float bound = s.optimize ? c_rand - s.tr_cost[(size_t)t * s.tr_cap + near] : FLT_MAX;
+    for (int a = 0; a < A; ++a) {
+        const int rid = s.tr_rid[((size_t)t * s.tr_cap + near) * A + a];
+        const int deg = s.rm_deg[(size_t)a * s.rm_cap + rid];
+        const int *nb = s.rm_nbr + ((size_t)a * s.rm_cap + rid) * s.rm_K;
+        const float *rq = s.rm_q + (size_t)a * s.rm_cap * 8;
+        float bang = 1e3f;
+        int bidx = 0x7fffffff;
+        for (int j = lane; j < deg; j += 64) {
+            const int r = nb[j];
+            if (r == rid) continue;
+            const float *p = rq + (size_t)r * 8;
+            float d2 = 0.f;
+            for (int k = 0; k < 7; ++k) {
+                const float x = vn[7 * a + k] - p[k];
+                d2 = __builtin_fmaf(x, x, d2);
+            }
+            const float d = sqrtf(d2);
+            if ((s.optimize && d > bound) || d > s.range) continue;
+            const float ang = edge_angle(&vn[7 * a], &q[7 * a], p);
+            if (ang < bang || (ang == bang && r < bidx)) {
+                bang = ang;
+                bidx = r;
+            }
+        }
+        wave_argmin(bang, bidx);
+        if (bidx == 0x7fffffff) return;  // this arm has no admissible neighbour
+        if (lane < 7) nq[7 * a + lane] = rq[(size_t)bidx * 8 + lane];
+        if (lane == 0) nrid[a] = bidx;
+        __syncthreads();
+        float d2 = 0.f;
+        for (int k = 0; k < 7; ++k) {
+            const float x = nq[7 * a + k] - vn[7 * a + k];
+            d2 = __builtin_fmaf(x, x, d2);
+        }
+        bound -= sqrtf(d2);
+    }
+    __syncthreads();
+
+    // ---- existing node within squared distance 1e-5 (pop_planner.cu:526-533)
+    int exist = 0x7fffffff;
+    for (int i = lane; i < nt; i += 64)
+        if (sqdist(tree_q(s, t, i), nq, D) <= 1e-5f) {
+            exist = i;
+            break;
+        }
+    exist = wave_min_int(exist);
+    const float new_cost = s.tr_cost[(size_t)t * s.tr_cap + near] + dnear;  // (pop_planner.cu:541)
+    if (exist != 0x7fffffff) {
+        if (!s.optimize || new_cost >= s.tr_cost[(size_t)t * s.tr_cap + exist]) return;
+    }
+    if (s.optimize && new_cost >= best) return;
+
+    // ---- candidate: edge near -> new for the arm-arm check
+    if (lane == 0) {
+        s.ex_state[b] = exist != 0x7fffffff ? 2 : 1;
+        s.ex_near[b] = near;
+        s.ex_exist[b] = exist;
+        s.ex_cost[b] = new_cost;
+    }
+    if (lane < A) s.ex_rid[b * A + lane] = nrid[lane];
+    if (lane < D) s.ex_q[(size_t)b * D + lane] = nq[lane];
+    int slot = 0;
+    if (lane == 0) slot = atomicAdd(s.ed_count, 1);
+    slot = __shfl(slot, 0);
+    if (lane < D) {
+        s.ed_from[(size_t)slot * D + lane] = vn[lane];
+        s.ed_to[(size_t)slot * D + lane] = nq[lane];
+    }
+    if (lane == 0) s.ed_b[slot] = b;
+}
+
+// Commit the checked expansions in expansion order (pop_planner.cu:571-622):
+// free rewires update an existing node (lowest cost wins, then lowest b),
+// free inserts get consecutive indices per tree.  One block per slot.
+__global__ __launch_bounds__(256) void k_dr_commit(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    if (*s.status != kRunning) return;
+    const int tid = threadIdx.x;
+    const int ne = *s.ed_count;
+    // verdicts back to expansions
+    for (int e = tid; e < ne; e += 256) {
+        const int b = s.ed_b[e];
+        if (s.ed_coll[e]) s.ex_state[b] = 0;
+    }
+    __syncthreads();
+    // rewire arbitration
+    for (int b = tid; b < s.B; b += 256) {
+        if (s.ex_state[b] == 2) {
+            const int t = s.ex_tree[b];
+            const unsigned long long key =
+                ((unsigned long long)__float_as_uint(s.ex_cost[b]) << 32) | (unsigned)b;
+            atomicMin(&s.rw_key[(size_t)t * s.tr_cap + s.ex_exist[b]], key);
+        }
+    }
+    __syncthreads();
+    for (int b = tid; b < s.B; b += 256) {
+        if (s.ex_state[b] == 2) {
+            const int t = s.ex_tree[b], x = s.ex_exist[b];
+            const unsigned long long key = s.rw_key[(size_t)t * s.tr_cap + x];
+            if ((unsigned)key == (unsigned)b) {
+                s.tr_parent[(size_t)t * s.tr_cap + x] = s.ex_near[b];
+                s.tr_cost[(size_t)t * s.tr_cap + x] = s.ex_cost[b];
+            }
+        }
+    }
+    __syncthreads();
+    for (int b = tid; b < s.B; b += 256) {
+        if (s.ex_state[b] == 2) s.rw_key[(size_t)s.ex_tree[b] * s.tr_cap + s.ex_exist[b]] = ~0ull;
+    }
+    // insert: prefix over b per tree
+    __shared__ int sh0[256], sh1[256];
+    const int per = (s.B + 255) / 256;
+    const int b0 = tid * per;
+    int c0 = 0, c1 = 0;
+    for (int b = b0; b < min(b0 + per, s.B); ++b)
+        if (s.ex_state[b] == 1) (s.ex_tree[b] == 0 ? c0 : c1)++;
+    sh0[tid] = c0;
+    sh1[tid] = c1;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int a0 = tid >= off ? sh0[tid - off] : 0, a1 = tid >= off ? sh1[tid - off] : 0;
+        __syncthreads();
+        sh0[tid] += a0;
+        sh1[tid] += a1;
+        __syncthreads();
+    }
+    const int n0 = s.tr_n[0], n1 = s.tr_n[1];
+    int i0 = n0 + sh0[tid] - c0, i1 = n1 + sh1[tid] - c1;
+    bool overflow = false;
+    for (int b = b0; b < min(b0 + per, s.B); ++b) {
+        if (s.ex_state[b] != 1) continue;
+        const int t = s.ex_tree[b];
+        const int idx = t == 0 ? i0++ : i1++;
+        if (idx >= s.tr_cap) {
+            overflow = true;
+            s.ex_state[b] = 0;
+            continue;
+        }
+        s.ex_index[b] = idx;
+        float *dst = tree_q(s, t, idx);
+        const float *src = s.ex_q + (size_t)b * s.D;
+        for (int d = 0; d < s.D; ++d) dst[d] = src[d];
+        for (int a = 0; a < s.A; ++a) s.tr_rid[((size_t)t * s.tr_cap + idx) * s.A + a] = s.ex_rid[b * s.A + a];
+        s.tr_parent[(size_t)t * s.tr_cap + idx] = s.ex_near[b];
+        s.tr_cost[(size_t)t * s.tr_cap + idx] = s.ex_cost[b];
+    }
+    if (overflow) atomicExch(s.status, kFailedOverflow);  // (pop_planner.cu:587-589)
+    __syncthreads();
+    if (tid == 255) {
+        s.tr_n[0] = min(n0 + sh0[255], s.tr_cap);
+        s.tr_n[1] = min(n1 + sh1[255], s.tr_cap);
+        *s.ed_count = 0;
+        *s.ce_count = 0;
+    }
+}
+
+// Nearest node of the other tree for every inserted node, and the straight
+// connection split into n_ext = floor(d / range) + 1 chunks of G samples
+// (pop_planner.cu:624-703).  One wave per expansion.
+__global__ __launch_bounds__(64) void k_dr_connect(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int b = blockIdx.x;
+    if (b >= s.B || *s.status != kRunning) return;
+    const int lane = threadIdx.x;
+    if (lane == 0) s.cn_near[b] = -1;
+    const int idx = s.ex_index[b];
+    if (s.ex_state[b] != 1 || idx < 0) return;
+    const int t = s.ex_tree[b], o = 1 - t, D = s.D;
+    __shared__ float q[35];
+    if (lane < D) q[lane] = tree_q(s, t, idx)[lane];
+    __syncthreads();
+    const float best = *s.best_cost;
+    const float ub = best - s.tr_cost[(size_t)t * s.tr_cap + idx];
+    const int no = s.tr_n[o];
+    float bd = FLT_MAX;
+    int bi = 0x7fffffff;
+    for (int i = lane; i < no; i += 64) {
+        const float d = sqdist(tree_q(s, o, i), q, D);
+        if (d < bd && (!s.optimize || sqrtf(d) < ub)) {
+            bd = d;
+            bi = i;
+        }
+    }
+    wave_argmin(bd, bi);
+    if (bi == 0x7fffffff) return;
+    const float d = sqrtf(bd);
+    const int next = (int)(d / s.range) + 1;
+    if (next > s.ce_cap / s.B) return;  // longer than the chunk budget per expansion: no attempt
+    int base = 0;
+    if (lane == 0) base = atomicAdd(s.ce_count, next);
+    base = __shfl(base, 0);
+    if (lane == 0) {
+        s.cn_near[b] = bi;
+        s.cn_next[b] = next;
+        s.cn_d[b] = d;
+        s.cn_fail[b] = 0;
+    }
+    if (lane < D) {
+        const float *on = tree_q(s, o, bi);
+        const float vec = (on[lane] - q[lane]) / (float)next;
+        float cur = q[lane];
+        for (int c = 0; c < next; ++c) {
+            const size_t e = (size_t)(base + c);
+            s.ce_from[e * D + lane] = cur;
+            cur += vec;  // the reference advances the configuration chunk by chunk
+            s.ce_to[e * D + lane] = cur;
+        }
+    }
+    for (int c = lane; c < next; c += 64) s.ce_b[base + c] = b;
+}
+
+// Connections whose chunks are all free add their end node to tree t; the
+// lowest expansion among them records the path (pop_planner.cu:705-731,
+// path_processing.cu:10-70).  One block per slot.
+__global__ __launch_bounds__(256) void k_dr_finish(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    if (*s.status != kRunning) return;
+    const int tid = threadIdx.x;
+    const int nce = *s.ce_count;
+    for (int e = tid; e < nce; e += 256)
+        if (s.ce_coll[e]) s.cn_fail[s.ce_b[e]] = 1;
+    __syncthreads();
+    if (tid != 0) return;
+    int winner = -1, wnode = -1;
+    for (int b = 0; b < s.B; ++b) {
+        if (s.cn_near[b] < 0 || s.cn_fail[b]) continue;
+        const int t = s.ex_tree[b], o = 1 - t;
+        const int idx = s.tr_n[t];
+        if (idx >= s.tr_cap) {
+            *s.status = kFailedOverflow;
+            return;
+        }
+        s.tr_n[t] = idx + 1;
+        // end configuration: the start advanced by n_ext chunks, as in the chunk edges
+        const float *st = tree_q(s, t, s.ex_index[b]), *on = tree_q(s, o, s.cn_near[b]);
+        float *dst = tree_q(s, t, idx);
+        const int next = s.cn_next[b];
+        for (int d = 0; d < s.D; ++d) {
+            const float vec = (on[d] - st[d]) / (float)next;
+            float cur = st[d];
+            for (int c = 0; c < next; ++c) cur += vec;
+            dst[d] = cur;
+        }
+        for (int a = 0; a < s.A; ++a)
+            s.tr_rid[((size_t)t * s.tr_cap + idx) * s.A + a] = s.tr_rid[((size_t)o * s.tr_cap + s.cn_near[b]) * s.A + a];
+        s.tr_parent[(size_t)t * s.tr_cap + idx] = s.ex_index[b];
+        // the reference adds one chunk's length here (pop_planner.cu:709-710)
+        s.tr_cost[(size_t)t * s.tr_cap + idx] = s.tr_cost[(size_t)t * s.tr_cap + s.ex_index[b]] + s.cn_d[b] / next;
+        if (winner < 0) {
+            winner = b;
+            wnode = idx;
+        }
+    }
+    if (winner < 0) return;
+    // ---- trace both trees to their roots; path runs goal -> start
+    const int t = s.ex_tree[winner], o = 1 - t, D = s.D;
+    int chain_n[2] = {0, 0};
+    // count chains
+    const int start_node[2] = {t == 0 ? wnode : s.cn_near[winner], t == 1 ? wnode : s.cn_near[winner]};
+    for (int tr = 0; tr < 2; ++tr) {
+        int cur = start_node[tr], n = 1;
+        while (s.tr_parent[(size_t)tr * s.tr_cap + cur] != cur) {
+            cur = s.tr_parent[(size_t)tr * s.tr_cap + cur];
+            ++n;
+        }
+        chain_n[tr] = n;
+    }
+    const int total = chain_n[0] + chain_n[1];
+    if (total > s.path_cap) {
+        *s.status = kFailedOverflow;
+        return;
+    }
+    float cost = 0.f;
+    // tree 1 reversed first: its root (node 0 of tree 1) ... its chain start
+    {
+        int cur = start_node[1], k = chain_n[1] - 1;
+        while (true) {
+            const int par = s.tr_parent[(size_t)s.tr_cap + cur];
+            const int node = par == cur ? 0 : cur;  // the root written is node 0 of the tree
+            const float *src = tree_q(s, 1, node);
+            for (int d = 0; d < D; ++d) s.path[(size_t)k * D + d] = src[d];
+            if (par == cur) break;
+            cost += sqrtf(sqdist(tree_q(s, 1, cur), tree_q(s, 1, par), D));
+            cur = par;
+            --k;
+        }
+    }
+    {
+        int cur = start_node[0], k = chain_n[1];
+        while (true) {
+            const int par = s.tr_parent[cur];
+            const int node = par == cur ? 0 : cur;
+            const float *src = tree_q(s, 0, node);
+            for (int d = 0; d < D; ++d) s.path[(size_t)k * D + d] = src[d];
+            if (par == cur) break;
+            cost += sqrtf(sqdist(tree_q(s, 0, cur), tree_q(s, 0, par), D));
+            cur = par;
+            ++k;
+        }
+    }
+    *s.path_n = total;
+    *s.path_cost = cost;
+    s.path_meta[0] = s.tr_n[0];
+    s.path_meta[1] = s.tr_n[1];
+    s.path_meta[2] = s.round + 1;
+    *s.best_cost = fminf(*s.best_cost, cost);
+    *s.status = winner + 1;
+}
+
+}  // namespace plan
+}  // namespace mrpop

@@ -1,0 +1,390 @@
+"""Scenes and planning problems: the reference's problem-file schema and the
+synthetic scene generators used for benchmarking.
+
+Scene construction follows the reference's factory and harness:
+  * Euler XYZ orientation -> R = Rz(phi) * Ry(theta) * Rx(rho) and cuboid axes
+    = columns of R (src/collision/factory.hh:26-41);
+  * cylinder/capsule from centre + Euler + length: endpoints at +-length/2 on
+    the local z axis, rdv = 1 / |v|^2 (factory.hh:100-114, 144-153);
+  * problem-file mapping: "sphere" -> sphere, "cylinder" -> capsule (the
+    reference checks cylinders as capsules), "box" -> cuboid
+    (scripts/evaluate_mr.cu:14-79).  The file's base poses are not used by the
+    reference (its base transforms are compiled in); here they are kept and
+    compared against the compiled-in defaults by the tests.
+
+Records are float32, laid out as the C ABI's mrpop_scene expects
+(include/mrpop_c.h).
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+DOF_PER_ARM = 7
+# Joint limits (src/planning/Robots.hh:13-23): q = u * SCALE + LOW.
+SCALE = np.array([5.9342, 3.6652, 5.9342, 3.2289, 5.9342, 3.9095999999999997, 5.9342], np.float32)
+LOW = np.array([-2.9671, -1.8326, -2.9671, -3.1416, -2.9671, -0.0873, -2.9671], np.float32)
+
+
+def euler_xyz_matrix(rho: float, theta: float, phi: float) -> np.ndarray:
+    """R = Rz(phi) * Ry(theta) * Rx(rho), in float32 (factory.hh:29-31)."""
+    cr, sr = np.float32(math.cos(rho)), np.float32(math.sin(rho))
+    ct, st = np.float32(math.cos(theta)), np.float32(math.sin(theta))
+    cp, sp = np.float32(math.cos(phi)), np.float32(math.sin(phi))
+    rx = np.array([[1, 0, 0], [0, cr, -sr], [0, sr, cr]], np.float32)
+    ry = np.array([[ct, 0, st], [0, 1, 0], [-st, 0, ct]], np.float32)
+    rz = np.array([[cp, -sp, 0], [sp, cp, 0], [0, 0, 1]], np.float32)
+    return (rz @ ry @ rx).astype(np.float32)
+
+
+def cuboid_record(center, euler_xyz, half_extents) -> np.ndarray:
+    r = euler_xyz_matrix(*euler_xyz)
+    rec = np.zeros(15, np.float32)
+    rec[0:3] = center
+    rec[3:6] = r[:, 0]
+    rec[6:9] = r[:, 1]
+    rec[9:12] = r[:, 2]
+    rec[12:15] = half_extents
+    return rec
+
+
+def capsule_record_endpoints(p1, p2, radius) -> np.ndarray:
+    p1 = np.asarray(p1, np.float32)
+    p2 = np.asarray(p2, np.float32)
+    v = (p2 - p1).astype(np.float32)
+    dot = np.float32(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+    rec = np.zeros(8, np.float32)
+    rec[0:3] = p1
+    rec[3:6] = v
+    rec[6] = radius
+    rec[7] = np.float32(1.0 / float(dot))
+    return rec
+
+
+def capsule_record(center, euler_xyz, radius, length) -> np.ndarray:
+    r = euler_xyz_matrix(*euler_xyz)
+    c = np.asarray(center, np.float32)
+    half = np.float32(length) / np.float32(2)
+    p1 = r @ np.array([0, 0, half], np.float32) + c
+    p2 = r @ np.array([0, 0, -half], np.float32) + c
+    return capsule_record_endpoints(p1, p2, radius)
+
+
+def sphere_record(center, radius) -> np.ndarray:
+    return np.array([center[0], center[1], center[2], radius], np.float32)
+
+
+class _CScene(ctypes.Structure):
+    _fields_ = [("n_spheres", ctypes.c_int), ("n_capsules", ctypes.c_int), ("n_z_capsules", ctypes.c_int),
+                ("n_cuboids", ctypes.c_int), ("n_z_cuboids", ctypes.c_int),
+                ("spheres", ctypes.c_void_p), ("capsules", ctypes.c_void_p), ("z_capsules", ctypes.c_void_p),
+                ("cuboids", ctypes.c_void_p), ("z_cuboids", ctypes.c_void_p)]
+
+
+@dataclass
+class Scene:
+    """Obstacle lists as float32 record arrays (see include/mrpop_c.h)."""
+    spheres: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.float32))
+    capsules: np.ndarray = field(default_factory=lambda: np.zeros((0, 8), np.float32))
+    z_capsules: np.ndarray = field(default_factory=lambda: np.zeros((0, 8), np.float32))
+    cuboids: np.ndarray = field(default_factory=lambda: np.zeros((0, 15), np.float32))
+    z_cuboids: np.ndarray = field(default_factory=lambda: np.zeros((0, 15), np.float32))
+
+    def __post_init__(self):
+        for name, w in (("spheres", 4), ("capsules", 8), ("z_capsules", 8), ("cuboids", 15), ("z_cuboids", 15)):
+            a = np.ascontiguousarray(np.asarray(getattr(self, name), np.float32).reshape(-1, w))
+            setattr(self, name, a)
+
+    @property
+    def n_obstacles(self) -> int:
+        return sum(len(a) for a in (self.spheres, self.capsules, self.z_capsules, self.cuboids, self.z_cuboids))
+
+    def as_c(self) -> _CScene:
+        """ctypes mrpop_scene viewing this scene's arrays (keep the Scene alive)."""
+        def p(a):
+            return a.ctypes.data if len(a) else None
+        return _CScene(len(self.spheres), len(self.capsules), len(self.z_capsules), len(self.cuboids),
+                       len(self.z_cuboids), p(self.spheres), p(self.capsules), p(self.z_capsules),
+                       p(self.cuboids), p(self.z_cuboids))
+
+    def inflated(self, margin: float) -> "Scene":
+        """Every obstacle grown by `margin` metres (radius or half-extents)."""
+        s = Scene(self.spheres.copy(), self.capsules.copy(), self.z_capsules.copy(), self.cuboids.copy(),
+                  self.z_cuboids.copy())
+        s.spheres[:, 3] += margin
+        s.capsules[:, 6] += margin
+        s.z_capsules[:, 6] += margin
+        s.cuboids[:, 12:15] += margin
+        s.z_cuboids[:, 12:15] += margin
+        return s
+
+    def transformed(self, rot: np.ndarray, trans: np.ndarray) -> "Scene":
+        """The scene moved by the rigid motion x -> rot @ x + trans (float32)."""
+        rot = np.asarray(rot, np.float32)
+        trans = np.asarray(trans, np.float32)
+        sp = self.spheres.copy()
+        sp[:, 0:3] = sp[:, 0:3] @ rot.T + trans
+        cap = self.capsules.copy()
+        cap[:, 0:3] = cap[:, 0:3] @ rot.T + trans
+        cap[:, 3:6] = cap[:, 3:6] @ rot.T
+        cub = self.cuboids.copy()
+        cub[:, 0:3] = cub[:, 0:3] @ rot.T + trans
+        for k in range(3):
+            cub[:, 3 + 3 * k:6 + 3 * k] = cub[:, 3 + 3 * k:6 + 3 * k] @ rot.T
+        if len(self.z_capsules) or len(self.z_cuboids):
+            raise ValueError("z-aligned obstacles are not closed under general rotations")
+        return Scene(sp, cap, self.z_capsules, cub, self.z_cuboids)
+
+
+@dataclass
+class Problem:
+    """One planning query: arm layout, scene, start and goal configurations."""
+    name: str
+    n_arms: int
+    bases: np.ndarray            # (n_arms, 12) world <- base, row-major 3x4
+    scene: Scene
+    start: np.ndarray            # (7 * n_arms,)
+    goals: np.ndarray            # (n_goals, 7 * n_arms)
+    seed: int = 0
+
+    @property
+    def dim(self) -> int:
+        return DOF_PER_ARM * self.n_arms
+
+
+# ------------------------------------------------------------------ bases --
+def yaw_pose(x, y, z, yaw) -> np.ndarray:
+    c, s = math.cos(yaw), math.sin(yaw)
+    return np.array([c, -s, 0, x, s, c, 0, y, 0, 0, 1, z], np.float32)
+
+
+def quat_wxyz_pose(pos, q) -> np.ndarray:
+    w, x, y, z = q
+    r = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    m = np.zeros(12, np.float32)
+    m[[0, 1, 2, 4, 5, 6, 8, 9, 10]] = r.reshape(-1)
+    m[[3, 7, 11]] = pos
+    return m
+
+
+# ------------------------------------------------------- problem files ----
+def scene_from_problem_dict(problem: dict) -> Scene:
+    """scripts/evaluate_mr.cu:14-79: spheres, cylinders as capsules, boxes as cuboids."""
+    spheres = [sphere_record(o["position"], o["radius"]) for o in problem.get("sphere", [])]
+    if problem.get("name") == "box":
+        # evaluate_mr.cu:28-37 -- only for a problem literally named "box"
+        cub_cyl = [cuboid_record(o["position"], o["orientation_euler_xyz"],
+                                 [o["radius"], o["radius"], o["radius"] / 2.0]) for o in problem.get("cylinder", [])]
+        capsules = []
+    else:
+        cub_cyl = []
+        capsules = [capsule_record(o["position"], o["orientation_euler_xyz"], o["radius"], o["length"])
+                    for o in problem.get("cylinder", [])]
+    cuboids = cub_cyl + [cuboid_record(o["position"], o["orientation_euler_xyz"], o["half_extents"])
+                         for o in problem.get("box", [])]
+    return Scene(np.array(spheres, np.float32).reshape(-1, 4), np.array(capsules, np.float32).reshape(-1, 8),
+                 np.zeros((0, 8), np.float32), np.array(cuboids, np.float32).reshape(-1, 15))
+
+
+def load_problem_file(path, default_bases: Optional[np.ndarray] = None) -> List[Problem]:
+    """Read a reference problem file (schema of scripts/panda_*_problems.json).
+
+    Only problems with "valid": true are returned, as the reference's driver
+    skips the others (evaluate_mr.cu:121-124).  Bases: `default_bases` when
+    given (the reference ignores the file's poses), else the file's poses.
+    """
+    data = json.loads(Path(path).read_text())
+    out = []
+    for p in data:
+        if not p.get("valid", True):
+            continue
+        n_arms = len(p["start"]) // DOF_PER_ARM
+        if default_bases is not None:
+            bases = np.asarray(default_bases, np.float32).reshape(n_arms, 12)
+        else:
+            bases = np.stack([quat_wxyz_pose(pos, q) for pos, q in zip(p["base_positions"], p["base_orientation_wxyz"])])
+        out.append(Problem(p["name"], n_arms, bases, scene_from_problem_dict(p),
+                           np.asarray(p["start"], np.float32), np.asarray(p["goals"], np.float32).reshape(len(p["goals"]), -1),
+                           int(p.get("source_index", 0))))
+    return out
+
+
+def problem_to_dict(p: Problem) -> dict:
+    """Inverse of load_problem_file for generated problems (reference schema)."""
+    def euler_from_axes(rec):
+        r = np.stack([rec[3:6], rec[6:9], rec[9:12]], axis=1).astype(np.float64)
+        theta = -math.asin(max(-1.0, min(1.0, r[2, 0])))
+        rho = math.atan2(r[2, 1], r[2, 2])
+        phi = math.atan2(r[1, 0], r[0, 0])
+        return [rho, theta, phi]
+    boxes = [{"name": f"box{i}", "position": c[0:3].tolist(), "orientation_euler_xyz": euler_from_axes(c),
+              "half_extents": c[12:15].tolist()} for i, c in enumerate(p.scene.cuboids)]
+    cyl = []
+    for i, c in enumerate(p.scene.capsules):
+        p1, v = c[0:3].astype(np.float64), c[3:6].astype(np.float64)
+        length = float(np.linalg.norm(v))
+        d = -v / length   # local +z points from p2 to p1
+        theta = math.asin(max(-1.0, min(1.0, d[0])))
+        rho = math.atan2(-d[1], d[2])
+        cyl.append({"name": f"cylinder{i}", "position": (p1 + v / 2).tolist(),
+                    "orientation_euler_xyz": [rho, theta, 0.0], "radius": float(c[6]), "length": length})
+    sph = [{"name": f"sphere{i}", "position": s[0:3].tolist(), "radius": float(s[3])} for i, s in enumerate(p.scene.spheres)]
+    pos = [b[[3, 7, 11]].tolist() for b in p.bases]
+    return {"name": p.name, "valid": True, "source_index": p.seed, "base_positions": pos,
+            "base_orientation_wxyz": [[1.0, 0.0, 0.0, 0.0]] * p.n_arms, "start": p.start.tolist(),
+            "goals": p.goals.tolist(), "sphere": sph, "cylinder": cyl, "box": boxes}
+
+
+# ------------------------------------------------------ scene generators ----
+def default_bases(n_arms: int) -> np.ndarray:
+    """Default base layout per arm count (4 and 5: the reference's tables)."""
+    from . import geometry
+    return geometry.default_bases(n_arms)
+
+
+def _far_from_bases(xy, bases, clearance):
+    return all(math.hypot(xy[0] - b[3], xy[1] - b[7]) > clearance for b in bases)
+
+
+def scene_random_cuboids(rng: np.random.Generator, bases, n=16, half=(0.04, 0.12), lo=(-0.6, -0.6, 0.0),
+                         hi=(0.6, 0.6, 1.2), base_clearance=0.3) -> Scene:
+    """n axis-aligned cuboids, half-extents U[half]^3, centres U[lo, hi], kept
+    `base_clearance` metres (in xy) away from every arm base."""
+    cub = []
+    while len(cub) < n:
+        c = rng.uniform(lo, hi)
+        if not _far_from_bases(c, bases, base_clearance):
+            continue
+        cub.append(cuboid_record(c, (0.0, 0.0, 0.0), rng.uniform(half[0], half[1], 3)))
+    return Scene(cuboids=np.array(cub, np.float32))
+
+
+def _random_obstacles(rng, n, bases, lo, hi, size, base_clearance, avoid=None):
+    sph, cap, cub = [], [], []
+    while len(sph) + len(cap) + len(cub) < n:
+        c = rng.uniform(lo, hi)
+        if not _far_from_bases(c, bases, base_clearance):
+            continue
+        if avoid is not None and avoid(c):
+            continue
+        kind = rng.integers(3)
+        s = rng.uniform(size[0], size[1])
+        if kind == 0:
+            sph.append(sphere_record(c, s))
+        elif kind == 1:
+            cub.append(cuboid_record(c, (0.0, 0.0, rng.uniform(-math.pi, math.pi)), rng.uniform(size[0], size[1], 3)))
+        else:
+            length = rng.uniform(2 * size[0], 2 * size[1])
+            cap.append(capsule_record(c, (rng.uniform(-math.pi, math.pi), rng.uniform(-math.pi / 2, math.pi / 2), 0.0),
+                                      s, length))
+    return sph, cap, cub
+
+
+def bin_bases() -> np.ndarray:
+    """4 arms at the corners (+-0.6, +-0.6, 0) of a 1.2 m table, facing the centre."""
+    out = []
+    for x, y in [(0.6, -0.6), (-0.6, -0.6), (-0.6, 0.6), (0.6, 0.6)]:
+        out.append(yaw_pose(x, y, 0.0, math.atan2(-y, -x)))
+    return np.stack(out)
+
+
+def scene_bin(rng: np.random.Generator, bases, n_random=24) -> Scene:
+    """Bin of 5 cuboids (1 cm walls, 40x40x20 cm) on a 1.2 m table, plus
+    n_random spheres / cuboids / cylinders (capsules) of size 3-10 cm."""
+    cub = [cuboid_record((0, 0, -0.01), (0, 0, 0), (0.6, 0.6, 0.01)),          # table top
+           cuboid_record((0, 0, 0.005), (0, 0, 0), (0.2, 0.2, 0.005)),         # bin floor
+           cuboid_record((0.195, 0, 0.1), (0, 0, 0), (0.005, 0.2, 0.1)),
+           cuboid_record((-0.195, 0, 0.1), (0, 0, 0), (0.005, 0.2, 0.1)),
+           cuboid_record((0, 0.195, 0.1), (0, 0, 0), (0.2, 0.005, 0.1)),
+           cuboid_record((0, -0.195, 0.1), (0, 0, 0), (0.2, 0.005, 0.1))]
+    def in_bin(c):
+        return abs(c[0]) < 0.32 and abs(c[1]) < 0.32 and c[2] < 0.35
+    sph, cap, cub2 = _random_obstacles(rng, n_random, bases, (-0.75, -0.75, 0.1), (0.75, 0.75, 1.0), (0.03, 0.10),
+                                       0.35, in_bin)
+    return Scene(np.array(sph, np.float32).reshape(-1, 4), np.array(cap, np.float32).reshape(-1, 8),
+                 np.zeros((0, 8), np.float32), np.array(cub + cub2, np.float32).reshape(-1, 15))
+
+
+def scene_shelf(rng: np.random.Generator, n_cylinders=15) -> Scene:
+    """Shelf between the two rows of the 5-arm layout: 4 boards of 2 cm
+    (1.0 m x 0.35 m) plus two side walls, with n_cylinders capsules
+    (r 2-5 cm, h 8-20 cm) standing on the boards."""
+    cx, cy = 0.3, 0.0
+    levels = [0.25, 0.5, 0.75, 1.0]
+    cub = [cuboid_record((cx, cy, z), (0, 0, 0), (0.5, 0.175, 0.01)) for z in levels]
+    cub.append(cuboid_record((cx - 0.51, cy, 0.55), (0, 0, 0), (0.01, 0.175, 0.45)))
+    cub.append(cuboid_record((cx + 0.51, cy, 0.55), (0, 0, 0), (0.01, 0.175, 0.45)))
+    cap = []
+    for _ in range(n_cylinders):
+        z = levels[rng.integers(len(levels) - 1)] + 0.01
+        r = rng.uniform(0.02, 0.05)
+        h = rng.uniform(0.08, 0.20)
+        x = rng.uniform(cx - 0.45, cx + 0.45)
+        y = rng.uniform(cy - 0.13, cy + 0.13)
+        cap.append(capsule_record((x, y, z + h / 2), (0.0, 0.0, 0.0), r, h))
+    return Scene(capsules=np.array(cap, np.float32), cuboids=np.array(cub, np.float32))
+
+
+def scene_table3(rng: np.random.Generator, bases, n_random=12) -> Scene:
+    """Round 3-arm work cell: a table top plus n_random mixed obstacles."""
+    cub = [cuboid_record((0, 0, -0.01), (0, 0, 0), (0.8, 0.8, 0.01))]
+    sph, cap, cub2 = _random_obstacles(rng, n_random, bases, (-0.6, -0.6, 0.1), (0.6, 0.6, 1.0), (0.03, 0.10), 0.35)
+    return Scene(np.array(sph, np.float32).reshape(-1, 4), np.array(cap, np.float32).reshape(-1, 8),
+                 np.zeros((0, 8), np.float32), np.array(cub + cub2, np.float32).reshape(-1, 15))
+
+
+Checker = Callable[[Scene, int, np.ndarray, np.ndarray], np.ndarray]
+"""checker(scene, n_arms, bases, cfgs[n, 7*n_arms]) -> bool[n] (True = colliding),
+for single configurations under the full multi-arm composite check."""
+
+
+def sample_free_configs(rng: np.random.Generator, scene: Scene, n_arms: int, bases: np.ndarray, count: int,
+                        checker: Checker, batch: int = 4096, max_rounds: int = 400) -> np.ndarray:
+    """`count` collision-free configurations by rejection sampling, uniform in
+    the joint limits."""
+    dim = DOF_PER_ARM * n_arms
+    found = []
+    for _ in range(max_rounds):
+        u = rng.uniform(0.0, 1.0, (batch, dim)).astype(np.float32)
+        u = np.maximum(u, np.float32(1e-7))
+        q = u * np.tile(SCALE, n_arms) + np.tile(LOW, n_arms)
+        ok = ~checker(scene, n_arms, bases, q)
+        found.extend(q[ok])
+        if len(found) >= count:
+            return np.stack(found[:count]).astype(np.float32)
+    raise RuntimeError(f"could not sample {count} free configurations ({len(found)} found)")
+
+
+def make_problem(kind: str, seed: int, checker: Checker, n_goals: int = 1) -> Problem:
+    """Seeded synthetic problem.  kind: 'plain2' (2 arms, 16 random cuboids),
+    'table3' (3 arms), 'bin4' (4 arms around a bin), 'shelf5' (5 arms, shelf)."""
+    rng = np.random.default_rng([seed, {"plain2": 2, "table3": 3, "bin4": 4, "shelf5": 5}[kind]])
+    if kind == "plain2":
+        n_arms, bases = 2, default_bases(2)
+        scene = scene_random_cuboids(rng, bases)
+    elif kind == "table3":
+        n_arms, bases = 3, default_bases(3)
+        scene = scene_table3(rng, bases)
+    elif kind == "bin4":
+        n_arms, bases = 4, bin_bases()
+        scene = scene_bin(rng, bases)
+    elif kind == "shelf5":
+        n_arms, bases = 5, default_bases(5)
+        scene = scene_shelf(rng)
+    else:
+        raise ValueError(kind)
+    cfgs = sample_free_configs(rng, scene, n_arms, bases, 1 + n_goals, checker)
+    return Problem(f"{kind}_{seed}", n_arms, bases, scene, cfgs[0], cfgs[1:], seed)
+
+
+def scene_records_equal(a: Scene, b: Scene) -> bool:
+    return all(np.array_equal(getattr(a, k), getattr(b, k))
+               for k in ("spheres", "capsules", "z_capsules", "cuboids", "z_cuboids"))
