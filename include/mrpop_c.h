@@ -166,6 +166,96 @@ int mrpop_check_edges_device(mrpop_ctx *ctx, const mrpop_world *w, int mode, int
 int mrpop_sphere_centres(mrpop_ctx *ctx, const mrpop_world *w, int n_cfg, const float *cfgs,
                          float *out_coarse, float *out_fine);
 
+/* ---- planning ----------------------------------------------------------- */
+/* One planning query (the inputs of dRRT::solve, Planners.hh:57-64). */
+typedef struct mrpop_problem {
+    int n_arms;                  /* 2..5 */
+    const float *bases;          /* n_arms*12 world<-base (row-major 3x4); NULL = mrpop_default_bases */
+    const mrpop_scene *scene;
+    const float *start;          /* 7*n_arms */
+    int n_goals;                 /* 1..8 */
+    const float *goals;          /* n_goals * 7*n_arms */
+    uint64_t seed;               /* random stream of this problem */
+} mrpop_problem;
+
+/* Resources of one problem.  block_per_robot is dRRT::solve's argument
+ * (roadmap samples per arm per round = 230 * block_per_robot, as
+ * ROADMAP_BUILD_ITER * block_per_robot in roadmap_interior.cuh:20, 97). */
+typedef struct mrpop_solve_options {
+    int block_per_robot;       /* default 8 */
+    double time_budget_s;      /* wall-clock budget per problem; <= 0: none (default 30) */
+    int roadmap_capacity;      /* nodes per arm roadmap (default 100000 = MAX_ROADMAP_SIZE) */
+    int tree_capacity;         /* nodes per tree (0: min(max_samples, 262144)) */
+    int neighbour_capacity;    /* adjacency entries per roadmap node (default 128) */
+    int neighbour_candidates;  /* radius neighbours tried per new roadmap node, 1..64 (default 64) */
+} mrpop_solve_options;
+
+int mrpop_solve_options_default(mrpop_solve_options *o);
+
+/* Result fields of PlannerResult (Planners.hh:14-32) plus timing of the
+ * first solution and the status of the solve. */
+typedef struct mrpop_result_info {
+    int solved;
+    float cost;
+    int path_length;           /* waypoints of the final path */
+    int start_tree_size;
+    int goal_tree_size;
+    int iters;                 /* dRRT round of the final solution */
+    uint64_t wall_ns;          /* submit -> done */
+    uint64_t kernel_ns;        /* first kernel -> done */
+    uint64_t copy_ns;          /* result copies */
+    uint64_t ttfs_ns;          /* submit -> first solution (0 if none) */
+    int n_seed_paths;          /* improving solutions recorded (PlannerResult::seed_paths) */
+    int passes;                /* AO-x passes run */
+    int rounds;                /* dRRT rounds run */
+    int roadmap_rounds;
+    int roadmap_size[MRPOP_MAX_ARMS];
+    int status;                /* MRPOP_OK, or MRPOP_ERR_TIMEOUT if the time budget ended it */
+} mrpop_result_info;
+
+typedef struct mrpop_solution mrpop_solution;
+
+/* dRRT + AO-x (pop_planner.cu:739-1030) on one problem.  settings NULL =
+ * pop_settings defaults, options NULL = mrpop_solve_options_default. */
+int mrpop_solve(mrpop_ctx *ctx, const mrpop_problem *problem, const mrpop_settings *settings,
+                const mrpop_solve_options *options, mrpop_solution **out);
+/* n problems planned together (one round of every active problem per step;
+ * problems are admitted while their device memory fits).  settings and
+ * options are arrays of n (or NULL for defaults); out receives n solutions. */
+int mrpop_solve_batch(mrpop_ctx *ctx, int n, const mrpop_problem *problems, const mrpop_settings *settings,
+                      const mrpop_solve_options *options, mrpop_solution **out);
+
+int mrpop_solution_info(const mrpop_solution *s, mrpop_result_info *info);
+/* which = -1: final path; which = k >= 0: k-th improving solution.  Copies at
+ * most cap waypoints (7*n_arms floats each, goal -> start); *n_waypoints gets
+ * the path length, *cost its cost (either may be NULL). */
+int mrpop_solution_path(const mrpop_solution *s, int which, float *out, int cap, int *n_waypoints, float *cost);
+/* Times (ns since submit) at which each improving solution was found. */
+int mrpop_solution_improvements(const mrpop_solution *s, uint64_t *out_ns, int cap, int *n);
+int mrpop_solution_free(mrpop_solution *s);
+
+/* ---- serving: continuous batching over one context --------------------- */
+typedef struct mrpop_server mrpop_server;
+
+/* hbm_budget_bytes bounds the device memory of admitted problems (0: 90% of
+ * the free memory); max_active bounds their number (0: unbounded). */
+int mrpop_server_create(mrpop_ctx *ctx, size_t hbm_budget_bytes, int max_active, mrpop_server **out);
+int mrpop_server_destroy(mrpop_server *srv);
+/* Queue a problem; returns its ticket. */
+int mrpop_server_submit(mrpop_server *srv, const mrpop_problem *problem, const mrpop_settings *settings,
+                        const mrpop_solve_options *options, uint64_t *ticket);
+/* Admit waiting problems that fit, then run one round of every active
+ * problem.  Reports the number of active and waiting problems. */
+int mrpop_server_step(mrpop_server *srv, int *n_active, int *n_waiting);
+/* state: 0 waiting, 1 active, 2 done, -1 unknown ticket.  When out is not
+ * NULL it receives a snapshot of the best solution so far (free it with
+ * mrpop_solution_free). */
+int mrpop_server_poll(mrpop_server *srv, uint64_t ticket, int *state, mrpop_solution **out);
+/* Drop a waiting or active problem. */
+int mrpop_server_cancel(mrpop_server *srv, uint64_t ticket);
+/* Device bytes held by active problems. */
+size_t mrpop_server_bytes(const mrpop_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,361 @@
+// capi_plan.hip -- planning entry points of the C ABI: one problem (solve),
+// many at once (solve_batch) and continuous batching (server).  All three
+// run on one Scheduler: a queue of waiting problems, admitted to the engine
+// while their device memory fits the budget, and one engine step per tick.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <deque>
+#include <map>
+#include <memory>
+#include <new>
+
+#include "engine.h"
+
+using namespace mrpop;
+
+struct mrpop_solution {
+    int D = 0;
+    SlotResult res;
+};
+
+namespace {
+
+struct Request {
+    uint64_t ticket;
+    int n_arms;
+    std::vector<float> bases;  // empty = default
+    std::vector<float> spheres, capsules, zcaps, cuboids, zcubs;
+    std::vector<float> start, goals;
+    int n_goals;
+    uint64_t seed;
+    mrpop_settings st;
+    SolveOptions opt;
+    Slot *slot = nullptr;
+    std::chrono::steady_clock::time_point t_submit;
+    int state = 0;  // 0 waiting, 1 active, 2 done
+    SlotResult done_result;
+};
+
+int copy_problem(const mrpop_problem &p, Request &r) {
+    if (!p.scene || !p.start || !p.goals || p.n_arms < 2 || p.n_arms > MRPOP_MAX_ARMS || p.n_goals < 1)
+        return MRPOP_ERR_INVALID;
+    const int D = 7 * p.n_arms;
+    const mrpop_scene &s = *p.scene;
+    if (s.n_spheres < 0 || s.n_capsules < 0 || s.n_z_capsules < 0 || s.n_cuboids < 0 || s.n_z_cuboids < 0)
+        return MRPOP_ERR_INVALID;
+    r.n_arms = p.n_arms;
+    if (p.bases) r.bases.assign(p.bases, p.bases + 12 * p.n_arms);
+    auto cp = [](std::vector<float> &v, const float *src, int n, int w) {
+        if (n > 0 && src) v.assign(src, src + (size_t)n * w);
+    };
+    cp(r.spheres, s.spheres, s.n_spheres, 4);
+    cp(r.capsules, s.capsules, s.n_capsules, 8);
+    cp(r.zcaps, s.z_capsules, s.n_z_capsules, 8);
+    cp(r.cuboids, s.cuboids, s.n_cuboids, 15);
+    cp(r.zcubs, s.z_cuboids, s.n_z_cuboids, 15);
+    if ((s.n_spheres && !s.spheres) || (s.n_capsules && !s.capsules) || (s.n_z_capsules && !s.z_capsules) ||
+        (s.n_cuboids && !s.cuboids) || (s.n_z_cuboids && !s.z_cuboids))
+        return MRPOP_ERR_INVALID;
+    r.start.assign(p.start, p.start + D);
+    r.goals.assign(p.goals, p.goals + (size_t)p.n_goals * D);
+    r.n_goals = p.n_goals;
+    r.seed = p.seed;
+    return MRPOP_OK;
+}
+
+SolveOptions to_options(const mrpop_solve_options *o) {
+    SolveOptions so;
+    if (o) {
+        so.block_per_robot = o->block_per_robot;
+        so.time_budget_s = o->time_budget_s;
+        so.roadmap_capacity = o->roadmap_capacity;
+        so.tree_capacity = o->tree_capacity;
+        so.neighbour_capacity = o->neighbour_capacity;
+        so.neighbour_candidates = o->neighbour_candidates;
+    }
+    return so;
+}
+
+}  // namespace
+
+struct mrpop_server {
+    Context *ctx = nullptr;
+    std::unique_ptr<Engine> engine;
+    size_t budget = 0;
+    int max_active = 0;
+    uint64_t next_ticket = 1;
+    std::deque<Request *> waiting;
+    std::vector<Request *> active;
+    std::map<uint64_t, Request *> all;
+
+    ~mrpop_server() {
+        engine.reset();
+        for (auto &kv : all) delete kv.second;
+    }
+
+    int submit(const mrpop_problem &p, const mrpop_settings *st, const mrpop_solve_options *o, uint64_t *ticket) {
+        auto *r = new (std::nothrow) Request();
+        if (!r) return MRPOP_ERR_BUDGET;
+        if (int rc = copy_problem(p, *r)) {
+            delete r;
+            return ctx->fail(rc, "invalid problem");
+        }
+        if (st) r->st = *st;
+        else mrpop_settings_default(&r->st);
+        r->opt = to_options(o);
+        r->ticket = next_ticket++;
+        r->t_submit = std::chrono::steady_clock::now();
+        waiting.push_back(r);
+        all[r->ticket] = r;
+        if (ticket) *ticket = r->ticket;
+        return MRPOP_OK;
+    }
+
+    // Admit waiting problems in arrival order while they fit the budget.
+    int admit() {
+        while (!waiting.empty()) {
+            if (max_active > 0 && (int)active.size() >= max_active) break;
+            Request *r = waiting.front();
+            const size_t need = engine->slot_bytes(r->n_arms, r->st, r->opt);
+            if (engine->device_bytes() + need > budget) {
+                if (active.empty())
+                    return ctx->fail(MRPOP_ERR_BUDGET, "one problem needs %zu bytes, budget %zu", need, budget);
+                break;
+            }
+            mrpop_scene sc{(int)(r->spheres.size() / 4), (int)(r->capsules.size() / 8), (int)(r->zcaps.size() / 8),
+                           (int)(r->cuboids.size() / 15), (int)(r->zcubs.size() / 15),
+                           r->spheres.data(), r->capsules.data(), r->zcaps.data(), r->cuboids.data(), r->zcubs.data()};
+            Slot *s = nullptr;
+            const int rc = engine->add(sc, r->n_arms, r->bases.empty() ? nullptr : r->bases.data(), r->start.data(),
+                                       r->n_goals, r->goals.data(), r->seed, r->st, r->opt, &s);
+            if (rc) {
+                if (rc == MRPOP_ERR_BUDGET && !active.empty()) break;  // wait for memory to free up
+                waiting.pop_front();
+                r->state = 2;
+                r->done_result.status = rc;
+                continue;
+            }
+            s->t_submit = r->t_submit;  // time-to-solution counts from submission
+            s->user = r;
+            r->slot = s;
+            r->state = 1;
+            waiting.pop_front();
+            active.push_back(r);
+        }
+        return MRPOP_OK;
+    }
+
+    // Move finished slots' results out and free their memory.
+    void reap() {
+        for (size_t i = 0; i < active.size();) {
+            Request *r = active[i];
+            if (r->slot->phase == Phase::Done) {
+                r->done_result = r->slot->res;
+                engine->release(r->slot);
+                r->slot = nullptr;
+                r->state = 2;
+                active.erase(active.begin() + i);
+            } else {
+                ++i;
+            }
+        }
+    }
+
+    int step(int *n_active, int *n_waiting) {
+        if (int rc = admit()) return rc;
+        if (!active.empty())
+            if (int rc = engine->step()) return rc;
+        reap();
+        if (n_active) *n_active = (int)active.size();
+        if (n_waiting) *n_waiting = (int)waiting.size();
+        return MRPOP_OK;
+    }
+};
+
+namespace {
+
+int make_server(mrpop_ctx *c, size_t budget, int max_active, mrpop_server **out) {
+    auto *srv = new (std::nothrow) mrpop_server();
+    if (!srv) return MRPOP_ERR_BUDGET;
+    srv->ctx = (Context *)c;
+    srv->engine.reset(new Engine((Context *)c));
+    if (budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            delete srv;
+            return ((Context *)c)->fail(MRPOP_ERR_HIP, "hipMemGetInfo failed");
+        }
+        budget = free_b / 10 * 9;
+    }
+    srv->budget = budget;
+    srv->max_active = max_active;
+    *out = srv;
+    return MRPOP_OK;
+}
+
+mrpop_solution *solution_of(const Request *r, int D) {
+    auto *s = new (std::nothrow) mrpop_solution();
+    if (!s) return nullptr;
+    s->D = D;
+    s->res = r->state == 2 ? r->done_result : (r->slot ? r->slot->res : SlotResult{});
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mrpop_solve_options_default(mrpop_solve_options *o) {
+    if (!o) return MRPOP_ERR_INVALID;
+    SolveOptions d;
+    o->block_per_robot = d.block_per_robot;
+    o->time_budget_s = d.time_budget_s;
+    o->roadmap_capacity = d.roadmap_capacity;
+    o->tree_capacity = d.tree_capacity;
+    o->neighbour_capacity = d.neighbour_capacity;
+    o->neighbour_candidates = d.neighbour_candidates;
+    return MRPOP_OK;
+}
+
+int mrpop_solve_batch(mrpop_ctx *c, int n, const mrpop_problem *problems, const mrpop_settings *settings,
+                      const mrpop_solve_options *options, mrpop_solution **out) {
+    if (!c || n < 0 || (n > 0 && (!problems || !out))) return MRPOP_ERR_INVALID;
+    if (hipSetDevice(((Context *)c)->device) != hipSuccess) return MRPOP_ERR_HIP;
+    mrpop_server *srv = nullptr;
+    if (int rc = make_server(c, 0, 0, &srv)) return rc;
+    std::vector<uint64_t> tickets(n);
+    int rc = MRPOP_OK;
+    for (int i = 0; i < n && !rc; ++i)
+        rc = srv->submit(problems[i], settings ? &settings[i] : nullptr, options ? &options[i] : nullptr, &tickets[i]);
+    while (!rc && (!srv->active.empty() || !srv->waiting.empty())) rc = srv->step(nullptr, nullptr);
+    if (!rc)
+        for (int i = 0; i < n; ++i) {
+            Request *r = srv->all[tickets[i]];
+            out[i] = solution_of(r, 7 * problems[i].n_arms);
+            if (!out[i]) rc = MRPOP_ERR_BUDGET;
+        }
+    delete srv;
+    return rc;
+}
+
+int mrpop_solve(mrpop_ctx *c, const mrpop_problem *problem, const mrpop_settings *settings,
+                const mrpop_solve_options *options, mrpop_solution **out) {
+    if (!problem || !out) return MRPOP_ERR_INVALID;
+    return mrpop_solve_batch(c, 1, problem, settings, options, out);
+}
+
+int mrpop_solution_info(const mrpop_solution *s, mrpop_result_info *info) {
+    if (!s || !info) return MRPOP_ERR_INVALID;
+    const SlotResult &r = s->res;
+    info->solved = r.solved;
+    info->cost = r.cost;
+    info->path_length = s->D ? (int)(r.path.size() / s->D) : 0;
+    info->start_tree_size = r.start_tree_size;
+    info->goal_tree_size = r.goal_tree_size;
+    info->iters = r.iters;
+    info->wall_ns = r.wall_ns;
+    info->kernel_ns = r.kernel_ns;
+    info->copy_ns = r.copy_ns;
+    info->ttfs_ns = r.ttfs_ns;
+    info->n_seed_paths = (int)r.seed_paths.size();
+    info->passes = r.passes;
+    info->rounds = r.rounds;
+    info->roadmap_rounds = r.roadmap_rounds;
+    for (int a = 0; a < MRPOP_MAX_ARMS; ++a) info->roadmap_size[a] = r.roadmap_size[a];
+    info->status = r.status;
+    return MRPOP_OK;
+}
+
+int mrpop_solution_path(const mrpop_solution *s, int which, float *out, int cap, int *n_waypoints, float *cost) {
+    if (!s || s->D <= 0) return MRPOP_ERR_INVALID;
+    const std::vector<float> *p;
+    float cst;
+    if (which < 0) {
+        p = &s->res.path;
+        cst = s->res.cost;
+    } else {
+        if (which >= (int)s->res.seed_paths.size()) return MRPOP_ERR_INVALID;
+        p = &s->res.seed_paths[which].path;
+        cst = s->res.seed_paths[which].cost;
+    }
+    const int n = (int)(p->size() / s->D);
+    if (n_waypoints) *n_waypoints = n;
+    if (cost) *cost = cst;
+    if (out && cap > 0) std::copy(p->begin(), p->begin() + (size_t)std::min(n, cap) * s->D, out);
+    return MRPOP_OK;
+}
+
+int mrpop_solution_improvements(const mrpop_solution *s, uint64_t *out_ns, int cap, int *n) {
+    if (!s) return MRPOP_ERR_INVALID;
+    const int k = (int)s->res.improve_ns.size();
+    if (n) *n = k;
+    if (out_ns)
+        for (int i = 0; i < std::min(k, cap); ++i) out_ns[i] = s->res.improve_ns[i];
+    return MRPOP_OK;
+}
+
+int mrpop_solution_free(mrpop_solution *s) {
+    delete s;
+    return MRPOP_OK;
+}
+
+int mrpop_server_create(mrpop_ctx *c, size_t budget, int max_active, mrpop_server **out) {
+    if (!c || !out) return MRPOP_ERR_INVALID;
+    *out = nullptr;
+    if (hipSetDevice(((Context *)c)->device) != hipSuccess) return MRPOP_ERR_HIP;
+    return make_server(c, budget, max_active, out);
+}
+
+int mrpop_server_destroy(mrpop_server *srv) {
+    delete srv;
+    return MRPOP_OK;
+}
+
+int mrpop_server_submit(mrpop_server *srv, const mrpop_problem *p, const mrpop_settings *st,
+                        const mrpop_solve_options *o, uint64_t *ticket) {
+    if (!srv || !p) return MRPOP_ERR_INVALID;
+    return srv->submit(*p, st, o, ticket);
+}
+
+int mrpop_server_step(mrpop_server *srv, int *n_active, int *n_waiting) {
+    if (!srv) return MRPOP_ERR_INVALID;
+    return srv->step(n_active, n_waiting);
+}
+
+int mrpop_server_poll(mrpop_server *srv, uint64_t ticket, int *state, mrpop_solution **out) {
+    if (!srv) return MRPOP_ERR_INVALID;
+    auto it = srv->all.find(ticket);
+    if (it == srv->all.end()) {
+        if (state) *state = -1;
+        return MRPOP_OK;
+    }
+    Request *r = it->second;
+    if (state) *state = r->state;
+    if (out) {
+        *out = solution_of(r, 7 * r->n_arms);
+        if (!*out) return MRPOP_ERR_BUDGET;
+    }
+    return MRPOP_OK;
+}
+
+int mrpop_server_cancel(mrpop_server *srv, uint64_t ticket) {
+    if (!srv) return MRPOP_ERR_INVALID;
+    auto it = srv->all.find(ticket);
+    if (it == srv->all.end()) return MRPOP_ERR_INVALID;
+    Request *r = it->second;
+    if (r->state == 0) {
+        srv->waiting.erase(std::find(srv->waiting.begin(), srv->waiting.end(), r));
+        r->state = 2;
+        r->done_result.status = MRPOP_ERR_STATE;
+    } else if (r->state == 1) {
+        r->slot->phase = Phase::Done;
+        r->slot->res.status = MRPOP_ERR_STATE;
+        srv->reap();
+    }
+    return MRPOP_OK;
+}
+
+size_t mrpop_server_bytes(const mrpop_server *srv) { return srv ? srv->engine->device_bytes() : 0; }
+
+}  // extern "C"

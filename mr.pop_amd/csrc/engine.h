@@ -1,0 +1,131 @@
+// engine.h -- host engine running many planning problems ("slots") on one
+// GPU, one round of every active slot per step.
+//
+// Reference host flow (src/planning/pop_planner.cu:739-1030): for each AO-x
+// pass, reset the trees; in pass 0 grow the per-arm roadmaps until every
+// arm's start and goal connect, then alternate dRRT launches with one more
+// roadmap round until solved; in later passes (optimize) grow the roadmaps
+// one round and run one optimising dRRT launch; simplify the path after a
+// solved pass and keep it if it improves the cost.  Here each slot runs that
+// flow as a state machine, and a step advances every active slot by one
+// round with shared kernel launches.
+#pragma once
+
+#include <chrono>
+#include <cstdint>
+#include <vector>
+
+#include "internal.h"
+#include "planner.h"
+
+namespace mrpop {
+
+struct SolveOptions {
+    int block_per_robot = 8;       // roadmap samples per arm per round = 230 * block_per_robot
+    double time_budget_s = 30.0;   // wall-clock budget of one problem (<= 0: none)
+    int roadmap_capacity = 100000; // nodes per arm roadmap (MAX_ROADMAP_SIZE, roadmap_interior.cuh:21)
+    int tree_capacity = 0;         // 0: min(settings.max_samples, 262144)
+    int neighbour_capacity = 128;  // adjacency entries per roadmap node
+    int neighbour_candidates = 64; // radius neighbours examined per new roadmap node
+};
+
+struct SeedPath {
+    float cost;
+    std::vector<float> path;  // n x D
+};
+
+struct SlotResult {
+    bool solved = false;
+    float cost = 0.f;
+    std::vector<float> path;  // n x D, goal -> start
+    std::vector<SeedPath> seed_paths;
+    int start_tree_size = 0, goal_tree_size = 0, iters = 0;
+    uint64_t wall_ns = 0, kernel_ns = 0, copy_ns = 0;
+    uint64_t ttfs_ns = 0;          // submit -> first solution
+    std::vector<uint64_t> improve_ns;  // submit -> each improving solution
+    int passes = 0, rounds = 0, roadmap_rounds = 0;
+    int roadmap_size[geom::kMaxArms] = {};
+    int status = 0;                // MRPOP_OK, MRPOP_ERR_TIMEOUT, MRPOP_ERR_BUDGET
+};
+
+enum class Phase { Roadmap, Search, Simplify, Done };
+
+struct Slot {
+    int id = -1;
+    int A = 0, D = 0, n_goals = 0;
+    mrpop_settings st{};
+    SolveOptions opt{};
+    unsigned long long seed = 0;
+    std::vector<float> sg;  // (1 + n_goals) x D
+    // device memory
+    void *mem = nullptr;
+    size_t mem_bytes = 0;
+    float *wbuf = nullptr;
+    dev::WorldDesc wdesc{};
+    SlotDev dv{};
+    dev::EdgeGroup g_valid[geom::kMaxArms], g_rm[geom::kMaxArms], g_dr, g_cn, g_sp;
+    // state machine
+    Phase phase = Phase::Roadmap;
+    int pass = 0;
+    bool rm_until_connected = true;
+    bool pass_solved = false;
+    int dr_round = 0, dr_budget = 0;
+    int sp_stage = 0;
+    float best_cost = 3.402823466e38f;
+    std::chrono::steady_clock::time_point t_submit, t_kernel_start;
+    SlotResult res;
+    void *user = nullptr;
+};
+
+// Per-slot report copied back after each step.
+struct SlotReport {
+    int status;
+    int rm_connected[geom::kMaxArms];
+    int rm_n[geom::kMaxArms];
+    int tr_n[2];
+    int path_n;
+    float path_cost;
+    float best_cost;
+    int path_meta[3];
+    int sp_count;
+    int pad;
+};
+
+class Engine {
+  public:
+    explicit Engine(Context *c) : c_(c) {}
+    ~Engine();
+    // Allocate a slot for a problem; returns MRPOP_OK and the slot in *out.
+    int add(const mrpop_scene &scene, int n_arms, const float *bases, const float *start, int n_goals,
+            const float *goals, unsigned long long seed, const mrpop_settings &st, const SolveOptions &opt,
+            Slot **out);
+    // Release a slot's device memory (after it is done).
+    void release(Slot *s);
+    // One round for every active slot.  Returns MRPOP_OK or an error.
+    int step();
+    bool idle() const;
+    size_t slot_bytes(int n_arms, const mrpop_settings &st, const SolveOptions &opt) const;
+    const std::vector<Slot *> &slots() const { return slots_; }
+    size_t device_bytes() const { return bytes_in_use_; }
+
+  private:
+    int ensure_capacity(int n);
+    int finish_pass(Slot *s, const SlotReport &r);
+    int start_pass(Slot *s);
+    void compute_phs(Slot *s);
+    int launch_groups(int mode, const std::vector<const dev::EdgeGroup *> &groups_by_slot, int per_slot,
+                      const std::vector<Slot *> &list, size_t &cursor);
+    Context *c_;
+    std::vector<Slot *> slots_;
+    int cap_ = 0;
+    SlotDev *d_slots_ = nullptr, *h_slots_ = nullptr;
+    dev::WorldDesc *d_worlds_ = nullptr, *h_worlds_ = nullptr;
+    int *d_ids_ = nullptr, *h_ids_ = nullptr;
+    dev::EdgeGroup *d_groups_ = nullptr, *h_groups_ = nullptr;
+    unsigned *d_wave_ = nullptr;
+    SlotReport *d_report_ = nullptr, *h_report_ = nullptr;
+    size_t group_cap_ = 0;
+    size_t bytes_in_use_ = 0;
+};
+
+}  // namespace mrpop

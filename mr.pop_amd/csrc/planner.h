@@ -53,6 +53,9 @@ struct SlotDev {
     float *rs_q;              // [A][M][7]
     uint32_t *rs_coll;        // [A][M]
     int *rs_id;               // [A][M]
+    int *rs_ncand;            // [A][M]  radius candidates of each sample
+    int *rs_first;            // [A][M]  first packed edge of each sample
+    int *rs_cand;             // [A][M][Kc] staged candidates
     int *re_count;            // [A]
     float *re_from, *re_to;   // [A][M*Kc][7]
     int *re_pair;             // [A][M*Kc][2]  (neighbour, new node)
@@ -139,4 +142,28 @@ __host__ __device__ inline uint4 slot_random(unsigned long long seed, unsigned s
     return philox4x32_10(make_uint4(a, b, chunk, stream), make_uint2((unsigned)seed, (unsigned)(seed >> 32)));
 }
 
+}  // namespace mrpop
+
+namespace mrpop {
+namespace plan {
+// Kernel launchers (planner.hip); n = number of slot ids in `ids`.
+void launch_roadmap_init(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_tree_reset(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_rm_sample(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream);
+void launch_rm_assign(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_rm_radius(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream);
+void launch_rm_compact(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_rm_link(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_rm_connected(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_dr_expand(SlotDev *d, const int *ids, int n, int max_b, hipStream_t stream);
+void launch_dr_commit(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_dr_connect(SlotDev *d, const int *ids, int n, int max_b, hipStream_t stream);
+void launch_dr_finish(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_sp_edges(SlotDev *d, const int *ids, int n, int stage, int bspline_step, float range, float min_change,
+                     hipStream_t stream);
+void launch_sp_collect(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_sp_smooth_apply(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_sp_shortcut(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_report(SlotDev *d, const int *ids, int n, void *out, hipStream_t stream);
+}  // namespace plan
 }  // namespace mrpop

@@ -137,22 +137,28 @@ __global__ void k_rm_assign(SlotDev *slots, const int *ids) {
 
 // Radius neighbours of each new node among all nodes with a smaller id
 // (existing ones and earlier samples of this round), first rm_Kc in id order
-// (roadmap_interior.cuh:183-197).  Candidate edges go neighbour -> new node
-// (roadmap_interior.cuh:214-234).  One wave per (slot, arm, sample).
+// (roadmap_interior.cuh:183-197).  Candidates are staged at fixed rows
+// sample*Kc + k; k_rm_compact packs them in sample order.  One wave per
+// (slot, arm, sample).
 __global__ __launch_bounds__(64) void k_rm_radius(SlotDev *slots, const int *ids) {
     SlotDev &s = slots[ids[blockIdx.y]];
     const int t = blockIdx.x;
     if (t >= s.A * s.M) return;
     const int a = t / s.M, i = t - a * s.M;
     const int id = s.rs_id[(size_t)a * s.M + i];
-    if (id < 0) return;
     const int lane = threadIdx.x;
+    int *cnt = s.rs_ncand + (size_t)a * s.M;
+    if (id < 0) {
+        if (lane == 0) cnt[i] = 0;
+        return;
+    }
     const float *rq = s.rm_q + (size_t)a * s.rm_cap * 8;
     const float *qn = rq + (size_t)id * 8;
     float q[7];
 #pragma unroll
     for (int j = 0; j < 7; ++j) q[j] = qn[j];
     const float r2 = s.range * s.range;
+    int *stage = s.rs_cand + ((size_t)a * s.M + i) * s.rm_Kc;
     int found = 0;
     for (int base = 0; base < id && found < s.rm_Kc; base += 64) {
         const int n = base + lane;
@@ -170,36 +176,73 @@ __global__ __launch_bounds__(64) void k_rm_radius(SlotDev *slots, const int *ids
         const unsigned long long bal = __ballot(near);
         if (near) {
             const int pos = found + __popcll(bal & ((1ull << lane) - 1ull));
-            if (pos < s.rm_Kc) {
-                const int slot = atomicAdd(&s.re_count[a], 1);
-                const size_t e = (size_t)a * s.M * s.rm_Kc + slot;
-                const float *p = rq + (size_t)n * 8;
-                for (int j = 0; j < 7; ++j) {
-                    s.re_from[e * 7 + j] = p[j];
-                    s.re_to[e * 7 + j] = q[j];
-                }
-                s.re_pair[e * 2 + 0] = n;
-                s.re_pair[e * 2 + 1] = id;
-            }
+            if (pos < s.rm_Kc) stage[pos] = n;
         }
         found += __popcll(bal);
     }
+    if (lane == 0) cnt[i] = min(found, s.rm_Kc);
 }
 
+// Pack the staged candidates of each arm in sample order (prefix over
+// samples) into the edge list neighbour -> new node (roadmap_interior.cuh:
+// 214-234).  One block per (slot, arm).
+__global__ __launch_bounds__(1024) void k_rm_compact(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int a = blockIdx.x;
+    if (a >= s.A) return;
+    __shared__ int sh[1024];
+    const int tid = threadIdx.x;
+    const int per = (s.M + 1023) / 1024;
+    const int i0 = tid * per;
+    const int *cnt = s.rs_ncand + (size_t)a * s.M;
+    int c = 0;
+    for (int i = i0; i < min(i0 + per, s.M); ++i) c += cnt[i];
+    sh[tid] = c;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int add = tid >= off ? sh[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += add;
+        __syncthreads();
+    }
+    int e = sh[tid] - c;
+    const float *rq = s.rm_q + (size_t)a * s.rm_cap * 8;
+    const size_t ebase = (size_t)a * s.M * s.rm_Kc;
+    for (int i = i0; i < min(i0 + per, s.M); ++i) {
+        s.rs_first[(size_t)a * s.M + i] = e;
+        const int id = s.rs_id[(size_t)a * s.M + i];
+        const int *stage = s.rs_cand + ((size_t)a * s.M + i) * s.rm_Kc;
+        for (int k = 0; k < cnt[i]; ++k, ++e) {
+            const int n = stage[k];
+            const float *p = rq + (size_t)n * 8, *q = rq + (size_t)id * 8;
+            for (int j = 0; j < 7; ++j) {
+                s.re_from[(ebase + e) * 7 + j] = p[j];
+                s.re_to[(ebase + e) * 7 + j] = q[j];
+            }
+            s.re_pair[(ebase + e) * 2 + 0] = n;
+            s.re_pair[(ebase + e) * 2 + 1] = id;
+        }
+    }
+    if (tid == 1023) s.re_count[a] = sh[1023];
+}
+
+// Parents always have smaller ids than their children, so the walk ends; the
+// step cap only guards against corrupted input.
 __device__ __forceinline__ int uf_find(int *uf, int x) {
-    while (true) {
+    for (int steps = 0; steps < (1 << 24); ++steps) {
         const int p = uf[x];
         if (p == x) return x;
         const int gp = uf[p];
         if (gp != p) atomicCAS(&uf[x], p, gp);  // path halving
         x = p;
     }
+    return x;
 }
 
 // Lock-free union: the larger root is hooked under the smaller one, so each
 // component's root is its smallest node and the partition is deterministic.
 __device__ __forceinline__ void uf_union(int *uf, int a, int b) {
-    while (true) {
+    for (int tries = 0; tries < (1 << 20); ++tries) {
         a = uf_find(uf, a);
         b = uf_find(uf, b);
         if (a == b) return;
@@ -213,22 +256,47 @@ __device__ __forceinline__ void uf_union(int *uf, int a, int b) {
 }
 
 // Collision-free candidate edges join both adjacency lists (the reference's
-// symmetric bitset, roadmap_interior.cuh:245-258) and the union-find.
-__global__ void k_rm_link(SlotDev *slots, const int *ids) {
+// symmetric bitset, roadmap_interior.cuh:245-258) and the union-find.  New
+// nodes are taken in id order; the older endpoints of one new node's edges
+// are distinct, so a wave appends them without conflicts and every list is
+// filled in the same order on every run.  One wave per (slot, arm).
+__global__ __launch_bounds__(64) void k_rm_link(SlotDev *slots, const int *ids) {
     SlotDev &s = slots[ids[blockIdx.y]];
-    for (int a = 0; a < s.A; ++a) {
-        const int n = s.re_count[a];
-        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-            const size_t e = (size_t)a * s.M * s.rm_Kc + t;
-            if (s.re_coll[e]) continue;
-            const int u = s.re_pair[e * 2], v = s.re_pair[e * 2 + 1];
-            int *deg = s.rm_deg + (size_t)a * s.rm_cap;
-            int *nbr = s.rm_nbr + (size_t)a * s.rm_cap * s.rm_K;
-            const int pu = atomicAdd(&deg[u], 1);
-            if (pu < s.rm_K) nbr[(size_t)u * s.rm_K + pu] = v;
-            const int pv = atomicAdd(&deg[v], 1);
-            if (pv < s.rm_K) nbr[(size_t)v * s.rm_K + pv] = u;
-            uf_union(s.rm_uf + (size_t)a * s.rm_cap, u, v);
+    const int a = blockIdx.x;
+    if (a >= s.A) return;
+    const int lane = threadIdx.x;
+    int *deg = s.rm_deg + (size_t)a * s.rm_cap;
+    int *nbr = s.rm_nbr + (size_t)a * s.rm_cap * s.rm_K;
+    int *uf = s.rm_uf + (size_t)a * s.rm_cap;
+    const size_t ebase = (size_t)a * s.M * s.rm_Kc;
+    const int *cnt = s.rs_ncand + (size_t)a * s.M;
+    for (int i = 0; i < s.M; ++i) {
+        const int n = cnt[i];
+        if (n == 0) continue;
+        const int e0 = s.rs_first[(size_t)a * s.M + i];
+        bool ok = false;
+        int u = 0, v = 0;
+        if (lane < n) {
+            const size_t e = ebase + e0 + lane;
+            ok = s.re_coll[e] == 0u;
+            u = s.re_pair[e * 2];
+            v = s.re_pair[e * 2 + 1];
+        }
+        const unsigned long long bal = __ballot(ok);
+        // The older endpoints u are distinct within one new node, so each
+        // atomic below has one writer per iteration and returns the same
+        // slot on every run; atomics keep the counts coherent in L2 across
+        // the iterations of this wave.
+        if (ok) {
+            const int du = atomicAdd(&deg[u], 1);
+            if (du < s.rm_K) nbr[(size_t)u * s.rm_K + du] = v;
+            const int dv = __popcll(bal & ((1ull << lane) - 1ull));
+            if (dv < s.rm_K) nbr[(size_t)v * s.rm_K + dv] = u;
+            uf_union(uf, u, v);
+        }
+        if (bal && lane == 0) {
+            const int vv = s.re_pair[(ebase + e0) * 2 + 1];
+            atomicAdd(&deg[vv], __popcll(bal));
         }
     }
 }
@@ -243,11 +311,6 @@ __global__ void k_rm_connected(SlotDev *slots, const int *ids) {
     int c = 0;
     for (int g = 0; g < s.n_goals; ++g) c |= uf_find(uf, 1 + g) == r0;
     s.rm_connected[a] = c;
-    // adjacency lists may have overflowed their capacity: clamp the degrees
-    const int n = s.rm_n[a];
-    int *deg = s.rm_deg + (size_t)a * s.rm_cap;
-    for (int i = 0; i < n; ++i)
-        if (deg[i] > s.rm_K) deg[i] = s.rm_K;
 }
 
 // ====================================================== tree search ======
@@ -307,10 +370,9 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
     __shared__ float vn[35];
     __shared__ float nq[35];
     __shared__ int nrid[kMaxArms];
-    __shared__ float red[2];
 
     // ---- tree choice (pop_planner.cu:366-376)
-    int t = s.ex_tree[b];
+    int t = s.round == 0 ? (b < s.B / 2 ? 0 : 1) : s.ex_tree[b];  // first iteration of a launch
     if (s.round > 0 && s.balance == 2) {
         const int nt = s.tr_n[t], no = s.tr_n[1 - t];
         const float ratio = (float)abs(nt - no) / (float)nt;
@@ -403,7 +465,8 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
     float bound = s.optimize ? c_rand - s.tr_cost[(size_t)t * s.tr_cap + near] : FLT_MAX;
     for (int a = 0; a < A; ++a) {
         const int rid = s.tr_rid[((size_t)t * s.tr_cap + near) * A + a];
-        const int deg = s.rm_deg[(size_t)a * s.rm_cap + rid];
+        // a degree counts every edge; the list keeps the first rm_K
+        const int deg = min(s.rm_deg[(size_t)a * s.rm_cap + rid], s.rm_K);
         const int *nb = s.rm_nbr + ((size_t)a * s.rm_cap + rid) * s.rm_K;
         const float *rq = s.rm_q + (size_t)a * s.rm_cap * 8;
         float bang = 1e3f;
@@ -656,13 +719,15 @@ __global__ __launch_bounds__(256) void k_dr_finish(SlotDev *slots, const int *id
     }
     if (winner < 0) return;
     // ---- trace both trees to their roots; path runs goal -> start
-    const int t = s.ex_tree[winner], o = 1 - t, D = s.D;
+    const int t = s.ex_tree[winner], D = s.D;
     int chain_n[2] = {0, 0};
     // count chains
     const int start_node[2] = {t == 0 ? wnode : s.cn_near[winner], t == 1 ? wnode : s.cn_near[winner]};
+    // parent links strictly lower the (round-start) node cost, so chains end at a
+    // root; the length check also guards against corrupted links
     for (int tr = 0; tr < 2; ++tr) {
         int cur = start_node[tr], n = 1;
-        while (s.tr_parent[(size_t)tr * s.tr_cap + cur] != cur) {
+        while (s.tr_parent[(size_t)tr * s.tr_cap + cur] != cur && n <= s.path_cap) {
             cur = s.tr_parent[(size_t)tr * s.tr_cap + cur];
             ++n;
         }
@@ -709,6 +774,275 @@ __global__ __launch_bounds__(256) void k_dr_finish(SlotDev *slots, const int *id
     *s.best_cost = fminf(*s.best_cost, cost);
     *s.status = winner + 1;
 }
+
+}  // namespace plan
+}  // namespace mrpop
+
+// ====================================================== simplification ====
+// path_simplification.cu:41-238.  Stages 0..bspline_step-1 smooth the odd
+// waypoints; the last stage checks every shortcut i -> j (j - i <= 16) and
+// runs the shortcut DP.  Chunk edges are allocated with a block-wide prefix
+// so the assignment is deterministic; candidates that do not fit are treated
+// as colliding (the path stays valid, only less simplified).
+namespace mrpop {
+namespace plan {
+
+__device__ __forceinline__ float l2(const float *a, const float *b, int D) { return sqrtf(sqdist(a, b, D)); }
+
+// Block-wide exclusive prefix of per-thread counts (blockDim.x == 1024).
+__device__ __forceinline__ int block_prefix(int v, int *sh, int &total) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int add = tid >= off ? sh[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += add;
+        __syncthreads();
+    }
+    total = sh[1023];
+    const int ex = sh[tid] - v;
+    __syncthreads();
+    return ex;
+}
+
+// Writes chunk edges for candidates; fail[item] starts at 0.
+//   stage < bspline_step: items are odd waypoints i: edges p[i-1] -> mid and p[i+1] -> mid
+//   otherwise:            items are (i, b) pairs, j = i + b + 1, b in 1..15: edge p[i] -> p[j]
+__global__ __launch_bounds__(1024) void k_sp_edges(SlotDev *slots, const int *ids, int stage, int bspline_step,
+                                                   float range, float min_change) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int N = *s.path_n, D = s.D;
+    __shared__ int sh[1024];
+    const bool smooth = stage < bspline_step;
+    const int n_items = smooth ? N : N * 16;
+    int base = 0;
+    for (int chunk0 = 0; chunk0 < n_items; chunk0 += 1024) {
+        const int item = chunk0 + threadIdx.x;
+        int cnt = 0;  // chunks of this item
+        int c1 = 0, c2 = 0;
+        bool live = false;
+        if (item < n_items) {
+            s.sp_fail[item] = 0;
+            if (smooth) {
+                const int i = item;
+                if ((i & 1) && i < N - 1) {
+                    const float *p0 = s.path + (size_t)(i - 1) * D, *p1 = s.path + (size_t)i * D,
+                                *p2 = s.path + (size_t)(i + 1) * D;
+                    float *mid = s.sp_mid + (size_t)i * D;
+                    float moved = 0.f;
+                    for (int d = 0; d < D; ++d) {
+                        const float b1 = (p0[d] + p1[d]) / 2.0f, b2 = (p1[d] + p2[d]) / 2.0f;
+                        mid[d] = (b1 + b2) / 2.0f;
+                        const float x = p1[d] - mid[d];
+                        moved = __builtin_fmaf(x, x, moved);
+                    }
+                    if (sqrtf(moved) >= min_change) {
+                        live = true;
+                        c1 = (int)ceilf(l2(p0, mid, D) / range);
+                        c2 = (int)ceilf(l2(p2, mid, D) / range);
+                        cnt = c1 + c2;
+                    }
+                }
+                if (!live) s.sp_fail[i] = 1;  // nothing to apply
+            } else {
+                const int i = item >> 4, b = item & 15, j = i + b + 1;
+                if (b >= 1 && j < N) {
+                    live = true;
+                    const float d = l2(s.path + (size_t)i * D, s.path + (size_t)j * D, D);
+                    c1 = (int)ceilf((d - 0.001f) / range);  // path_simplification.cu:159-162
+                    if (c1 < 0) c1 = 0;
+                    cnt = c1;
+                }
+            }
+        }
+        int total;
+        int off = block_prefix(cnt, sh, total);
+        off += base;
+        if (live && cnt > 0) {
+            if (off + cnt > s.sp_cap) {
+                s.sp_fail[item] = 1;  // no room: keep the waypoint / no shortcut
+            } else {
+                int e = off;
+                for (int side = 0; side < (smooth ? 2 : 1); ++side) {
+                    const float *from, *to;
+                    int nc;
+                    if (smooth) {
+                        from = s.path + (size_t)(side == 0 ? item - 1 : item + 1) * D;
+                        to = s.sp_mid + (size_t)item * D;
+                        nc = side == 0 ? c1 : c2;
+                    } else {
+                        from = s.path + (size_t)(item >> 4) * D;
+                        to = s.path + (size_t)((item >> 4) + (item & 15) + 1) * D;
+                        nc = c1;
+                    }
+                    for (int d = 0; d < D; ++d) {
+                        const float delta = (to[d] - from[d]) / (float)nc;
+                        float cur = from[d];
+                        for (int c = 0; c < nc; ++c) {
+                            s.sp_from[(size_t)(e + c) * D + d] = cur;
+                            cur += delta;  // the reference advances by delta per chunk
+                            s.sp_to[(size_t)(e + c) * D + d] = cur;
+                        }
+                    }
+                    for (int c = 0; c < nc; ++c) s.sp_item[e + c] = item;
+                    e += nc;
+                }
+            }
+        }
+        base += total;
+    }
+    if (threadIdx.x == 0) *s.sp_count = min(base, s.sp_cap);
+}
+
+// Chunk verdicts back to their items.
+__global__ void k_sp_collect(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int n = *s.sp_count;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+        if (s.sp_coll[e]) s.sp_fail[s.sp_item[e]] = 1;
+}
+
+// Smoothing stage: replace free odd waypoints by their midpoints.
+__global__ void k_sp_smooth_apply(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int N = *s.path_n, D = s.D;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < N * D; t += gridDim.x * blockDim.x) {
+        const int i = t / D;
+        if ((i & 1) && i < N - 1 && !s.sp_fail[i]) s.path[t] = s.sp_mid[t];
+    }
+}
+
+// Shortcut DP over the checked window edges, path rewrite with copies of
+// the kept waypoints, and the simplified cost (path_simplification.cu:143-238).
+__global__ void k_sp_shortcut(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    if (threadIdx.x != 0) return;
+    const int N = *s.path_n, D = s.D;
+    float *cost = s.sp_cost;
+    int *par = s.sp_parent;
+    cost[0] = 0.f;
+    par[0] = -1;
+    for (int t = 1; t < N; ++t) {
+        cost[t] = 3.402823466e38f;
+        par[t] = -1;
+    }
+    for (int i = 0; i < N - 1; ++i) {
+        for (int b = 0; b < 16; ++b) {
+            const int j = i + b + 1;
+            if (j >= N) break;
+            if (b > 0 && s.sp_fail[i * 16 + b]) continue;
+            const float c = l2(s.path + (size_t)i * D, s.path + (size_t)j * D, D) + cost[i];
+            if (cost[j] > c) {
+                cost[j] = c;
+                par[j] = i;
+            }
+        }
+    }
+    const float best = *s.best_cost;
+    if (cost[N - 1] < best) {
+        int child = N - 1, parent = par[child];
+        while (parent < N && parent != -1) {
+            for (int idx = parent + 1; idx < child; ++idx)
+                for (int d = 0; d < D; ++d) s.path[(size_t)idx * D + d] = s.path[(size_t)parent * D + d];
+            child = parent;
+            parent = par[child];
+        }
+    }
+    *s.path_cost = cost[N - 1];
+    *s.best_cost = fminf(best, cost[N - 1]);
+}
+
+// Per-slot report for the host.
+struct ReportDev {
+    int status;
+    int rm_connected[kMaxArms];
+    int rm_n[kMaxArms];
+    int tr_n[2];
+    int path_n;
+    float path_cost;
+    float best_cost;
+    int path_meta[3];
+    int sp_count;
+    int pad;
+};
+
+__global__ void k_report(SlotDev *slots, const int *ids, int n, ReportDev *out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int id = ids[t];
+    SlotDev &s = slots[id];
+    ReportDev &r = out[id];
+    r.status = *s.status;
+    for (int a = 0; a < kMaxArms; ++a) {
+        r.rm_connected[a] = a < s.A ? s.rm_connected[a] : 1;
+        r.rm_n[a] = a < s.A ? s.rm_n[a] : 0;
+    }
+    r.tr_n[0] = s.tr_n[0];
+    r.tr_n[1] = s.tr_n[1];
+    r.path_n = *s.path_n;
+    r.path_cost = *s.path_cost;
+    r.best_cost = *s.best_cost;
+    for (int k = 0; k < 3; ++k) r.path_meta[k] = s.path_meta[k];
+    r.sp_count = *s.sp_count;
+}
+
+// ------------------------------------------------------------ launchers ----
+#define MRPOP_L(kernel, grid, block, ...) hipLaunchKernelGGL(kernel, grid, block, 0, stream, __VA_ARGS__)
+
+void launch_roadmap_init(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+    MRPOP_L(k_roadmap_init, dim3(1, n), dim3(256), d, ids);
+}
+void launch_tree_reset(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+    MRPOP_L(k_tree_reset, dim3(1, n), dim3(256), d, ids);
+}
+void launch_rm_sample(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream) {
+    MRPOP_L(k_rm_sample, dim3((max_am + 255) / 256, n), dim3(256), d, ids);
+}
+void launch_rm_assign(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+    MRPOP_L(k_rm_assign, dim3(kMaxArms, n), dim3(256), d, ids);
+}
+void launch_rm_radius(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream) {
+    MRPOP_L(k_rm_radius, dim3(max_am, n), dim3(64), d, ids);
+}
+void launch_rm_compact(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+    MRPOP_L(k_rm_compact, dim3(kMaxArms, n), dim3(1024), d, ids);
+}
+void launch_rm_link(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+    MRPOP_L(k_rm_link, dim3(kMaxArms, n), dim3(64), d, ids);
+}
+void launch_rm_connected(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+    MRPOP_L(k_rm_connected, dim3(1, n), dim3(64), d, ids);
+}
+void launch_dr_expand(SlotDev *d, const int *ids, int n, int max_b, hipStream_t stream) {
+    MRPOP_L(k_dr_expand, dim3(max_b, n), dim3(64), d, ids);
+}
+void launch_dr_commit(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+    MRPOP_L(k_dr_commit, dim3(1, n), dim3(256), d, ids);
+}
+void launch_dr_connect(SlotDev *d, const int *ids, int n, int max_b, hipStream_t stream) {
+    MRPOP_L(k_dr_connect, dim3(max_b, n), dim3(64), d, ids);
+}
+void launch_dr_finish(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+    MRPOP_L(k_dr_finish, dim3(1, n), dim3(256), d, ids);
+}
+void launch_sp_edges(SlotDev *d, const int *ids, int n, int stage, int bspline_step, float range, float min_change,
+                     hipStream_t stream) {
+    MRPOP_L(k_sp_edges, dim3(1, n), dim3(1024), d, ids, stage, bspline_step, range, min_change);
+}
+void launch_sp_collect(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+    MRPOP_L(k_sp_collect, dim3(16, n), dim3(256), d, ids);
+}
+void launch_sp_smooth_apply(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+    MRPOP_L(k_sp_smooth_apply, dim3(16, n), dim3(256), d, ids);
+}
+void launch_sp_shortcut(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+    MRPOP_L(k_sp_shortcut, dim3(1, n), dim3(64), d, ids);
+}
+void launch_report(SlotDev *d, const int *ids, int n, void *out, hipStream_t stream) {
+    MRPOP_L(k_report, dim3((n + 63) / 64), dim3(64), d, ids, n, (ReportDev *)out);
+}
+#undef MRPOP_L
 
 }  // namespace plan
 }  // namespace mrpop

@@ -33,6 +33,27 @@ class Settings(ctypes.Structure):
                 ("perturb_empty_step", I)]
 
 
+class SolveOptions(ctypes.Structure):
+    """mrpop_solve_options (include/mrpop_c.h)."""
+    _fields_ = [("block_per_robot", I), ("time_budget_s", ctypes.c_double), ("roadmap_capacity", I),
+                ("tree_capacity", I), ("neighbour_capacity", I), ("neighbour_candidates", I)]
+
+
+class CProblem(ctypes.Structure):
+    _fields_ = [("n_arms", I), ("bases", P), ("scene", P), ("start", P), ("n_goals", I), ("goals", P),
+                ("seed", ctypes.c_uint64)]
+
+
+U64 = ctypes.c_uint64
+
+
+class ResultInfo(ctypes.Structure):
+    _fields_ = [("solved", I), ("cost", ctypes.c_float), ("path_length", I), ("start_tree_size", I),
+                ("goal_tree_size", I), ("iters", I), ("wall_ns", U64), ("kernel_ns", U64), ("copy_ns", U64),
+                ("ttfs_ns", U64), ("n_seed_paths", I), ("passes", I), ("rounds", I), ("roadmap_rounds", I),
+                ("roadmap_size", I * 5), ("status", I)]
+
+
 class MrpopError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"mrpop error {code}: {msg}")
@@ -53,6 +74,20 @@ SIGNATURES = {
     "mrpop_check_edges": (I, [P, P, I, I, I, I, P, P, P, P]),
     "mrpop_check_edges_device": (I, [P, P, I, I, I, I, P, P, P, P, P]),
     "mrpop_sphere_centres": (I, [P, P, I, P, P, P]),
+    "mrpop_solve_options_default": (I, [P]),
+    "mrpop_solve": (I, [P, P, P, P, P]),
+    "mrpop_solve_batch": (I, [P, I, P, P, P, P]),
+    "mrpop_solution_info": (I, [P, P]),
+    "mrpop_solution_path": (I, [P, I, P, I, P, P]),
+    "mrpop_solution_improvements": (I, [P, P, I, P]),
+    "mrpop_solution_free": (I, [P]),
+    "mrpop_server_create": (I, [P, ctypes.c_size_t, I, P]),
+    "mrpop_server_destroy": (I, [P]),
+    "mrpop_server_submit": (I, [P, P, P, P, P]),
+    "mrpop_server_step": (I, [P, P, P]),
+    "mrpop_server_poll": (I, [P, U64, P, P]),
+    "mrpop_server_cancel": (I, [P, U64]),
+    "mrpop_server_bytes": (ctypes.c_size_t, [P]),
 }
 
 _lib = None
@@ -194,3 +229,140 @@ def gpu_checker(ctx: Context):
             cache[key] = World(ctx, scene, n_arms, bases)
         return cache[key].config_collides(cfgs)
     return checker
+
+
+# --------------------------------------------------------------- planning --
+def solve_options_default(**kw) -> SolveOptions:
+    o = SolveOptions()
+    lib().mrpop_solve_options_default(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def settings(**kw) -> Settings:
+    """pop_settings defaults with overrides."""
+    s = settings_default()
+    for k, v in kw.items():
+        if not hasattr(s, k):
+            raise AttributeError(k)
+        setattr(s, k, v)
+    return s
+
+
+class _ProblemRef:
+    """Keeps the numpy arrays behind a CProblem alive."""
+
+    def __init__(self, problem):
+        self.p = problem
+        self.scene_c = problem.scene.as_c()
+        self.bases = _f32(problem.bases).reshape(-1)
+        self.start = _f32(problem.start).reshape(-1)
+        self.goals = _f32(problem.goals).reshape(-1)
+        self.c = CProblem(problem.n_arms, self.bases.ctypes.data, ctypes.addressof(self.scene_c),
+                          self.start.ctypes.data, len(problem.goals), self.goals.ctypes.data, int(problem.seed))
+
+
+class Solution:
+    """Result of a solve: PlannerResult fields plus timing."""
+
+    def __init__(self, handle, dim):
+        self.dim = dim
+        info = ResultInfo()
+        lib().mrpop_solution_info(handle, ctypes.byref(info))
+        self.info = {f: getattr(info, f) for f, _ in ResultInfo._fields_ if f != "roadmap_size"}
+        self.info["roadmap_size"] = list(info.roadmap_size)
+        self.solved = bool(info.solved)
+        self.cost = float(info.cost)
+        self.path = self._path(handle, -1)
+        self.seed_paths = [self._path(handle, k) for k in range(info.n_seed_paths)]
+        n = ctypes.c_int()
+        lib().mrpop_solution_improvements(handle, None, 0, ctypes.byref(n))
+        arr = (U64 * max(1, n.value))()
+        lib().mrpop_solution_improvements(handle, arr, n.value, ctypes.byref(n))
+        self.improve_s = [arr[i] * 1e-9 for i in range(n.value)]
+        lib().mrpop_solution_free(handle)
+
+    def _path(self, handle, which):
+        n = ctypes.c_int()
+        c = ctypes.c_float()
+        lib().mrpop_solution_path(handle, which, None, 0, ctypes.byref(n), ctypes.byref(c))
+        out = np.zeros((n.value, self.dim), np.float32)
+        lib().mrpop_solution_path(handle, which, out.ctypes.data, n.value, ctypes.byref(n), ctypes.byref(c))
+        return out if which < 0 else (float(c.value), out)
+
+    @property
+    def ttfs_s(self):
+        return self.info["ttfs_ns"] * 1e-9
+
+    @property
+    def wall_s(self):
+        return self.info["wall_ns"] * 1e-9
+
+
+def solve(ctx: Context, problem, settings=None, options=None) -> Solution:
+    ref = _ProblemRef(problem)
+    out = P()
+    ctx.check(lib().mrpop_solve(ctx.handle, ctypes.byref(ref.c), None if settings is None else ctypes.byref(settings),
+                                None if options is None else ctypes.byref(options), ctypes.byref(out)))
+    return Solution(out, problem.dim)
+
+
+def solve_batch(ctx: Context, problems, settings=None, options=None):
+    refs = [_ProblemRef(p) for p in problems]
+    n = len(refs)
+    arr = (CProblem * n)(*[r.c for r in refs])
+    st = None if settings is None else (Settings * n)(*settings)
+    op = None if options is None else (SolveOptions * n)(*options)
+    outs = (P * n)()
+    ctx.check(lib().mrpop_solve_batch(ctx.handle, n, arr, st, op, outs))
+    return [Solution(outs[i], problems[i].dim) for i in range(n)]
+
+
+class Server:
+    """Continuous batching: submit problems at any time, step, poll."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes=0, max_active=0):
+        self.ctx = ctx
+        self._h = P()
+        self._refs = {}
+        ctx.check(lib().mrpop_server_create(ctx.handle, int(hbm_budget_bytes), int(max_active), ctypes.byref(self._h)))
+        ctx._children.add(self)
+
+    def submit(self, problem, settings=None, options=None) -> int:
+        ref = _ProblemRef(problem)
+        t = U64()
+        self.ctx.check(lib().mrpop_server_submit(self._h, ctypes.byref(ref.c),
+                                                 None if settings is None else ctypes.byref(settings),
+                                                 None if options is None else ctypes.byref(options), ctypes.byref(t)))
+        self._refs[t.value] = (ref, problem.dim)
+        return t.value
+
+    def step(self):
+        a, w = I(), I()
+        self.ctx.check(lib().mrpop_server_step(self._h, ctypes.byref(a), ctypes.byref(w)))
+        return a.value, w.value
+
+    def poll(self, ticket, snapshot=False):
+        st = I()
+        out = P()
+        self.ctx.check(lib().mrpop_server_poll(self._h, ticket, ctypes.byref(st), ctypes.byref(out) if snapshot else None))
+        sol = Solution(out, self._refs[ticket][1]) if snapshot and out else None
+        return st.value, sol
+
+    def cancel(self, ticket):
+        self.ctx.check(lib().mrpop_server_cancel(self._h, ticket))
+
+    def bytes(self):
+        return lib().mrpop_server_bytes(self._h)
+
+    def close(self):
+        if self._h:
+            lib().mrpop_server_destroy(self._h)
+            self._h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

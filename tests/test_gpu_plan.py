@@ -1,0 +1,233 @@
+"""Planner parity on the GPU: dRRT + AO-x through the C ABI.
+
+A returned path must (a) run from a goal to the start, (b) stay inside the
+joint limits, (c) cost the sum of its segment lengths, and (d) be collision
+free at the resolution the planner checks it with.  (d) is decided by the
+FP64 oracle: a segment is free if, for one of the discretisations the
+planner uses for the edges it commits, every chunk is free with contacts
+closer than 1e-5 m read as free:
+
+* a tree extension edge: one edge of ``granularity`` samples
+  (pop_planner.cu:521-546 checks it with the same ``granularity``),
+* a tree connection: ``floor(d / range) + 1`` chunks (pop_planner.cu:657-690),
+* a simplified segment: ``ceil(d / range)`` chunks
+  (path_simplification.cu:159-162).
+
+The reference's planner output is random (curand streams and scheduling), so
+its own tests check properties like these rather than paths; here the
+planner is also deterministic for a seed, and a problem gives the same path
+whether it is solved alone or in a batch with others.
+"""
+import dataclasses
+import math
+import time
+
+import numpy as np
+import pytest
+
+from mrpop import _lib, geometry, scenes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# settings of the fixture runs: 300 new configurations per round, 64 samples
+# per edge, range 2.0, three AO-x passes, short optimisation passes
+SETTINGS = dict(num_new_configs=300, granularity=64, range=2.0, rrtc_iter=3, optimize_iters=1500)
+BUDGET_S = 60.0
+
+
+def _settings(**kw):
+    d = dict(SETTINGS)
+    d.update(kw)
+    return _lib.settings(**d)
+
+
+def _options(**kw):
+    d = dict(time_budget_s=BUDGET_S)
+    d.update(kw)
+    return _lib.solve_options_default(**d)
+
+
+def _fixture_problems():
+    four = scenes.load_problem_file("tests/golden/panda_four_problems.json", geometry.default_bases(4))
+    five = scenes.load_problem_file("tests/golden/panda_five_problems.json", geometry.default_bases(5))
+    return four[:3] + five[:2]
+
+
+PROBLEMS = _fixture_problems()
+NAMES = [p.name for p in PROBLEMS]
+
+
+def _chunks(a, b, n):
+    return [(a + (b - a) * c / n, a + (b - a) * (c + 1) / n) for c in range(n)]
+
+
+def segment_verdicts(problem, path, granularity, rng):
+    """Per segment: True if free under one of the planner's discretisations."""
+    path = np.asarray(path, np.float64)
+    rows, owner = [], []
+    for i in range(len(path) - 1):
+        a, b = path[i], path[i + 1]
+        d = float(np.linalg.norm(b - a))
+        for variant, n in enumerate((1, max(1, math.ceil(d / rng)), int(d / rng) + 1)):
+            for f, t in _chunks(a, b, n):
+                rows.append((f, t))
+                owner.append((i, variant))
+    if not rows:
+        return []
+    frm = np.array([r[0] for r in rows], np.float32)
+    to = np.array([r[1] for r in rows], np.float32)
+    r = O.check_edges(problem.scene, O.MODE_FULL, problem.bases, granularity, frm, to)
+    hit = {}
+    for (i, v), lo in zip(owner, r["lo"]):
+        hit[(i, v)] = hit.get((i, v), False) or bool(lo)
+    return [any(not hit[(i, v)] for v in range(3)) for i in range(len(path) - 1)]
+
+
+def check_solution(problem, sol, granularity=SETTINGS["granularity"], rng=SETTINGS["range"]):
+    assert sol.solved, f"{problem.name}: not solved ({sol.info})"
+    path = sol.path
+    assert path.shape[1] == problem.dim and len(path) >= 2
+    assert np.allclose(path[-1], problem.start, atol=1e-6), "path must end at the start"
+    assert min(np.abs(path[0] - g).max() for g in problem.goals) < 1e-6, "path must begin at a goal"
+    low = np.tile(np.array(geometry.table("kLow"), np.float32), problem.n_arms)
+    high = low + np.tile(np.array(geometry.table("kScale"), np.float32), problem.n_arms)
+    assert (path >= low - 1e-5).all() and (path <= high + 1e-5).all(), "path leaves the joint limits"
+    length = float(np.linalg.norm(np.diff(path.astype(np.float64), axis=0), axis=1).sum())
+    assert sol.cost == pytest.approx(length, rel=1e-3, abs=1e-3)
+    free = segment_verdicts(problem, path, granularity, rng)
+    assert all(free), f"{problem.name}: segments in collision: {[i for i, f in enumerate(free) if not f]}"
+
+
+@pytest.fixture(scope="module")
+def alone(ctx):
+    out = {}
+    for p in PROBLEMS:
+        print(f"[plan] solve {p.name}", flush=True)
+        out[p.name] = _lib.solve(ctx, p, _settings(), _options())
+    return out
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_fixture_problem_solved_and_free(alone, name):
+    p = PROBLEMS[NAMES.index(name)]
+    sol = alone[name]
+    assert sol.info["status"] == _lib.OK, sol.info
+    check_solution(p, sol)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_anytime_improvements(alone, name):
+    """Every recorded solution improves on the one before and is itself a valid path."""
+    p = PROBLEMS[NAMES.index(name)]
+    sol = alone[name]
+    costs = [c for c, _ in sol.seed_paths]
+    assert len(costs) >= 1 and len(costs) == len(sol.improve_s)
+    assert all(b < a for a, b in zip(costs, costs[1:])), costs
+    assert sol.cost == pytest.approx(costs[-1])
+    assert sol.improve_s == sorted(sol.improve_s)
+    assert sol.ttfs_s == pytest.approx(sol.improve_s[0]) and 0 < sol.ttfs_s <= sol.wall_s
+    for c, path in sol.seed_paths:
+        assert len(path) >= 2
+        assert np.allclose(path[-1], p.start, atol=1e-6)
+
+
+def test_deterministic_for_a_seed(ctx, alone):
+    p = PROBLEMS[0]
+    again = _lib.solve(ctx, p, _settings(), _options())
+    assert np.array_equal(again.path, alone[p.name].path)
+    assert again.info["rounds"] == alone[p.name].info["rounds"]
+
+
+def test_seed_changes_the_search(ctx, alone):
+    p = PROBLEMS[1]
+    q = dataclasses.replace(p, seed=p.seed + 1)
+    other = _lib.solve(ctx, q, _settings(), _options())
+    check_solution(q, other)
+    assert not np.array_equal(other.path, alone[p.name].path)
+
+
+def test_same_alone_and_in_the_batch(ctx, alone):
+    """Slots share every launch but not their state: batching changes no result."""
+    sols = _lib.solve_batch(ctx, PROBLEMS, [_settings()] * len(PROBLEMS), [_options()] * len(PROBLEMS))
+    for p, s in zip(PROBLEMS, sols):
+        assert s.info["status"] == _lib.OK
+        assert np.array_equal(s.path, alone[p.name].path), p.name
+        assert [c for c, _ in s.seed_paths] == [c for c, _ in alone[p.name].seed_paths]
+        assert s.info["rounds"] == alone[p.name].info["rounds"]
+
+
+@pytest.mark.parametrize("kind,seed", [("plain2", 3), ("table3", 5)])
+def test_generated_two_and_three_arm(ctx, kind, seed):
+    p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
+    sol = _lib.solve(ctx, p, _settings(), _options())
+    check_solution(p, sol)
+
+
+def test_without_simplification(ctx):
+    p = PROBLEMS[0]
+    sol = _lib.solve(ctx, p, _settings(path_simplify=False, rrtc_iter=1), _options())
+    check_solution(p, sol)
+    assert sol.info["passes"] == 1
+
+
+def test_server_continuous_batching(ctx, alone):
+    """Problems submitted while others run finish with their solo results."""
+    srv = _lib.Server(ctx)
+    tickets = {}
+    tickets[srv.submit(PROBLEMS[0], _settings(), _options())] = PROBLEMS[0]
+    steps = 0
+    pending = list(PROBLEMS[1:4])
+    cancelled = srv.submit(PROBLEMS[4], _settings(), _options())
+    srv.cancel(cancelled)
+    assert srv.poll(cancelled)[0] == 2
+    deadline = time.time() + 3 * BUDGET_S
+    while True:
+        if pending and steps % 50 == 0:
+            p = pending.pop(0)
+            tickets[srv.submit(p, _settings(), _options())] = p
+        active, waiting = srv.step()
+        steps += 1
+        if steps == 10:  # anytime: a snapshot while running is a valid (maybe unsolved) result
+            st, snap = srv.poll(next(iter(tickets)), snapshot=True)
+            assert st in (1, 2) and snap is not None
+        if not pending and active == 0 and waiting == 0:
+            break
+        assert time.time() < deadline, "server did not finish"
+    assert srv.bytes() == 0
+    for t, p in tickets.items():
+        st, sol = srv.poll(t, snapshot=True)
+        assert st == 2
+        assert np.array_equal(sol.path, alone[p.name].path), p.name
+    assert srv.poll(123456789)[0] == -1
+    srv.close()
+
+
+def test_server_budget(ctx):
+    """A problem larger than the whole budget is refused, not queued forever."""
+    srv = _lib.Server(ctx, hbm_budget_bytes=1 << 20)
+    srv.submit(PROBLEMS[0], _settings(), _options())
+    with pytest.raises(_lib.MrpopError) as e:
+        srv.step()
+    assert e.value.code == _lib.ERR_BUDGET
+    srv.close()
+
+
+def test_time_budget(ctx):
+    p = PROBLEMS[3]
+    sol = _lib.solve(ctx, p, _settings(optimize_iters=400000), _options(time_budget_s=1.0))
+    assert sol.info["status"] == _lib.ERR_TIMEOUT
+    assert sol.wall_s < 2.0
+    if sol.solved:  # an anytime result survives the cut
+        check_solution(p, sol)
+
+
+def test_invalid_problems(ctx):
+    p = PROBLEMS[0]
+    bad = dataclasses.replace(p, n_arms=6, bases=np.zeros((6, 12), np.float32), start=np.zeros(42, np.float32),
+                              goals=np.zeros((1, 42), np.float32))
+    with pytest.raises(_lib.MrpopError):
+        _lib.solve(ctx, bad)
+    nog = dataclasses.replace(p, goals=p.goals[:0])
+    with pytest.raises(_lib.MrpopError):
+        _lib.solve(ctx, nog)
