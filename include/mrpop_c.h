@@ -188,6 +188,7 @@ typedef struct mrpop_solve_options {
     int tree_capacity;         /* nodes per tree (0: min(max_samples, 262144)) */
     int neighbour_capacity;    /* adjacency entries per roadmap node (default 128) */
     int neighbour_candidates;  /* radius neighbours tried per new roadmap node, 1..64 (default 64) */
+    int rounds_per_step;       /* dRRT rounds queued between host checks (default 8) */
 } mrpop_solve_options;
 
 int mrpop_solve_options_default(mrpop_solve_options *o);

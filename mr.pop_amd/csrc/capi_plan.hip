@@ -73,6 +73,7 @@ SolveOptions to_options(const mrpop_solve_options *o) {
         so.tree_capacity = o->tree_capacity;
         so.neighbour_capacity = o->neighbour_capacity;
         so.neighbour_candidates = o->neighbour_candidates;
+        so.rounds_per_step = o->rounds_per_step;
     }
     return so;
 }
@@ -215,6 +216,7 @@ int mrpop_solve_options_default(mrpop_solve_options *o) {
     o->tree_capacity = d.tree_capacity;
     o->neighbour_capacity = d.neighbour_capacity;
     o->neighbour_candidates = d.neighbour_candidates;
+    o->rounds_per_step = d.rounds_per_step;
     return MRPOP_OK;
 }
 

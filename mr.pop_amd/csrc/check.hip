@@ -694,12 +694,14 @@ __global__ void wave_prefix_kernel(const dev::EdgeGroup *groups, int n_groups, i
 
 template <int A, int MODE>
 void enqueue_check(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups, const dev::EdgeGroup &g0,
-                   int n_groups, unsigned *d_wave_start, const FineQueue &aux, int check_blocks,
+                   int n_groups, unsigned *d_wave_start, const FineQueue &aux, int check_blocks, bool zero,
                    hipStream_t stream) {
     constexpr int S = 64 / A;
-    hipLaunchKernelGGL(zero_outputs_kernel, dim3(d_groups ? 4 : std::max(1, std::min(1024, (g0.n_edges + 255) / 256)),
-                                                 std::min(n_groups, 1024)),
-                       dim3(256), 0, stream, d_groups, g0, n_groups);
+    if (zero)
+        hipLaunchKernelGGL(zero_outputs_kernel,
+                           dim3(d_groups ? 4 : std::max(1, std::min(1024, (g0.n_edges + 255) / 256)),
+                                std::min(n_groups, 1024)),
+                           dim3(256), 0, stream, d_groups, g0, n_groups);
     if (d_groups)
         hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups, S, d_wave_start);
     hipLaunchKernelGGL((dev::check_kernel<A, MODE>), dim3(check_blocks), dim3(64), 0, stream, d_worlds, d_groups, g0,
@@ -708,10 +710,10 @@ void enqueue_check(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_group
 
 int dispatch(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
              const dev::EdgeGroup &g0, int n_groups, unsigned *d_wave_start, const FineQueue &aux, int check_blocks,
-             hipStream_t stream) {
+             bool zero, hipStream_t stream) {
 #define MRPOP_CASE(M, AA)                                                                                         \
     case M * 8 + AA:                                                                                              \
-        enqueue_check<AA, M>(d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, stream);          \
+        enqueue_check<AA, M>(d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, zero, stream);    \
         break;
     switch (mode * 8 + A) {
         MRPOP_CASE(MRPOP_MODE_SINGLE, 1)
@@ -749,15 +751,16 @@ int launch_check(Context *c, const World *w, int mode, int arm, int granularity,
     const int S = 64 / A;
     const size_t waves = (total + S - 1) / S;
     const int check_blocks = (int)std::min<size_t>(waves, (size_t)c->num_cus * 32);
-    return dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux, check_blocks, stream);
+    return dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux, check_blocks, true, stream);
 }
 
 int launch_check_groups(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
-                        int n_groups, unsigned *d_wave_start, hipStream_t stream) {
+                        int n_groups, unsigned *d_wave_start, size_t max_waves, bool zero, hipStream_t stream) {
     if (n_groups <= 0) return MRPOP_OK;
     const FineQueue aux{nullptr};
     dev::EdgeGroup g0{};
-    return dispatch(c, mode, A, d_worlds, d_groups, g0, n_groups, d_wave_start, aux, c->num_cus * 32, stream);
+    const int check_blocks = (int)std::max<size_t>(1, std::min<size_t>(max_waves, (size_t)c->num_cus * 32));
+    return dispatch(c, mode, A, d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, zero, stream);
 }
 
 int launch_centres(Context *c, const World *w, int n_cfg, const float *d_cfg, float *d_coarse, float *d_fine,

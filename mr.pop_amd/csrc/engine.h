@@ -27,6 +27,7 @@ struct SolveOptions {
     int tree_capacity = 0;         // 0: min(settings.max_samples, 262144)
     int neighbour_capacity = 128;  // adjacency entries per roadmap node
     int neighbour_candidates = 64; // radius neighbours examined per new roadmap node
+    int rounds_per_step = 8;       // dRRT rounds queued per host step
 };
 
 struct SeedPath {
@@ -88,8 +89,9 @@ struct SlotReport {
     float best_cost;
     int path_meta[3];
     int sp_count;
-    int pad;
+    int ctr[4];  // device round counters (kCtr*)
 };
+constexpr int plan_ctr_round = kCtrRound, plan_ctr_run = kCtrRun;
 
 class Engine {
   public:
@@ -113,8 +115,19 @@ class Engine {
     int finish_pass(Slot *s, const SlotReport &r);
     int start_pass(Slot *s);
     void compute_phs(Slot *s);
-    int launch_groups(int mode, const std::vector<const dev::EdgeGroup *> &groups_by_slot, int per_slot,
-                      const std::vector<Slot *> &list, size_t &cursor);
+    struct Staged {
+        int mode = 0;
+        struct Bucket {
+            int A;
+            size_t first;
+            int n;
+            size_t max_waves;
+        };
+        std::vector<Bucket> buckets;
+    };
+    int stage_groups(int mode, const std::vector<const dev::EdgeGroup *> &groups_by_slot, int per_slot,
+                     const std::vector<Slot *> &list, size_t &cursor, Staged &out);
+    int launch_staged(const Staged &st);
     Context *c_;
     std::vector<Slot *> slots_;
     int cap_ = 0;

@@ -124,6 +124,11 @@ size_t Engine::slot_bytes(int A, const mrpop_settings &st, const SolveOptions &o
     L.add(sizeof(int) * A * M);                                             // rs_id
     L.add(sizeof(int) * A * M * 2);                                         // rs_ncand, rs_first
     L.add(sizeof(int) * (size_t)A * M * Kc);                                // rs_cand
+    const size_t rl_cap = std::min<size_t>((size_t)rm_cap, (size_t)M * Kc), rl_words = ((size_t)M + 31) / 32;
+    L.add(sizeof(int) * (size_t)A * rm_cap);                                // rl_slot
+    L.add(sizeof(int) * (size_t)A * rl_cap);                                // rl_touched
+    L.add(sizeof(int) * 16);                                                // rs_base, rl_ntouched
+    L.add(sizeof(unsigned) * (size_t)A * rl_cap * rl_words);                // rl_bits
     L.add(sizeof(float) * (size_t)A * M * Kc * 7);                          // re_from
     L.add(sizeof(float) * (size_t)A * M * Kc * 7);                          // re_to
     L.add(sizeof(int) * (size_t)A * M * Kc * 2);                            // re_pair
@@ -162,8 +167,9 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
         return c_->fail(MRPOP_ERR_INVALID, "num_new_configs outside 1..65535");
     if (!(st.range > 0)) return c_->fail(MRPOP_ERR_INVALID, "range must be positive");
     if (opt.block_per_robot < 1 || opt.roadmap_capacity < 1 + n_goals + 1 || opt.neighbour_capacity < 1 ||
-        opt.neighbour_candidates < 1 || opt.neighbour_candidates > 64)
-        return c_->fail(MRPOP_ERR_INVALID, "bad solve options (neighbour_candidates must be 1..64)");
+        opt.neighbour_candidates < 1 || opt.neighbour_candidates > 64 || opt.rounds_per_step < 1 ||
+        opt.rounds_per_step > 1024)
+        return c_->fail(MRPOP_ERR_INVALID, "bad solve options (neighbour_candidates 1..64, rounds_per_step 1..1024)");
     const int counts[5] = {scene.n_spheres, scene.n_capsules, scene.n_z_capsules, scene.n_cuboids, scene.n_z_cuboids};
     long n_obs = 0;
     for (int k = 0; k < 5; ++k) {
@@ -245,6 +251,14 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     v.rs_ncand = (int *)take(sizeof(int) * A * M * 2);
     v.rs_first = v.rs_ncand + A * M;
     v.rs_cand = (int *)take(sizeof(int) * (size_t)A * M * Kc);
+    const size_t rl_cap = std::min<size_t>((size_t)rm_cap, (size_t)M * Kc), rl_words = ((size_t)M + 31) / 32;
+    v.rl_cap = (int)rl_cap;
+    v.rl_words = (int)rl_words;
+    v.rl_slot = (int *)take(sizeof(int) * (size_t)A * rm_cap);
+    v.rl_touched = (int *)take(sizeof(int) * (size_t)A * rl_cap);
+    v.rs_base = (int *)take(sizeof(int) * 16);
+    v.rl_ntouched = v.rs_base + 8;
+    v.rl_bits = (unsigned *)take(sizeof(unsigned) * (size_t)A * rl_cap * rl_words);
     v.re_from = (float *)take(sizeof(float) * (size_t)A * M * Kc * 7);
     v.re_to = (float *)take(sizeof(float) * (size_t)A * M * Kc * 7);
     v.re_pair = (int *)take(sizeof(int) * (size_t)A * M * Kc * 2);
@@ -264,6 +278,7 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     v.path_n = ctr + 7;
     v.path_meta = ctr + 8;
     v.sp_count = ctr + 12;
+    v.ctr = ctr + 16;
     int *ex = (int *)take(sizeof(int) * B * 6 + sizeof(float) * B);
     v.ex_tree = ex;
     v.ex_state = ex + B;
@@ -359,6 +374,8 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     MRPOP_HIP(c_, hipMemcpyAsync(v.sg, s->sg.data(), s->sg.size() * sizeof(float), hipMemcpyHostToDevice, c_->stream));
     MRPOP_HIP(c_, hipMemsetAsync(v.rw_key, 0xff, sizeof(unsigned long long) * 2 * (size_t)tr_cap, c_->stream));
     MRPOP_HIP(c_, hipMemsetAsync(ctr, 0, sizeof(int) * 64, c_->stream));
+    MRPOP_HIP(c_, hipMemsetAsync(v.rl_slot, 0xff, sizeof(int) * (size_t)A * rm_cap, c_->stream));
+    MRPOP_HIP(c_, hipMemsetAsync(v.rl_bits, 0, sizeof(unsigned) * (size_t)A * rl_cap * rl_words, c_->stream));
     {
         const float fm = kFltMax;
         MRPOP_HIP(c_, hipMemcpyAsync(v.best_cost, &fm, sizeof(float), hipMemcpyHostToDevice, c_->stream));
@@ -369,11 +386,15 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
         s->g_valid[a] = dev::EdgeGroup{v.rs_q + (size_t)a * M * 7, v.rs_q + (size_t)a * M * 7, 7, 1, M, a, nullptr,
                                        v.rs_coll + (size_t)a * M, nullptr, id, 0};
         s->g_rm[a] = dev::EdgeGroup{v.re_from + (size_t)a * M * Kc * 7, v.re_to + (size_t)a * M * Kc * 7, 7,
-                                    st.granularity, 0, a, v.re_count + a, v.re_coll + (size_t)a * M * Kc, nullptr, id, 0};
+                                    st.granularity, M * Kc, a, v.re_count + a, v.re_coll + (size_t)a * M * Kc, nullptr,
+                                    id, 0};
     }
-    s->g_dr = dev::EdgeGroup{v.ed_from, v.ed_to, D, st.granularity, 0, 0, v.ed_count, v.ed_coll, nullptr, id, 0};
-    s->g_cn = dev::EdgeGroup{v.ce_from, v.ce_to, D, st.granularity, 0, 0, v.ce_count, v.ce_coll, nullptr, id, 0};
-    s->g_sp = dev::EdgeGroup{v.sp_from, v.sp_to, D, st.granularity, 0, 0, v.sp_count, v.sp_coll, nullptr, id, 0};
+    // groups with a device count carry their capacity in n_edges (a bound for the grid)
+    s->g_dr = dev::EdgeGroup{v.ed_from, v.ed_to, D, st.granularity, B, 0, v.ed_count, v.ed_coll, nullptr, id, 0};
+    s->g_cn = dev::EdgeGroup{v.ce_from, v.ce_to, D, st.granularity, (int)ce_cap, 0, v.ce_count, v.ce_coll, nullptr,
+                             id, 0};
+    s->g_sp = dev::EdgeGroup{v.sp_from, v.sp_to, D, st.granularity, (int)sp_cap, 0, v.sp_count, v.sp_coll, nullptr,
+                             id, 0};
 
     h_worlds_[id] = w;
     h_slots_[id] = v;
@@ -453,26 +474,44 @@ int Engine::start_pass(Slot *s) {
     return MRPOP_OK;
 }
 
-// Launch one batched check over the groups of `list` (per_slot groups each,
-// selected by `pick`), bucketed by arm count for the multi-arm modes.
-int Engine::launch_groups(int mode, const std::vector<const dev::EdgeGroup *> &groups, int per_slot,
-                          const std::vector<Slot *> &list, size_t &cursor) {
+// Copy the edge-group descriptors of `list` (per_slot groups each) to the
+// device, bucketed by arm count for the multi-arm modes.  The descriptors
+// hold device pointers and counts only, so one copy serves every round of a
+// step.
+int Engine::stage_groups(int mode, const std::vector<const dev::EdgeGroup *> &groups, int per_slot,
+                         const std::vector<Slot *> &list, size_t &cursor, Staged &out) {
+    out.mode = mode;
+    out.buckets.clear();
+    const size_t begin = cursor;
     for (int A = (mode == MRPOP_MODE_SINGLE ? 1 : 2); A <= (mode == MRPOP_MODE_SINGLE ? 1 : 5); ++A) {
-        const size_t first = cursor;
+        Staged::Bucket bk{A, cursor, 0, 0};
+        const int S = 64 / A;
         for (size_t i = 0; i < list.size(); ++i) {
             if (mode != MRPOP_MODE_SINGLE && list[i]->A != A) continue;
             const int n = mode == MRPOP_MODE_SINGLE ? list[i]->A : 1;
             for (int k = 0; k < n && k < per_slot; ++k) {
                 if (cursor >= group_cap_) return c_->fail(MRPOP_ERR_BUDGET, "edge-group staging full");
-                h_groups_[cursor++] = groups[i][k];
+                const dev::EdgeGroup &g = groups[i][k];
+                h_groups_[cursor++] = g;
+                bk.max_waves += ((size_t)g.n_edges * g.G + S - 1) / S;
             }
         }
-        const int ng = (int)(cursor - first);
-        if (!ng) continue;
-        MRPOP_HIP(c_, hipMemcpyAsync(d_groups_ + first, h_groups_ + first, sizeof(dev::EdgeGroup) * ng,
-                                     hipMemcpyHostToDevice, c_->stream));
-        if (int rc = launch_check_groups(c_, mode, A, d_worlds_, d_groups_ + first, ng, d_wave_, c_->stream)) return rc;
+        bk.n = (int)(cursor - bk.first);
+        if (bk.n) out.buckets.push_back(bk);
     }
+    if (cursor > begin)
+        MRPOP_HIP(c_, hipMemcpyAsync(d_groups_ + begin, h_groups_ + begin, sizeof(dev::EdgeGroup) * (cursor - begin),
+                                     hipMemcpyHostToDevice, c_->stream));
+    return MRPOP_OK;
+}
+
+// One batched check launch per arm-count bucket.  Every producer of planner
+// edges clears their verdict words, so the launches skip the zeroing pass.
+int Engine::launch_staged(const Staged &st) {
+    for (const Staged::Bucket &bk : st.buckets)
+        if (int rc = launch_check_groups(c_, st.mode, bk.A, d_worlds_, d_groups_ + bk.first, bk.n, d_wave_, bk.max_waves,
+                                         false, c_->stream))
+            return rc;
     return MRPOP_OK;
 }
 
@@ -520,31 +559,48 @@ int Engine::step() {
             gv.push_back(s->g_valid);
             gr.push_back(s->g_rm);
         }
-        if (int rc = launch_groups(MRPOP_MODE_SINGLE, gv, geom::kMaxArms, rm, cursor)) return rc;
+        Staged sv, sr;
+        if (int rc = stage_groups(MRPOP_MODE_SINGLE, gv, geom::kMaxArms, rm, cursor, sv)) return rc;
+        if (int rc = stage_groups(MRPOP_MODE_SINGLE, gr, geom::kMaxArms, rm, cursor, sr)) return rc;
+        if (int rc = launch_staged(sv)) return rc;
         plan::launch_rm_assign(d_slots_, d, n, st);
         plan::launch_rm_radius(d_slots_, d, n, max_am, st);
         plan::launch_rm_compact(d_slots_, d, n, st);
-        if (int rc = launch_groups(MRPOP_MODE_SINGLE, gr, geom::kMaxArms, rm, cursor)) return rc;
-        plan::launch_rm_link(d_slots_, d, n, st);
+        if (int rc = launch_staged(sr)) return rc;
+        plan::launch_rm_link(d_slots_, d, n, max_am, st);
         plan::launch_rm_connected(d_slots_, d, n, st);
     }
-    // ---- dRRT round (pop_planner.cu:350-736)
+    // ---- dRRT rounds (pop_planner.cu:350-736).  The round index lives on the
+    // device; a slot whose pass is solved or whose launch is out of rounds
+    // skips the remaining rounds of the step.
     if (!dr.empty()) {
         int *d = upload_ids(dr, 1);
         const int n = (int)dr.size();
-        int max_b = 0;
-        for (Slot *s : dr) max_b = std::max(max_b, s->dv.B);
-        plan::launch_dr_expand(d_slots_, d, n, max_b, st);
+        int max_b = 0, rounds = 1 << 30;
+        bool reset = false;
+        for (Slot *s : dr) {
+            max_b = std::max(max_b, s->dv.B);
+            rounds = std::min(rounds, std::max(1, s->opt.rounds_per_step));
+            reset |= s->dv.round_reset != 0;
+            s->dv.round_reset = 0;  // applied by this step's k_dr_begin
+        }
         std::vector<const dev::EdgeGroup *> g1, g2;
         for (Slot *s : dr) {
             g1.push_back(&s->g_dr);
             g2.push_back(&s->g_cn);
         }
-        if (int rc = launch_groups(MRPOP_MODE_DRRT, g1, 1, dr, cursor)) return rc;
-        plan::launch_dr_commit(d_slots_, d, n, st);
-        plan::launch_dr_connect(d_slots_, d, n, max_b, st);
-        if (int rc = launch_groups(MRPOP_MODE_FULL, g2, 1, dr, cursor)) return rc;
-        plan::launch_dr_finish(d_slots_, d, n, st);
+        Staged s1, s2;
+        if (int rc = stage_groups(MRPOP_MODE_DRRT, g1, 1, dr, cursor, s1)) return rc;
+        if (int rc = stage_groups(MRPOP_MODE_FULL, g2, 1, dr, cursor, s2)) return rc;
+        if (reset) plan::launch_dr_begin(d_slots_, d, n, st);
+        for (int r = 0; r < rounds; ++r) {
+            plan::launch_dr_expand(d_slots_, d, n, max_b, st);
+            if (int rc = launch_staged(s1)) return rc;
+            plan::launch_dr_commit(d_slots_, d, n, st);
+            plan::launch_dr_connect(d_slots_, d, n, max_b, st);
+            if (int rc = launch_staged(s2)) return rc;
+            plan::launch_dr_finish(d_slots_, d, n, st);
+        }
     }
     // ---- simplification stage (path_simplification.cu:41-238)
     if (!sp.empty()) {
@@ -581,7 +637,9 @@ int Engine::step() {
                                           s0->st.bspline_min_change, st);
                     std::vector<const dev::EdgeGroup *> gs;
                     for (Slot *s : same) gs.push_back(&s->g_sp);
-                    if (int rc = launch_groups(MRPOP_MODE_FULL, gs, 1, same, cursor)) return rc;
+                    Staged ss;
+                    if (int rc = stage_groups(MRPOP_MODE_FULL, gs, 1, same, cursor, ss)) return rc;
+                    if (int rc = launch_staged(ss)) return rc;
                     plan::launch_sp_collect(d_slots_, d, n, st);
                     if (stage < s0->st.bspline_step) plan::launch_sp_smooth_apply(d_slots_, d, n, st);
                     else plan::launch_sp_shortcut(d_slots_, d, n, st);
@@ -615,15 +673,14 @@ int Engine::step() {
             s->rm_until_connected = false;
             s->phase = Phase::Search;
             s->dr_round = 0;
-            s->dv.round = 0;
+            s->dv.round_reset = 1;  // a fresh dRRT launch
+            s->dv.round_limit = s->dr_budget;
         }
     }
     for (Slot *s : dr) {
         const SlotReport &r = h_report_[s->id];
-        s->dr_round++;
-        s->dv.round = s->dr_round;
-        s->dv.rng_round++;
-        s->res.rounds++;
+        s->dr_round = r.ctr[plan_ctr_round];
+        s->res.rounds = r.ctr[plan_ctr_run];
         if (r.status > 0) {
             s->pass_solved = true;
             s->best_cost = r.best_cost;

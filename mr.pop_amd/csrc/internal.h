@@ -68,12 +68,14 @@ int launch_check(Context *c, const World *w, int mode, int arm, int granularity,
 
 // Batched check over edge groups in device memory (check.hip).  Every group
 // of one launch uses the same mode and arm count A; d_wave_start holds
-// n_groups + 1 entries of workspace.
+// n_groups + 1 entries of workspace.  max_waves bounds the persistent grid
+// (an upper bound of the waves the groups can need); zero = false when the
+// producers of the edges already cleared the verdict words.
 namespace dev {
 struct EdgeGroup;
 }
 int launch_check_groups(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
-                        int n_groups, unsigned *d_wave_start, hipStream_t stream);
+                        int n_groups, unsigned *d_wave_start, size_t max_waves, bool zero, hipStream_t stream);
 
 // Sphere centres kernel (check.hip).
 int launch_centres(Context *c, const World *w, int n_cfg, const float *d_cfg, float *d_coarse, float *d_fine,

@@ -28,6 +28,15 @@ constexpr int kRunning = 0;
 // > 0: solved, value = 1 + winning expansion
 constexpr int kFailedOverflow = -1;  // tree capacity reached (pop_planner.cu:587-589)
 
+// Device round counters of a slot.  Several dRRT rounds run per host step,
+// so the round index lives on the device: k_dr_finish advances it.
+enum : int {
+    kCtrRound = 0,  // round within the current dRRT launch
+    kCtrRng = 1,    // dRRT rounds since the solve began (random stream position)
+    kCtrLimit = 2,  // rounds allowed in the current launch
+    kCtrRun = 3,    // dRRT rounds run since the solve began
+};
+
 struct SlotDev {
     // problem
     int A, D, n_goals, world;
@@ -36,9 +45,11 @@ struct SlotDev {
     int G, B, M;  // granularity, num_new_configs, roadmap samples per arm per round
     float range, tree_ratio;
     int balance, phs;
-    // per-pass / per-round state written by the host before each launch
-    int optimize, round, rm_round, pass;
-    int rng_round;            // dRRT rounds since the solve began (random stream position)
+    // per-pass / per-round state written by the host before each step
+    int optimize, rm_round, pass;
+    int round_reset;          // 1: k_dr_begin restarts the round count of a dRRT launch
+    int round_limit;          // rounds of the current dRRT launch (max_iters / optimize_iters)
+    int *ctr;                 // [4] device round counters, see kCtr*
     float phs_center[35];
     float *sg;                // [1 + n_goals][D]: start, then goals
     // roadmaps (arm-major)
@@ -57,6 +68,13 @@ struct SlotDev {
     int *rs_first;            // [A][M]  first packed edge of each sample
     int *rs_cand;             // [A][M][Kc] staged candidates
     int *re_count;            // [A]
+    int *rs_base;             // [A]  first node id of this round's samples
+    // link staging: bitsets over this round's new ids for each older endpoint
+    int rl_cap, rl_words;     // claimed bitsets per arm, 32-bit words per bitset
+    int *rl_slot;             // [A][rm_cap]  bitset of a node this round (-1 none)
+    int *rl_touched;          // [A][rl_cap]  node of each claimed bitset
+    int *rl_ntouched;         // [A]
+    unsigned *rl_bits;        // [A][rl_cap][rl_words]
     float *re_from, *re_to;   // [A][M*Kc][7]
     int *re_pair;             // [A][M*Kc][2]  (neighbour, new node)
     uint32_t *re_coll;        // [A][M*Kc]
@@ -153,8 +171,9 @@ void launch_rm_sample(SlotDev *d, const int *ids, int n, int max_am, hipStream_t
 void launch_rm_assign(SlotDev *d, const int *ids, int n, hipStream_t stream);
 void launch_rm_radius(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream);
 void launch_rm_compact(SlotDev *d, const int *ids, int n, hipStream_t stream);
-void launch_rm_link(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_rm_link(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream);
 void launch_rm_connected(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_dr_begin(SlotDev *d, const int *ids, int n, hipStream_t stream);
 void launch_dr_expand(SlotDev *d, const int *ids, int n, int max_b, hipStream_t stream);
 void launch_dr_commit(SlotDev *d, const int *ids, int n, hipStream_t stream);
 void launch_dr_connect(SlotDev *d, const int *ids, int n, int max_b, hipStream_t stream);

@@ -55,6 +55,12 @@ __device__ __forceinline__ float *tree_q(const SlotDev &s, int t, int i) {
     return s.tr_q + ((size_t)t * s.tr_cap + i) * s.D;
 }
 
+// A dRRT round runs for a slot while its pass is unsolved and the launch has
+// rounds left (pop_planner.cu:360-364).
+__device__ __forceinline__ bool searching(const SlotDev &s) {
+    return *s.status == kRunning && s.ctr[kCtrRound] < s.ctr[kCtrLimit];
+}
+
 // ============================================================ roadmaps ====
 // Roadmap of each arm: node 0 = the arm's start slice, nodes 1..n_goals its
 // goal slices (roadmap_interior.cuh:69-88).
@@ -92,6 +98,7 @@ __global__ void k_rm_sample(SlotDev *slots, const int *ids) {
     float *q = s.rs_q + ((size_t)a * s.M + i) * 7;
 #pragma unroll
     for (int j = 0; j < 7; ++j) q[j] = __builtin_fmaf(u01(bits[j]), kScale[j], kLow[j]);
+    s.rs_coll[(size_t)a * s.M + i] = 0u;  // the checker only sets bits
 }
 
 // Valid samples become nodes n, n+1, ... in sample order.  One block per (slot, arm).
@@ -104,6 +111,8 @@ __global__ void k_rm_assign(SlotDev *slots, const int *ids) {
     if (threadIdx.x == 0) {
         base = s.rm_n[a];
         s.re_count[a] = 0;
+        s.rs_base[a] = base;     // first id of this round's new nodes
+        s.rl_ntouched[a] = 0;
     }
     __syncthreads();
     const int per = (s.M + 255) / 256;
@@ -221,6 +230,7 @@ __global__ __launch_bounds__(1024) void k_rm_compact(SlotDev *slots, const int *
             }
             s.re_pair[(ebase + e) * 2 + 0] = n;
             s.re_pair[(ebase + e) * 2 + 1] = id;
+            s.re_coll[ebase + e] = 0u;
         }
     }
     if (tid == 1023) s.re_count[a] = sh[1023];
@@ -256,47 +266,108 @@ __device__ __forceinline__ void uf_union(int *uf, int a, int b) {
 }
 
 // Collision-free candidate edges join both adjacency lists (the reference's
-// symmetric bitset, roadmap_interior.cuh:245-258) and the union-find.  New
-// nodes are taken in id order; the older endpoints of one new node's edges
-// are distinct, so a wave appends them without conflicts and every list is
-// filled in the same order on every run.  One wave per (slot, arm).
-__global__ __launch_bounds__(64) void k_rm_link(SlotDev *slots, const int *ids) {
+// symmetric bitset, roadmap_interior.cuh:245-258) and the union-find.
+//
+// Lists are filled in id order, so their contents do not depend on the order
+// in which threads run: a new node v first lists its older neighbours (its
+// own candidates, already in id order), then newer nodes as later rounds
+// (or later samples of this round) link to it.  An older node u collects the
+// new nodes of this round in a bitset over the round's id range
+// [rs_base, rs_base + M); k_rm_link_append then appends them in ascending
+// order.  Three launches:
+//   k_rm_link_own    one wave per sample: v's own list, union-find, and a
+//                    bitset claimed for every older endpoint u;
+//   k_rm_link_bits   one wave per sample: set bit (v - base) in u's bitset;
+//   k_rm_link_append waves over the claimed bitsets: append, count, clear.
+__global__ __launch_bounds__(64) void k_rm_link_own(SlotDev *slots, const int *ids) {
     SlotDev &s = slots[ids[blockIdx.y]];
-    const int a = blockIdx.x;
-    if (a >= s.A) return;
+    const int t = blockIdx.x;
+    if (t >= s.A * s.M) return;
+    const int a = t / s.M, i = t - a * s.M;
+    const int n = s.rs_ncand[(size_t)a * s.M + i];
+    if (n == 0) return;
     const int lane = threadIdx.x;
+    const size_t e = (size_t)a * s.M * s.rm_Kc + s.rs_first[(size_t)a * s.M + i] + lane;
+    bool ok = false;
+    int u = 0, v = 0;
+    if (lane < n) {
+        ok = s.re_coll[e] == 0u;
+        u = s.re_pair[e * 2];
+        v = s.re_pair[e * 2 + 1];
+    }
+    const unsigned long long bal = __ballot(ok);
+    if (!bal) return;
     int *deg = s.rm_deg + (size_t)a * s.rm_cap;
     int *nbr = s.rm_nbr + (size_t)a * s.rm_cap * s.rm_K;
-    int *uf = s.rm_uf + (size_t)a * s.rm_cap;
-    const size_t ebase = (size_t)a * s.M * s.rm_Kc;
-    const int *cnt = s.rs_ncand + (size_t)a * s.M;
-    for (int i = 0; i < s.M; ++i) {
-        const int n = cnt[i];
-        if (n == 0) continue;
-        const int e0 = s.rs_first[(size_t)a * s.M + i];
-        bool ok = false;
-        int u = 0, v = 0;
-        if (lane < n) {
-            const size_t e = ebase + e0 + lane;
-            ok = s.re_coll[e] == 0u;
-            u = s.re_pair[e * 2];
-            v = s.re_pair[e * 2 + 1];
+    if (ok) {
+        const int pos = __popcll(bal & ((1ull << lane) - 1ull));
+        if (pos < s.rm_K) nbr[(size_t)v * s.rm_K + pos] = u;
+        uf_union(s.rm_uf + (size_t)a * s.rm_cap, u, v);
+        int *slot_of = s.rl_slot + (size_t)a * s.rm_cap;
+        if (atomicCAS(&slot_of[u], -1, -2) == -1) {  // first claim of u this round
+            const int k = atomicAdd(&s.rl_ntouched[a], 1);
+            s.rl_touched[(size_t)a * s.rl_cap + k] = u;
+            slot_of[u] = k;
         }
-        const unsigned long long bal = __ballot(ok);
-        // The older endpoints u are distinct within one new node, so each
-        // atomic below has one writer per iteration and returns the same
-        // slot on every run; atomics keep the counts coherent in L2 across
-        // the iterations of this wave.
-        if (ok) {
-            const int du = atomicAdd(&deg[u], 1);
-            if (du < s.rm_K) nbr[(size_t)u * s.rm_K + du] = v;
-            const int dv = __popcll(bal & ((1ull << lane) - 1ull));
-            if (dv < s.rm_K) nbr[(size_t)v * s.rm_K + dv] = u;
-            uf_union(uf, u, v);
+    }
+    // v is new: nothing else writes its degree before k_rm_link_append
+    if (lane == 0) deg[v] = __popcll(bal);
+}
+
+__global__ __launch_bounds__(64) void k_rm_link_bits(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int t = blockIdx.x;
+    if (t >= s.A * s.M) return;
+    const int a = t / s.M, i = t - a * s.M;
+    const int n = s.rs_ncand[(size_t)a * s.M + i];
+    const int lane = threadIdx.x;
+    if (lane >= n) return;
+    const size_t e = (size_t)a * s.M * s.rm_Kc + s.rs_first[(size_t)a * s.M + i] + lane;
+    if (s.re_coll[e]) return;
+    const int u = s.re_pair[e * 2], v = s.re_pair[e * 2 + 1];
+    const int k = s.rl_slot[(size_t)a * s.rm_cap + u];
+    const int j = v - s.rs_base[a];
+    atomicOr(&s.rl_bits[((size_t)a * s.rl_cap + k) * s.rl_words + (j >> 5)], 1u << (j & 31));
+}
+
+__global__ __launch_bounds__(256) void k_rm_link_append(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.z]];
+    const int a = blockIdx.y;
+    if (a >= s.A) return;
+    const int lane = threadIdx.x & 63;
+    const int n_touched = s.rl_ntouched[a];
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    const int base = s.rs_base[a];
+    int *deg = s.rm_deg + (size_t)a * s.rm_cap;
+    int *nbr = s.rm_nbr + (size_t)a * s.rm_cap * s.rm_K;
+    for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < n_touched; k += waves) {
+        const int u = s.rl_touched[(size_t)a * s.rl_cap + k];
+        unsigned *bits = s.rl_bits + ((size_t)a * s.rl_cap + k) * s.rl_words;
+        int d = deg[u];
+        for (int w0 = 0; w0 < s.rl_words; w0 += 64) {
+            const int w = w0 + lane;
+            const unsigned word = w < s.rl_words ? bits[w] : 0u;
+            const int c = __popc(word);
+            // exclusive prefix of the set-bit counts over the lanes
+            int incl = c;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int y = __shfl_up(incl, off);
+                if (lane >= off) incl += y;
+            }
+            int pos = d + incl - c;
+            unsigned rest = word;
+            while (rest) {
+                const int b = __ffs(rest) - 1;
+                rest &= rest - 1;
+                if (pos < s.rm_K) nbr[(size_t)u * s.rm_K + pos] = base + (w << 5) + b;
+                ++pos;
+            }
+            if (w < s.rl_words && word) bits[w] = 0u;  // the pool stays clear between rounds
+            d += __shfl(incl, 63);
         }
-        if (bal && lane == 0) {
-            const int vv = s.re_pair[(ebase + e0) * 2 + 1];
-            atomicAdd(&deg[vv], __popcll(bal));
+        if (lane == 0) {
+            deg[u] = d;  // every edge counts; the list keeps the first rm_K
+            s.rl_slot[(size_t)a * s.rm_cap + u] = -1;
         }
     }
 }
@@ -314,6 +385,15 @@ __global__ void k_rm_connected(SlotDev *slots, const int *ids) {
 }
 
 // ====================================================== tree search ======
+// A new dRRT launch restarts its round count (pop_planner.cu:350-364).
+__global__ void k_dr_begin(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.x]];
+    if (threadIdx.x == 0 && s.round_reset) {
+        s.ctr[kCtrRound] = 0;
+        s.ctr[kCtrLimit] = s.round_limit;
+    }
+}
+
 // Trees reset at the start of every AO-x pass (pop_planner.cu:933-950):
 // tree 0 = the start, tree 1 = the goals, each its own parent with cost 0.
 __global__ void k_tree_reset(SlotDev *slots, const int *ids) {
@@ -363,17 +443,19 @@ __device__ __forceinline__ float edge_angle(const float *v, const float *q, cons
 __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids) {
     SlotDev &s = slots[ids[blockIdx.y]];
     const int b = blockIdx.x;
-    if (b >= s.B || *s.status != kRunning) return;
+    if (b >= s.B || !searching(s)) return;
     const int lane = threadIdx.x;
     const int D = s.D, A = s.A;
+    const int round = s.ctr[kCtrRound];
+    const unsigned rng_round = (unsigned)s.ctr[kCtrRng];
     __shared__ float q[35];
     __shared__ float vn[35];
     __shared__ float nq[35];
     __shared__ int nrid[kMaxArms];
 
     // ---- tree choice (pop_planner.cu:366-376)
-    int t = s.round == 0 ? (b < s.B / 2 ? 0 : 1) : s.ex_tree[b];  // first iteration of a launch
-    if (s.round > 0 && s.balance == 2) {
+    int t = round == 0 ? (b < s.B / 2 ? 0 : 1) : s.ex_tree[b];  // first iteration of a launch
+    if (round > 0 && s.balance == 2) {
         const int nt = s.tr_n[t], no = s.tr_n[1 - t];
         const float ratio = (float)abs(nt - no) / (float)nt;
         if (ratio < s.tree_ratio) t = 1 - t;
@@ -387,7 +469,7 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
     const float best = *s.best_cost;
 
     // ---- sample (pop_planner.cu:389-447)
-    const uint4 rr = slot_random(s.seed, kStreamTree, (unsigned)s.rng_round, (unsigned)b, (unsigned)(lane >> 2));
+    const uint4 rr = slot_random(s.seed, kStreamTree, rng_round, (unsigned)b, (unsigned)(lane >> 2));
     const unsigned bits = (lane & 3) == 0 ? rr.x : (lane & 3) == 1 ? rr.y : (lane & 3) == 2 ? rr.z : rr.w;
     const float u = u01(bits);  // (0, 1]
     if (!s.phs || !s.optimize) {
@@ -399,7 +481,7 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
         float z = lane < D ? logf(uc * __frcp_rn(1.0f - uc)) * sqrtf((float)M_PI / 8.0f) : 0.f;
         const float nrm = sqrtf(wave_sum(z * z));
         z = z / nrm;
-        const uint4 rad = slot_random(s.seed, kStreamPhsRadius, (unsigned)s.rng_round, (unsigned)b, 0);
+        const uint4 rad = slot_random(s.seed, kStreamPhsRadius, rng_round, (unsigned)b, 0);
         const float scale = powf(u01(rad.x), 1.0f / (float)D);
         if (lane < D) nq[lane] = z * scale;
         __syncthreads();
@@ -439,7 +521,7 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
         }
         const float g_hat = sqrtf(wave_sum(dt)), h_hat = sqrtf(wave_sum(dd));
         const float c_range = fmaxf(best - (g_hat + h_hat), 0.0f);
-        const uint4 cr = slot_random(s.seed, kStreamCostBound, (unsigned)s.rng_round, (unsigned)b, 0);
+        const uint4 cr = slot_random(s.seed, kStreamCostBound, rng_round, (unsigned)b, 0);
         c_rand = u01(cr.x) * c_range + g_hat;
     }
 
@@ -532,7 +614,10 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
         s.ed_from[(size_t)slot * D + lane] = vn[lane];
         s.ed_to[(size_t)slot * D + lane] = nq[lane];
     }
-    if (lane == 0) s.ed_b[slot] = b;
+    if (lane == 0) {
+        s.ed_b[slot] = b;
+        s.ed_coll[slot] = 0u;  // the checker only sets bits
+    }
 }
 
 // Commit the checked expansions in expansion order (pop_planner.cu:571-622):
@@ -540,7 +625,7 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
 // free inserts get consecutive indices per tree.  One block per slot.
 __global__ __launch_bounds__(256) void k_dr_commit(SlotDev *slots, const int *ids) {
     SlotDev &s = slots[ids[blockIdx.y]];
-    if (*s.status != kRunning) return;
+    if (!searching(s)) return;
     const int tid = threadIdx.x;
     const int ne = *s.ed_count;
     // verdicts back to expansions
@@ -626,7 +711,7 @@ __global__ __launch_bounds__(256) void k_dr_commit(SlotDev *slots, const int *id
 __global__ __launch_bounds__(64) void k_dr_connect(SlotDev *slots, const int *ids) {
     SlotDev &s = slots[ids[blockIdx.y]];
     const int b = blockIdx.x;
-    if (b >= s.B || *s.status != kRunning) return;
+    if (b >= s.B || !searching(s)) return;
     const int lane = threadIdx.x;
     if (lane == 0) s.cn_near[b] = -1;
     const int idx = s.ex_index[b];
@@ -672,7 +757,10 @@ __global__ __launch_bounds__(64) void k_dr_connect(SlotDev *slots, const int *id
             s.ce_to[e * D + lane] = cur;
         }
     }
-    for (int c = lane; c < next; c += 64) s.ce_b[base + c] = b;
+    for (int c = lane; c < next; c += 64) {
+        s.ce_b[base + c] = b;
+        s.ce_coll[base + c] = 0u;  // the checker only sets bits
+    }
 }
 
 // Connections whose chunks are all free add their end node to tree t; the
@@ -680,23 +768,49 @@ __global__ __launch_bounds__(64) void k_dr_connect(SlotDev *slots, const int *id
 // path_processing.cu:10-70).  One block per slot.
 __global__ __launch_bounds__(256) void k_dr_finish(SlotDev *slots, const int *ids) {
     SlotDev &s = slots[ids[blockIdx.y]];
-    if (*s.status != kRunning) return;
+    if (!searching(s)) return;
     const int tid = threadIdx.x;
     const int nce = *s.ce_count;
     for (int e = tid; e < nce; e += 256)
         if (s.ce_coll[e]) s.cn_fail[s.ce_b[e]] = 1;
     __syncthreads();
-    if (tid != 0) return;
-    int winner = -1, wnode = -1;
-    for (int b = 0; b < s.B; ++b) {
+    // successful connections get consecutive indices per tree in expansion order
+    __shared__ int sh0[256], sh1[256];
+    __shared__ int s_winner, s_wnode, s_overflow;
+    if (tid == 0) {
+        s_winner = 0x7fffffff;
+        s_overflow = 0;
+    }
+    const int per = (s.B + 255) / 256;
+    const int b0 = tid * per, b1 = min(b0 + per, s.B);
+    int c0 = 0, c1 = 0;
+    for (int b = b0; b < b1; ++b)
+        if (s.cn_near[b] >= 0 && !s.cn_fail[b]) (s.ex_tree[b] == 0 ? c0 : c1)++;
+    sh0[tid] = c0;
+    sh1[tid] = c1;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int a0 = tid >= off ? sh0[tid - off] : 0, a1 = tid >= off ? sh1[tid - off] : 0;
+        __syncthreads();
+        sh0[tid] += a0;
+        sh1[tid] += a1;
+        __syncthreads();
+    }
+    const int n0 = s.tr_n[0], n1 = s.tr_n[1];
+    int i0 = n0 + sh0[tid] - c0, i1 = n1 + sh1[tid] - c1;
+    int my_first = 0x7fffffff, my_node = -1;  // lowest successful expansion of this thread
+    for (int b = b0; b < b1; ++b) {
         if (s.cn_near[b] < 0 || s.cn_fail[b]) continue;
         const int t = s.ex_tree[b], o = 1 - t;
-        const int idx = s.tr_n[t];
+        const int idx = t == 0 ? i0++ : i1++;
         if (idx >= s.tr_cap) {
-            *s.status = kFailedOverflow;
-            return;
+            s_overflow = 1;
+            continue;
         }
-        s.tr_n[t] = idx + 1;
+        if (my_first == 0x7fffffff) {
+            my_first = b;
+            my_node = idx;
+        }
         // end configuration: the start advanced by n_ext chunks, as in the chunk edges
         const float *st = tree_q(s, t, s.ex_index[b]), *on = tree_q(s, o, s.cn_near[b]);
         float *dst = tree_q(s, t, idx);
@@ -712,11 +826,22 @@ __global__ __launch_bounds__(256) void k_dr_finish(SlotDev *slots, const int *id
         s.tr_parent[(size_t)t * s.tr_cap + idx] = s.ex_index[b];
         // the reference adds one chunk's length here (pop_planner.cu:709-710)
         s.tr_cost[(size_t)t * s.tr_cap + idx] = s.tr_cost[(size_t)t * s.tr_cap + s.ex_index[b]] + s.cn_d[b] / next;
-        if (winner < 0) {
-            winner = b;
-            wnode = idx;
-        }
     }
+    if (my_first != 0x7fffffff) atomicMin(&s_winner, my_first);
+    __syncthreads();
+    if (my_first != 0x7fffffff && my_first == s_winner) s_wnode = my_node;
+    __syncthreads();
+    if (tid != 0) return;
+    s.tr_n[0] = min(n0 + sh0[255], s.tr_cap);
+    s.tr_n[1] = min(n1 + sh1[255], s.tr_cap);
+    s.ctr[kCtrRound] += 1;
+    s.ctr[kCtrRng] += 1;
+    s.ctr[kCtrRun] += 1;
+    if (s_overflow) {
+        *s.status = kFailedOverflow;  // (pop_planner.cu:587-589)
+        return;
+    }
+    const int winner = s_winner == 0x7fffffff ? -1 : s_winner, wnode = s_wnode;
     if (winner < 0) return;
     // ---- trace both trees to their roots; path runs goal -> start
     const int t = s.ex_tree[winner], D = s.D;
@@ -770,7 +895,7 @@ __global__ __launch_bounds__(256) void k_dr_finish(SlotDev *slots, const int *id
     *s.path_cost = cost;
     s.path_meta[0] = s.tr_n[0];
     s.path_meta[1] = s.tr_n[1];
-    s.path_meta[2] = s.round + 1;
+    s.path_meta[2] = s.ctr[kCtrRound];
     *s.best_cost = fminf(*s.best_cost, cost);
     *s.status = winner + 1;
 }
@@ -885,7 +1010,10 @@ __global__ __launch_bounds__(1024) void k_sp_edges(SlotDev *slots, const int *id
                             s.sp_to[(size_t)(e + c) * D + d] = cur;
                         }
                     }
-                    for (int c = 0; c < nc; ++c) s.sp_item[e + c] = item;
+                    for (int c = 0; c < nc; ++c) {
+                        s.sp_item[e + c] = item;
+                        s.sp_coll[e + c] = 0u;  // the checker only sets bits
+                    }
                     e += nc;
                 }
             }
@@ -964,7 +1092,7 @@ struct ReportDev {
     float best_cost;
     int path_meta[3];
     int sp_count;
-    int pad;
+    int ctr[4];
 };
 
 __global__ void k_report(SlotDev *slots, const int *ids, int n, ReportDev *out) {
@@ -985,6 +1113,7 @@ __global__ void k_report(SlotDev *slots, const int *ids, int n, ReportDev *out) 
     r.best_cost = *s.best_cost;
     for (int k = 0; k < 3; ++k) r.path_meta[k] = s.path_meta[k];
     r.sp_count = *s.sp_count;
+    for (int k = 0; k < 4; ++k) r.ctr[k] = s.ctr[k];
 }
 
 // ------------------------------------------------------------ launchers ----
@@ -1008,11 +1137,16 @@ void launch_rm_radius(SlotDev *d, const int *ids, int n, int max_am, hipStream_t
 void launch_rm_compact(SlotDev *d, const int *ids, int n, hipStream_t stream) {
     MRPOP_L(k_rm_compact, dim3(kMaxArms, n), dim3(1024), d, ids);
 }
-void launch_rm_link(SlotDev *d, const int *ids, int n, hipStream_t stream) {
-    MRPOP_L(k_rm_link, dim3(kMaxArms, n), dim3(64), d, ids);
+void launch_rm_link(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream) {
+    MRPOP_L(k_rm_link_own, dim3(max_am, n), dim3(64), d, ids);
+    MRPOP_L(k_rm_link_bits, dim3(max_am, n), dim3(64), d, ids);
+    MRPOP_L(k_rm_link_append, dim3(64, kMaxArms, n), dim3(256), d, ids);
 }
 void launch_rm_connected(SlotDev *d, const int *ids, int n, hipStream_t stream) {
     MRPOP_L(k_rm_connected, dim3(1, n), dim3(64), d, ids);
+}
+void launch_dr_begin(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+    MRPOP_L(k_dr_begin, dim3(n), dim3(64), d, ids);
 }
 void launch_dr_expand(SlotDev *d, const int *ids, int n, int max_b, hipStream_t stream) {
     MRPOP_L(k_dr_expand, dim3(max_b, n), dim3(64), d, ids);
