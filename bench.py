@@ -6,12 +6,16 @@ granularity 32.  One step = one batched collision check of a fixed batch of
 edges resident in HBM (FK + two-phase sphere checks of every interpolated
 sample).  value = collision-checked edges per second on one GPU.
 
-Also reported: the edge-check kernels' roofline (canonical FP32 work per
-sample against the MI355X vector peak) and the CPU baseline (the oracle's
-FP32 restatement of the reference checker, with the reference's early exits,
-timed on a bounded sample of the same edges on this host's cores).
+Also reported: the edge-check kernel's roofline (canonical FP32 work per
+sample against the MI355X vector peak), the CPU baseline (the oracle's FP32
+restatement of the reference checker, with the reference's early exits,
+timed on a bounded sample of the same edges on this host's cores), and the
+planner ("planning"): problems/s, time to first solution (p50/p99), solve
+rate and median cost of the plain form (one problem at a time) against the
+batched form (all problems in one engine), and of a serving run with Poisson
+arrivals and per-request deadlines.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--no-plan]
 """
 from __future__ import annotations
 
@@ -70,6 +74,116 @@ def cpu_baseline(problem, frm, to, G, budget_s=12.0):
             "single_thread_value": rate1}
 
 
+PLAN_KINDS = ("plain2", "table3", "bin4", "shelf5")   # 2 / 3 / 4 / 5 arms, ratio 1:1:1:1
+
+
+def plan_problems(checker, n):
+    """n seeded problems, arm counts 2/3/4/5 in turn (seeds 0, 0, 0, 0, 1, ...)."""
+    from mrpop import scenes
+    return [scenes.make_problem(PLAN_KINDS[i % 4], i // 4, checker) for i in range(n)]
+
+
+def _stats(sols, ttfs, wall):
+    solved = [s for s in sols if s.solved]
+    t = np.array(ttfs, dtype=np.float64)
+    costs = np.array([s.cost for s in solved]) if solved else np.array([np.nan])
+    return {"problems": len(sols), "wall_s": wall, "problems_per_s": len(sols) / wall,
+            "solve_rate": len(solved) / len(sols),
+            "ttfs_p50_s": float(np.percentile(t, 50)) if len(t) else None,
+            "ttfs_p99_s": float(np.percentile(t, 99)) if len(t) else None,
+            "median_cost": float(np.median(costs)),
+            "timeouts": sum(s.info["status"] == 4 for s in sols)}
+
+
+def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_rates, serve_s):
+    """Plain vs batched planning on the same seeded problems, and serving."""
+    from mrpop import _lib
+    probs = plan_problems(checker, max(n_plain, n_batch))
+    common = dict(granularity=32, range=2.0, rrtc_iter=6, optimize_iters=2000)
+    st_plain = _lib.settings(num_new_configs=plain_nnc, **common)
+    st_batch = _lib.settings(num_new_configs=batch_nnc, **common)
+    op = _lib.solve_options_default(time_budget_s=30.0)
+    out = {"settings": dict(common, plain_num_new_configs=plain_nnc, batch_num_new_configs=batch_nnc,
+                            time_budget_s=30.0),
+           "problems": "seeded mixed 2/3/4/5-arm scenes (random cuboids, 3-arm table, 4-arm bin, 5-arm shelf), "
+                       "one start and one goal each"}
+    # plain: one problem at a time, all submitted at t = 0 (TTFS includes the queue)
+    plain_sols, plain_ttfs, own_ttfs = [], [], []
+    t0 = time.perf_counter()
+    for p in probs[:n_plain]:
+        start = time.perf_counter() - t0
+        s = _lib.solve(ctx, p, st_plain, op)
+        plain_sols.append(s)
+        if s.solved:
+            plain_ttfs.append(start + s.ttfs_s)
+            own_ttfs.append(s.ttfs_s)
+    plain = _stats(plain_sols, plain_ttfs, time.perf_counter() - t0)
+    plain["ttfs_own_p50_s"] = float(np.percentile(own_ttfs, 50)) if own_ttfs else None
+    plain["ttfs_own_p99_s"] = float(np.percentile(own_ttfs, 99)) if own_ttfs else None
+    print(f"[bench] plain: {plain}", file=sys.stderr, flush=True)
+    # batched: every problem in one engine, all submitted at t = 0
+    t0 = time.perf_counter()
+    batch_sols = _lib.solve_batch(ctx, probs[:n_batch], [st_batch] * n_batch, [op] * n_batch)
+    wall = time.perf_counter() - t0
+    batched = _stats(batch_sols, [s.ttfs_s for s in batch_sols if s.solved], wall)
+    both = [(a, b) for a, b in zip(plain_sols, batch_sols) if a.solved and b.solved]
+    batched["median_cost_on_plain_problems"] = float(np.median([b.cost for _, b in both])) if both else None
+    plain["median_cost_on_plain_problems"] = float(np.median([a.cost for a, _ in both])) if both else None
+    print(f"[bench] batched: {batched}", file=sys.stderr, flush=True)
+    out.update(plain=plain, batched=batched,
+               batched_vs_plain_problems_per_s=batched["problems_per_s"] / plain["problems_per_s"])
+    out["serving"] = [serving_bench(ctx, checker, rate, serve_s, st_batch) for rate in serve_rates]
+    return out
+
+
+def serving_bench(ctx, checker, rate, duration_s, settings, deadlines=(0.05, 0.25, 1.0), seed=7):
+    """Poisson arrivals at `rate` req/s for `duration_s`; each request has a
+    budget of the last deadline.  Reports TTFS from arrival and the cost of
+    the best solution available at each deadline."""
+    from mrpop import _lib
+    rng = np.random.default_rng(seed)
+    n = max(1, int(rate * duration_s))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    probs = plan_problems(checker, n)
+    op = _lib.solve_options_default(time_budget_s=deadlines[-1])
+    srv = _lib.Server(ctx, hbm_budget_bytes=200 * 10**9)
+    tickets, done = [], {}
+    t0 = time.perf_counter()
+    i = 0
+    while len(done) < n:
+        now = time.perf_counter() - t0
+        while i < n and arrivals[i] <= now:
+            tickets.append(srv.submit(probs[i], settings, op))
+            i += 1
+        active, waiting = srv.step()
+        if active == 0 and waiting == 0 and i < n:
+            time.sleep(max(0.0, arrivals[i] - (time.perf_counter() - t0)))
+        for k, t in enumerate(tickets):
+            if t in done:
+                continue
+            state, sol = srv.poll(t, snapshot=False)
+            if state == 2:
+                done[t] = srv.poll(t, snapshot=True)[1]
+    wall = time.perf_counter() - t0
+    sols = [done[t] for t in tickets]
+    ttfs = [s.ttfs_s for s in sols if s.solved]
+    res = {"rate_req_s": rate, "requests": n, "wall_s": wall, "throughput_req_s": n / wall,
+           "solve_rate": len(ttfs) / n,
+           "ttfs_p50_s": float(np.percentile(ttfs, 50)) if ttfs else None,
+           "ttfs_p99_s": float(np.percentile(ttfs, 99)) if ttfs else None}
+    for d in deadlines:
+        costs = []
+        for s in sols:
+            c = [cost for (cost, _), t in zip(s.seed_paths, s.improve_s) if t <= d]
+            if c:
+                costs.append(min(c))
+        res[f"solved_by_{int(d * 1000)}ms"] = len(costs) / n
+        res[f"median_cost_at_{int(d * 1000)}ms"] = float(np.median(costs)) if costs else None
+    srv.close()
+    print(f"[bench] serving: {res}", file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +192,13 @@ def main():
     ap.add_argument("--edges", type=int, default=65536)
     ap.add_argument("--granularity", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plan", action="store_true", help="skip the planner measurements")
+    ap.add_argument("--plain-problems", type=int, default=8)
+    ap.add_argument("--batch-problems", type=int, default=64)
+    ap.add_argument("--plain-nnc", type=int, default=1024)
+    ap.add_argument("--batch-nnc", type=int, default=256)
+    ap.add_argument("--serve-rates", type=str, default="20,50")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: problems are independent and this build runs on a single MI355X; "
@@ -149,11 +270,15 @@ def main():
                    "obstacles": sc.n_obstacles, "colliding_fraction": round(colliding, 4)},
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "coarse_kernel<4,FULL> + fine_kernel<4,FULL> (one edge check)",
+                     "kernel": "check_kernel<4, FULL> (fused coarse + fine edge check)",
                      "ms_per_launch": ms_events, "flops_per_sample": w_sample},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(problem, frm, to, G)
+    if not args.no_plan:
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        out["planning"] = planning_bench(ctx, checker, args.plain_problems, args.batch_problems, args.plain_nnc,
+                                         args.batch_nnc, rates, args.serve_seconds)
     print(json.dumps(out), flush=True)
 
 

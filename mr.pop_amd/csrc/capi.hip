@@ -41,6 +41,156 @@ bool is_world_axes(const float *rec) {
 }  // namespace
 }  // namespace mrpop
 
+namespace mrpop {
+
+namespace {
+
+// Box of a segment swept by a ball of radius r.
+void seg_box(const float *p, const float *q, float r, float *box) {
+    for (int i = 0; i < 3; ++i) {
+        box[i] = std::min(p[i], q[i]) - r;
+        box[3 + i] = std::max(p[i], q[i]) + r;
+    }
+}
+
+// Box of {c + d : |A d|_k <= e_k for k = 0..2} where the rows of A are the
+// test's axes (sphere_obb / sphere_zcuboid): |d_i| <= sum_k |A^-1[i][k]| e_k.
+// *f receives max_i sum_k |A^-1[i][k]|, the growth of the box per unit of
+// sphere radius.  A singular A gives an unbounded box (no culling).
+void axes_box(const float *c, const double A[3][3], const double e[3], float *box, float *f) {
+    const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                       A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                       A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    if (!(std::fabs(det) > 1e-9)) {
+        for (int i = 0; i < 3; ++i) {
+            box[i] = -3.0e38f;
+            box[3 + i] = 3.0e38f;
+        }
+        *f = 0.f;
+        return;
+    }
+    double inv[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const int i1 = (j + 1) % 3, i2 = (j + 2) % 3, j1 = (i + 1) % 3, j2 = (i + 2) % 3;
+            inv[i][j] = (A[i1][j1] * A[i2][j2] - A[i1][j2] * A[i2][j1]) / det;  // adjugate / det
+        }
+    double fmax = 0;
+    for (int i = 0; i < 3; ++i) {
+        double ext = 0, row = 0;
+        for (int k = 0; k < 3; ++k) {
+            ext += std::fabs(inv[i][k]) * e[k];
+            row += std::fabs(inv[i][k]);
+        }
+        fmax = std::max(fmax, row);
+        box[i] = (float)(c[i] - ext);
+        box[3 + i] = (float)(c[i] + ext);
+    }
+    *f = (float)(fmax * (1.0 + 1e-6));
+}
+
+}  // namespace
+
+long pack_world(const mrpop_scene &s, const float *b, int n_arms, dev::WorldDesc &d, std::vector<float> &host) {
+    const int counts[5] = {s.n_spheres, s.n_capsules, s.n_z_capsules, s.n_cuboids, s.n_z_cuboids};
+    long total = 0;
+    for (int k = 0; k < 5; ++k) {
+        if (counts[k] < 0) return -1;
+        total += counts[k];
+    }
+    // split cuboids into world-axis-aligned boxes and general ones
+    std::vector<float> aabb, obb;
+    for (int i = 0; i < s.n_cuboids; ++i) {
+        const float *r = s.cuboids + 15 * i;
+        if (is_world_axes(r)) {
+            aabb.insert(aabb.end(), {r[0], r[1], r[2], r[12], r[13], r[14]});
+        } else {
+            obb.insert(obb.end(), r, r + 15);
+        }
+    }
+    host.clear();
+    auto put = [&](const float *p, size_t n) { host.insert(host.end(), p, p + n); };
+    d.o_sph = (int)host.size();
+    d.n_sph = s.n_spheres;
+    if (d.n_sph) put(s.spheres, 4 * (size_t)d.n_sph);
+    d.o_cap = (int)host.size();
+    d.n_cap = s.n_capsules;
+    if (d.n_cap) put(s.capsules, 8 * (size_t)d.n_cap);
+    d.o_zcap = (int)host.size();
+    d.n_zcap = s.n_z_capsules;
+    if (d.n_zcap) put(s.z_capsules, 8 * (size_t)d.n_zcap);
+    d.o_aabb = (int)host.size();
+    d.n_aabb = (int)(aabb.size() / 6);
+    put(aabb.data(), aabb.size());
+    d.o_obb = (int)host.size();
+    d.n_obb = (int)(obb.size() / 15);
+    put(obb.data(), obb.size());
+    d.o_zcub = (int)host.size();
+    d.n_zcub = s.n_z_cuboids;
+    if (d.n_zcub) put(s.z_cuboids, 15 * (size_t)d.n_zcub);
+    d.o_base = (int)host.size();
+    put(b, 12 * (size_t)n_arms);
+    d.n_arms = n_arms;
+    // cull boxes with a margin far above the tests' rounding
+    d.o_box = (int)host.size();
+    float box[8];
+    auto emit = [&](float f) {
+        for (int i = 0; i < 3; ++i) {
+            const float m = 1e-4f + 1e-6f * std::max(std::fabs(box[i]), std::fabs(box[3 + i]));
+            box[i] -= m;
+            box[3 + i] += m;
+        }
+        box[6] = f;
+        box[7] = 0.f;
+        put(box, 8);
+    };
+    for (int i = 0; i < d.n_sph; ++i) {
+        const float *o = s.spheres + 4 * i;
+        seg_box(o, o, std::fabs(o[3]), box);
+        emit(1.f);
+    }
+    for (int i = 0; i < d.n_cap; ++i) {  // segment p1 .. p1 + v (utils.cuh:148-158 clamps t to [0, 1])
+        const float *o = s.capsules + 8 * i;
+        const float q[3] = {o[0] + o[3], o[1] + o[4], o[2] + o[5]};
+        seg_box(o, q, std::fabs(o[6]), box);
+        emit(1.f);
+    }
+    for (int i = 0; i < d.n_zcap; ++i) {
+        const float *o = s.z_capsules + 8 * i;
+        const float q[3] = {o[0], o[1], o[2] + o[5]};
+        seg_box(o, q, std::fabs(o[6]), box);
+        emit(1.f);
+    }
+    for (size_t i = 0; i < aabb.size() / 6; ++i) {
+        const float *o = aabb.data() + 6 * i;
+        for (int k = 0; k < 3; ++k) {
+            box[k] = o[k] - std::fabs(o[3 + k]);
+            box[3 + k] = o[k] + std::fabs(o[3 + k]);
+        }
+        emit(1.f);
+    }
+    for (size_t i = 0; i < obb.size() / 15; ++i) {
+        const float *o = obb.data() + 15 * i;
+        const double A[3][3] = {{o[3], o[4], o[5]}, {o[6], o[7], o[8]}, {o[9], o[10], o[11]}};
+        const double e[3] = {std::fabs((double)o[12]), std::fabs((double)o[13]), std::fabs((double)o[14])};
+        float f;
+        axes_box(o, A, e, box, &f);
+        emit(f);
+    }
+    for (int i = 0; i < d.n_zcub; ++i) {
+        const float *o = s.z_cuboids + 15 * i;
+        const double A[3][3] = {{o[3], o[4], 0.0}, {o[6], o[7], 0.0}, {0.0, 0.0, 1.0}};
+        const double e[3] = {std::fabs((double)o[12]), std::fabs((double)o[13]), std::fabs((double)o[14])};
+        float f;
+        axes_box(o, A, e, box, &f);
+        emit(f);
+    }
+    return total;
+}
+
+}  // namespace mrpop
+
+
 using namespace mrpop;
 
 struct mrpop_ctx : Context {};
@@ -146,42 +296,11 @@ int mrpop_world_create(mrpop_ctx *c, const mrpop_scene *s, int n_arms, const flo
         if (n_arms < 2) return c->fail(MRPOP_ERR_INVALID, "a 1-arm world needs explicit base poses");
         b = &geom::default_bases(n_arms)[0][0];
     }
-    // split cuboids into world-axis-aligned boxes and general ones
-    std::vector<float> aabb, obb;
-    for (int i = 0; i < s->n_cuboids; ++i) {
-        const float *r = s->cuboids + 15 * i;
-        if (is_world_axes(r)) {
-            aabb.insert(aabb.end(), {r[0], r[1], r[2], r[12], r[13], r[14]});
-        } else {
-            obb.insert(obb.end(), r, r + 15);
-        }
-    }
     std::vector<float> host;
     auto *w = new (std::nothrow) mrpop_world();
     if (!w) return MRPOP_ERR_BUDGET;
     dev::WorldDesc &d = w->desc;
-    auto put = [&](const float *p, size_t n) { host.insert(host.end(), p, p + n); };
-    d.o_sph = (int)host.size();
-    d.n_sph = s->n_spheres;
-    if (d.n_sph) put(s->spheres, 4 * (size_t)d.n_sph);
-    d.o_cap = (int)host.size();
-    d.n_cap = s->n_capsules;
-    if (d.n_cap) put(s->capsules, 8 * (size_t)d.n_cap);
-    d.o_zcap = (int)host.size();
-    d.n_zcap = s->n_z_capsules;
-    if (d.n_zcap) put(s->z_capsules, 8 * (size_t)d.n_zcap);
-    d.o_aabb = (int)host.size();
-    d.n_aabb = (int)(aabb.size() / 6);
-    put(aabb.data(), aabb.size());
-    d.o_obb = (int)host.size();
-    d.n_obb = (int)(obb.size() / 15);
-    put(obb.data(), obb.size());
-    d.o_zcub = (int)host.size();
-    d.n_zcub = s->n_z_cuboids;
-    if (d.n_zcub) put(s->z_cuboids, 15 * (size_t)d.n_zcub);
-    d.o_base = (int)host.size();
-    put(b, 12 * (size_t)n_arms);
-    d.n_arms = n_arms;
+    pack_world(*s, b, n_arms, d, host);
     w->ctx = c;
     w->n_arms = n_arms;
     std::memcpy(w->bases, b, sizeof(float) * 12 * n_arms);
@@ -295,4 +414,21 @@ extern "C" int mrpop_diag_check_edges(mrpop_ctx *c, const mrpop_world *w, int mo
     if (rc == MRPOP_OK) rc = hipMemcpy(out_diag, d, words * 4, hipMemcpyDeviceToHost) == hipSuccess ? MRPOP_OK : MRPOP_ERR_HIP;
     (void)hipFree(d);
     return rc;
+}
+
+// Diagnostics (not part of the public ABI): the cull boxes the checker uses
+// for a scene, 8 floats per obstacle in the checker's order (spheres,
+// capsules, z-capsules, world-axis cuboids, other cuboids, z-cuboids).
+// Returns the number of obstacles, or -1.
+extern "C" int mrpop_diag_world_boxes(const mrpop_scene *s, float *out, int cap) {
+    if (!s) return -1;
+    std::vector<float> host;
+    mrpop::dev::WorldDesc d{};
+    static const float bases[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    const long n = mrpop::pack_world(*s, bases, 1, d, host);
+    if (n < 0) return -1;
+    if (out)
+        for (long i = 0; i < n && i < cap; ++i)
+            for (int k = 0; k < 8; ++k) out[8 * i + k] = host[d.o_box + 8 * i + k];
+    return (int)n;
 }

@@ -152,7 +152,7 @@ __constant__ JointTable kJT = build_joint_table();
 // Frame of joint `jt` (runtime 0..7) of an arm at sample (e, k), walked with a
 // rolled loop.  Every operation matches joint_step on the same inputs (fma
 // with a 0 / +-1 factor is exact), so the frame equals the unrolled one.
-__device__ __noinline__ Frame frame_rt(const WorldDesc &w, const EdgeBatch &eb, int base_arm, int dof_off, int e,
+__device__ __forceinline__ Frame frame_rt(const WorldDesc &w, const EdgeBatch &eb, int base_arm, int dof_off, int e,
                                        int k, int jt) {
     Frame T;
     load_base(w, base_arm, T);
@@ -197,7 +197,7 @@ __device__ __forceinline__ void centre_rt(const Frame &T, int s, float &x, float
 // ------------------------------------------------------------ fused check --
 // Fine environment tests of block k's spheres (all but sphere 0) in frame T,
 // only against obstacles that its bounding ball (centre b, radius R) touches.
-__device__ __noinline__ bool block_env(const WorldDesc &w, const Frame &T, int k, float bx, float by, float bz,
+__device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, int k, float bx, float by, float bz,
                                           float R) {
     bool hit = false;
     const int j0 = max((int)kTab.block_first[k], 1), j1 = kTab.block_first[k] + kTab.block_count[k];
@@ -352,6 +352,14 @@ template <int A> __device__ __forceinline__ void half_pair(int h, int &a, int &b
     i0 = (h & 1) ? 6 : 0;
 }
 
+// Largest radius among the environment-tested coarse spheres 1..10.
+constexpr float env_radius_max() {
+    float r = 0.f;
+    for (int s = 1; s < kCoarseSpheres; ++s) r = kCoarseSphere[s][3] > r ? kCoarseSphere[s][3] : r;
+    return r;
+}
+constexpr float kEnvRadiusMax = env_radius_max();
+
 template <int A, int MODE>
 __global__ __launch_bounds__(64) void check_kernel(const WorldDesc *worlds, const EdgeGroup *groups, EdgeGroup g0,
                                                    int n_groups, const unsigned *wave_start, FineQueue fq) {
@@ -425,33 +433,60 @@ __global__ __launch_bounds__(64) void check_kernel(const WorldDesc *worlds, cons
     // coarse environment: spheres 1..10 -> env bit of their joint (panda_four.cuh:241-261)
     unsigned hitmask = 0;
     if constexpr (kEnv) {
+        // Box of this lane's tested centres (spheres 1..10): an obstacle whose
+        // cull box, grown by the largest of their radii, misses it cannot hit
+        // any of them.
+        float lx = cx[1], ly = cy[1], lz = cz[1], hx = cx[1], hy = cy[1], hz = cz[1];
+#pragma unroll
+        for (int s = 2; s < kCoarseSpheres; ++s) {
+            lx = fminf(lx, cx[s]);
+            ly = fminf(ly, cy[s]);
+            lz = fminf(lz, cz[s]);
+            hx = fmaxf(hx, cx[s]);
+            hy = fmaxf(hy, cy[s]);
+            hz = fmaxf(hz, cz[s]);
+        }
+        const float *box = w.obs + w.o_box;
+        auto near = [&](int ob) {
+            const float *bb = box + 8 * ob;
+            const float g = kEnvRadiusMax * bb[6];
+            return !(hx < bb[0] - g || hy < bb[1] - g || hz < bb[2] - g || lx > bb[3] + g || ly > bb[4] + g ||
+                     lz > bb[5] + g);
+        };
         auto test_all = [&](auto &&fn) {
 #pragma unroll
             for (int s = 1; s < kCoarseSpheres; ++s)
                 if (fn(cx[s], cy[s], cz[s], kCoarseSphere[s][3]) < 0.f) hitmask |= 1u << s;
         };
-        for (int i = 0; i < w.n_sph; ++i) {
+        int ob = 0;
+        for (int i = 0; i < w.n_sph; ++i, ++ob) {
+            if (!near(ob)) continue;
             const float *o = w.obs + w.o_sph + 4 * i;
             const float ox = o[0], oy = o[1], oz = o[2], orr = o[3];
             test_all([&](float x, float y, float z, float r) { return sphere_sphere(ox, oy, oz, orr, x, y, z, r); });
         }
-        for (int i = 0; i < w.n_cap; ++i) {
+        for (int i = 0; i < w.n_cap; ++i, ++ob) {
+            if (!near(ob)) continue;
             const float *o = w.obs + w.o_cap + 8 * i;
             test_all([&](float x, float y, float z, float r) { return sphere_capsule(o, x, y, z, r); });
         }
-        for (int i = 0; i < w.n_zcap; ++i) {
+        for (int i = 0; i < w.n_zcap; ++i, ++ob) {
+            if (!near(ob)) continue;
             const float *o = w.obs + w.o_zcap + 8 * i;
             test_all([&](float x, float y, float z, float r) { return sphere_zcapsule(o, x, y, z, r); });
         }
-        for (int i = 0; i < w.n_aabb; ++i) {
+        for (int i = 0; i < w.n_aabb; ++i, ++ob) {
+            if (!near(ob)) continue;
             const float *o = w.obs + w.o_aabb + 6 * i;
             test_all([&](float x, float y, float z, float r) { return sphere_aabb(o, x, y, z, r * r); });
         }
-        for (int i = 0; i < w.n_obb; ++i) {
+        for (int i = 0; i < w.n_obb; ++i, ++ob) {
+            if (!near(ob)) continue;
             const float *o = w.obs + w.o_obb + 15 * i;
             test_all([&](float x, float y, float z, float r) { return sphere_obb(o, x, y, z, r * r); });
         }
-        for (int i = 0; i < w.n_zcub; ++i) {
+        for (int i = 0; i < w.n_zcub; ++i, ++ob) {
+            if (!near(ob)) continue;
             const float *o = w.obs + w.o_zcub + 15 * i;
             test_all([&](float x, float y, float z, float r) { return sphere_zcuboid(o, x, y, z, r * r); });
         }

@@ -85,6 +85,12 @@ struct WorldDesc {
     int n_sph, n_cap, n_zcap, n_aabb, n_obb, n_zcub;
     int o_sph, o_cap, o_zcap, o_aabb, o_obb, o_zcub, o_base;  // float offsets into obs
     int n_arms;
+    // cull boxes: [lo xyz, hi xyz, f, 0] per obstacle, in the order
+    // spheres, capsules, z-capsules, aabbs, obbs, z-cuboids.  Every centre
+    // of a sphere of radius r that the obstacle's test reports as colliding
+    // lies inside the box grown by r * f on every side (pack_world builds
+    // them).
+    int o_box;
 };
 
 // utils.cuh:100-107

@@ -12,6 +12,7 @@
 #pragma once
 
 #include <chrono>
+#include <map>
 #include <cstdint>
 #include <vector>
 
@@ -76,6 +77,7 @@ struct Slot {
     std::chrono::steady_clock::time_point t_submit, t_kernel_start;
     SlotResult res;
     void *user = nullptr;
+    int lane = -1;  // lane running this slot's work, -1 none
 };
 
 // Per-slot report copied back after each step.
@@ -93,6 +95,11 @@ struct SlotReport {
 };
 constexpr int plan_ctr_round = kCtrRound, plan_ctr_run = kCtrRun;
 
+// Work runs on two lanes, each a HIP stream with its own staging: the
+// roadmap lane grows the per-arm roadmaps of slots in the Roadmap phase, the
+// search lane runs dRRT rounds and simplification stages.  A slot is on at
+// most one lane at a time; its state machine advances when that lane's work
+// completes, so a long roadmap round does not hold back the searches.
 class Engine {
   public:
     explicit Engine(Context *c) : c_(c) {}
@@ -101,9 +108,9 @@ class Engine {
     int add(const mrpop_scene &scene, int n_arms, const float *bases, const float *start, int n_goals,
             const float *goals, unsigned long long seed, const mrpop_settings &st, const SolveOptions &opt,
             Slot **out);
-    // Release a slot's device memory (after it is done).
+    // Release a slot's device memory (waits for its lane if it is running).
     void release(Slot *s);
-    // One round for every active slot.  Returns MRPOP_OK or an error.
+    // Launch work on idle lanes, wait for one lane and advance its slots.
     int step();
     bool idle() const;
     size_t slot_bytes(int n_arms, const mrpop_settings &st, const SolveOptions &opt) const;
@@ -111,10 +118,6 @@ class Engine {
     size_t device_bytes() const { return bytes_in_use_; }
 
   private:
-    int ensure_capacity(int n);
-    int finish_pass(Slot *s, const SlotReport &r);
-    int start_pass(Slot *s);
-    void compute_phs(Slot *s);
     struct Staged {
         int mode = 0;
         struct Bucket {
@@ -125,20 +128,47 @@ class Engine {
         };
         std::vector<Bucket> buckets;
     };
-    int stage_groups(int mode, const std::vector<const dev::EdgeGroup *> &groups_by_slot, int per_slot,
-                     const std::vector<Slot *> &list, size_t &cursor, Staged &out);
-    int launch_staged(const Staged &st);
+    struct Lane {
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+        bool busy = false;
+        std::vector<Slot *> table;  // slot of each entry of d_slots / the report
+        std::vector<Slot *> rm, dr, sp;
+        SlotDev *d_slots = nullptr, *h_slots = nullptr;
+        int *d_ids = nullptr, *h_ids = nullptr;
+        size_t id_cursor = 0;
+        dev::EdgeGroup *d_groups = nullptr, *h_groups = nullptr;
+        size_t group_cursor = 0;
+        unsigned *d_wave = nullptr;
+        SlotReport *d_report = nullptr, *h_report = nullptr;
+    };
+    int ensure_capacity(int n);
+    int ensure_lanes();
+    int finish_pass(Slot *s, const SlotReport &r);
+    int start_pass(Slot *s);
+    void compute_phs(Slot *s);
+    int launch_lane(Lane &L);
+    int process_lane(Lane &L);
+    int drain();
+    int *upload_ids(Lane &L, const std::vector<Slot *> &list);
+    int stage_groups(Lane &L, int mode, const std::vector<const dev::EdgeGroup *> &groups_by_slot, int per_slot,
+                     const std::vector<Slot *> &list, Staged &out);
+    int launch_staged(Lane &L, const Staged &st);
+    int run_aux(Slot *s, void (*launch)(SlotDev *, const int *, int, hipStream_t));
+    void release_pool();
     Context *c_;
     std::vector<Slot *> slots_;
     int cap_ = 0;
-    SlotDev *d_slots_ = nullptr, *h_slots_ = nullptr;
+    Lane lanes_[2];  // 0: roadmap rounds, 1: dRRT rounds and simplification
     dev::WorldDesc *d_worlds_ = nullptr, *h_worlds_ = nullptr;
-    int *d_ids_ = nullptr, *h_ids_ = nullptr;
-    dev::EdgeGroup *d_groups_ = nullptr, *h_groups_ = nullptr;
-    unsigned *d_wave_ = nullptr;
-    SlotReport *d_report_ = nullptr, *h_report_ = nullptr;
+    SlotDev *d_aux_slot_ = nullptr, *h_aux_slot_ = nullptr;  // one-slot table for set-up kernels
+    int *d_aux_id_ = nullptr;
     size_t group_cap_ = 0;
     size_t bytes_in_use_ = 0;
+    // released slot blocks by size, reused by later problems (serving)
+    static constexpr size_t kPoolBytes = size_t(16) << 30;
+    std::multimap<size_t, void *> pool_;
+    size_t pool_bytes_ = 0;
 };
 
 }  // namespace mrpop

@@ -15,15 +15,11 @@ constexpr int kRoadmapIterPerBlock = 230;  // ROADMAP_BUILD_ITER (roadmap_interi
 constexpr int kPathCap = 2048;             // waypoints (MAX_PATH_SIZE floats in the reference)
 constexpr int kMaxConnectChunks = 64;      // chunks of one connection attempt
 constexpr float kFltMax = 3.402823466e38f;
+// obstacle records (at most 15 floats), their cull boxes (8), 5 base poses
+constexpr size_t kWorldFloats = (size_t)MRPOP_MAX_OBSTACLES * (15 + 8) + 5 * 12 + 64;
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-bool is_world_axes(const float *rec) {
-    static const float I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    for (int i = 0; i < 9; ++i)
-        if (rec[3 + i] != I[i]) return false;
-    return true;
-}
 
 uint64_t ns_since(std::chrono::steady_clock::time_point t) {
     return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t)
@@ -45,14 +41,24 @@ struct Layout {
 }  // namespace
 
 Engine::~Engine() {
+    for (Lane &L : lanes_)
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
     for (Slot *s : slots_) {
         if (s->mem) (void)hipFree(s->mem);
         delete s;
     }
-    for (void *p : {(void *)d_slots_, (void *)d_worlds_, (void *)d_ids_, (void *)d_groups_, (void *)d_wave_,
-                    (void *)d_report_})
+    for (Lane &L : lanes_) {
+        for (void *p : {(void *)L.d_slots, (void *)L.d_ids, (void *)L.d_groups, (void *)L.d_wave, (void *)L.d_report})
+            if (p) (void)hipFree(p);
+        for (void *p : {(void *)L.h_slots, (void *)L.h_ids, (void *)L.h_groups, (void *)L.h_report})
+            if (p) (void)hipHostFree(p);
+        if (L.done) (void)hipEventDestroy(L.done);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+    }
+    for (auto &kv : pool_) (void)hipFree(kv.second);
+    for (void *p : {(void *)d_worlds_, (void *)d_aux_slot_, (void *)d_aux_id_})
         if (p) (void)hipFree(p);
-    for (void *p : {(void *)h_slots_, (void *)h_worlds_, (void *)h_ids_, (void *)h_groups_, (void *)h_report_})
+    for (void *p : {(void *)h_worlds_, (void *)h_aux_slot_})
         if (p) (void)hipHostFree(p);
 }
 
@@ -62,42 +68,56 @@ bool Engine::idle() const {
     return true;
 }
 
+int Engine::ensure_lanes() {
+    if (lanes_[0].stream) return MRPOP_OK;
+    for (Lane &L : lanes_) {
+        // default priority: only the stream's blocking behaviour is set
+        MRPOP_HIP(c_, hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+        MRPOP_HIP(c_, hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+    }
+    MRPOP_HIP(c_, hipMalloc(&d_aux_slot_, sizeof(SlotDev)));
+    MRPOP_HIP(c_, hipHostMalloc(&h_aux_slot_, sizeof(SlotDev), hipHostMallocDefault));
+    MRPOP_HIP(c_, hipMalloc(&d_aux_id_, sizeof(int)));
+    MRPOP_HIP(c_, hipMemsetAsync(d_aux_id_, 0, sizeof(int), c_->stream));
+    return MRPOP_OK;
+}
+
+// Lane tables hold every slot at most once; they are reallocated only when
+// no lane is running.
 int Engine::ensure_capacity(int n) {
+    if (int rc = ensure_lanes()) return rc;
     if (n <= cap_) return MRPOP_OK;
-    int cap = std::max(n, cap_ ? cap_ * 2 : 16);
-    MRPOP_HIP(c_, hipStreamSynchronize(c_->stream));
-    auto grow = [&](void **d, void **h, size_t elem, size_t count, bool host) -> int {
+    if (int rc = drain()) return rc;
+    const int cap = std::max(n, cap_ ? cap_ * 2 : 16);
+    auto realloc_pair = [&](void **d, void **h, size_t bytes, size_t keep) -> int {
         void *nd = nullptr, *nh = nullptr;
-        MRPOP_HIP(c_, hipMalloc(&nd, elem * count));
-        if (host) MRPOP_HIP(c_, hipHostMalloc(&nh, elem * count, hipHostMallocDefault));
+        MRPOP_HIP(c_, hipMalloc(&nd, bytes));
+        if (h) MRPOP_HIP(c_, hipHostMalloc(&nh, bytes, hipHostMallocDefault));
         if (*d) {
-            MRPOP_HIP(c_, hipMemcpy(nd, *d, elem * cap_, hipMemcpyDeviceToDevice));
+            if (keep) MRPOP_HIP(c_, hipMemcpy(nd, *d, keep, hipMemcpyDeviceToDevice));
             (void)hipFree(*d);
         }
-        if (host && *h) {
-            std::memcpy(nh, *h, elem * cap_);
+        if (h && *h) {
+            if (keep) std::memcpy(nh, *h, keep);
             (void)hipHostFree(*h);
         }
         *d = nd;
-        if (host) *h = nh;
+        if (h) *h = nh;
         return MRPOP_OK;
     };
-    if (int rc = grow((void **)&d_slots_, (void **)&h_slots_, sizeof(SlotDev), cap, true)) return rc;
-    if (int rc = grow((void **)&d_worlds_, (void **)&h_worlds_, sizeof(dev::WorldDesc), cap, true)) return rc;
-    if (int rc = grow((void **)&d_report_, (void **)&h_report_, sizeof(SlotReport), cap, true)) return rc;
-    // slot-id lists of one step (each launch sequence appends its own list)
-    if (d_ids_) (void)hipFree(d_ids_);
-    if (h_ids_) (void)hipHostFree(h_ids_);
-    MRPOP_HIP(c_, hipMalloc(&d_ids_, sizeof(int) * cap * 32));
-    MRPOP_HIP(c_, hipHostMalloc(&h_ids_, sizeof(int) * cap * 32, hipHostMallocDefault));
-    // edge groups: per slot at most 2*5 roadmap + dr + cn + sp, twice over for safety
+    if (int rc = realloc_pair((void **)&d_worlds_, (void **)&h_worlds_, sizeof(dev::WorldDesc) * cap,
+                              sizeof(dev::WorldDesc) * cap_))
+        return rc;
+    // per slot at most 2 * 5 roadmap groups, or dr + cn + sp; twice over for safety
     group_cap_ = (size_t)cap * 32;
-    if (d_groups_) (void)hipFree(d_groups_);
-    if (h_groups_) (void)hipHostFree(h_groups_);
-    if (d_wave_) (void)hipFree(d_wave_);
-    MRPOP_HIP(c_, hipMalloc(&d_groups_, sizeof(dev::EdgeGroup) * group_cap_));
-    MRPOP_HIP(c_, hipHostMalloc(&h_groups_, sizeof(dev::EdgeGroup) * group_cap_, hipHostMallocDefault));
-    MRPOP_HIP(c_, hipMalloc(&d_wave_, sizeof(unsigned) * (group_cap_ + 1)));
+    for (Lane &L : lanes_) {
+        if (int rc = realloc_pair((void **)&L.d_slots, (void **)&L.h_slots, sizeof(SlotDev) * cap, 0)) return rc;
+        if (int rc = realloc_pair((void **)&L.d_report, (void **)&L.h_report, sizeof(SlotReport) * cap, 0)) return rc;
+        if (int rc = realloc_pair((void **)&L.d_ids, (void **)&L.h_ids, sizeof(int) * cap * 32, 0)) return rc;
+        if (int rc = realloc_pair((void **)&L.d_groups, (void **)&L.h_groups, sizeof(dev::EdgeGroup) * group_cap_, 0))
+            return rc;
+        if (int rc = realloc_pair((void **)&L.d_wave, nullptr, sizeof(unsigned) * (group_cap_ + 1), 0)) return rc;
+    }
     cap_ = cap;
     return MRPOP_OK;
 }
@@ -112,7 +132,7 @@ size_t Engine::slot_bytes(int A, const mrpop_settings &st, const SolveOptions &o
     const size_t ce_cap = (size_t)B * kMaxConnectChunks;
     const size_t sp_cap = (size_t)kPathCap * 40;
     Layout L;
-    L.add(sizeof(float) * (MRPOP_MAX_OBSTACLES * 15 + 64));                 // world
+    L.add(sizeof(float) * kWorldFloats);                                    // world
     L.add(sizeof(float) * (1 + kMaxGoals) * D);                             // sg
     L.add(sizeof(float) * A * rm_cap * 8);                                  // rm_q
     L.add(sizeof(int) * (size_t)A * rm_cap * K);                            // rm_nbr
@@ -220,7 +240,15 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     const int tr_cap = opt.tree_capacity > 0 ? opt.tree_capacity : std::min(st.max_samples, 262144);
     const size_t ce_cap = (size_t)B * kMaxConnectChunks, sp_cap = (size_t)kPathCap * 40;
     s->mem_bytes = slot_bytes(A, st, opt);
-    if (hipMalloc(&s->mem, s->mem_bytes) != hipSuccess) {
+    // reuse a released block of the same size (problems with the same arm
+    // count and settings have identical layouts), else allocate
+    auto pooled = pool_.find(s->mem_bytes);
+    if (pooled != pool_.end()) {
+        s->mem = pooled->second;
+        pool_bytes_ -= s->mem_bytes;
+        pool_.erase(pooled);
+    } else if (hipMalloc(&s->mem, s->mem_bytes) != hipSuccess && (release_pool(), true) &&
+               hipMalloc(&s->mem, s->mem_bytes) != hipSuccess) {
         (void)hipGetLastError();
         delete s;
         return c_->fail(MRPOP_ERR_BUDGET, "device memory for a slot (%zu bytes) not available", s->mem_bytes);
@@ -235,7 +263,7 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     };
     SlotDev &v = s->dv;
     std::memset(&v, 0, sizeof v);
-    s->wbuf = (float *)take(sizeof(float) * (MRPOP_MAX_OBSTACLES * 15 + 64));
+    s->wbuf = (float *)take(sizeof(float) * kWorldFloats);
     v.sg = (float *)take(sizeof(float) * (1 + kMaxGoals) * D);
     v.rm_q = (float *)take(sizeof(float) * A * rm_cap * 8);
     v.rm_nbr = (int *)take(sizeof(int) * (size_t)A * rm_cap * K);
@@ -338,37 +366,10 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     v.path_cap = kPathCap;
     v.sp_cap = (int)sp_cap;
 
-    // ---- world (same record layout as mrpop_world_create)
+    // ---- world (same layout as mrpop_world_create)
     std::vector<float> host;
-    std::vector<float> aabb, obb;
-    for (int i = 0; i < scene.n_cuboids; ++i) {
-        const float *r = scene.cuboids + 15 * i;
-        if (is_world_axes(r)) aabb.insert(aabb.end(), {r[0], r[1], r[2], r[12], r[13], r[14]});
-        else obb.insert(obb.end(), r, r + 15);
-    }
     dev::WorldDesc &w = s->wdesc;
-    auto put = [&](const float *p, size_t n) { host.insert(host.end(), p, p + n); };
-    w.o_sph = (int)host.size();
-    w.n_sph = scene.n_spheres;
-    if (w.n_sph) put(scene.spheres, 4 * (size_t)w.n_sph);
-    w.o_cap = (int)host.size();
-    w.n_cap = scene.n_capsules;
-    if (w.n_cap) put(scene.capsules, 8 * (size_t)w.n_cap);
-    w.o_zcap = (int)host.size();
-    w.n_zcap = scene.n_z_capsules;
-    if (w.n_zcap) put(scene.z_capsules, 8 * (size_t)w.n_zcap);
-    w.o_aabb = (int)host.size();
-    w.n_aabb = (int)(aabb.size() / 6);
-    put(aabb.data(), aabb.size());
-    w.o_obb = (int)host.size();
-    w.n_obb = (int)(obb.size() / 15);
-    put(obb.data(), obb.size());
-    w.o_zcub = (int)host.size();
-    w.n_zcub = scene.n_z_cuboids;
-    if (w.n_zcub) put(scene.z_cuboids, 15 * (size_t)w.n_zcub);
-    w.o_base = (int)host.size();
-    put(b, 12 * (size_t)A);
-    w.n_arms = A;
+    pack_world(scene, b, A, w, host);
     w.obs = s->wbuf;
     MRPOP_HIP(c_, hipMemcpyAsync(s->wbuf, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, c_->stream));
     MRPOP_HIP(c_, hipMemcpyAsync(v.sg, s->sg.data(), s->sg.size() * sizeof(float), hipMemcpyHostToDevice, c_->stream));
@@ -397,13 +398,8 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
                              id, 0};
 
     h_worlds_[id] = w;
-    h_slots_[id] = v;
     MRPOP_HIP(c_, hipMemcpyAsync(d_worlds_ + id, &h_worlds_[id], sizeof(dev::WorldDesc), hipMemcpyHostToDevice, c_->stream));
-    MRPOP_HIP(c_, hipMemcpyAsync(d_slots_ + id, &h_slots_[id], sizeof(SlotDev), hipMemcpyHostToDevice, c_->stream));
-    h_ids_[0] = id;
-    MRPOP_HIP(c_, hipMemcpyAsync(d_ids_, h_ids_, sizeof(int), hipMemcpyHostToDevice, c_->stream));
-    plan::launch_roadmap_init(d_slots_, d_ids_, 1, c_->stream);
-    MRPOP_HIP(c_, hipStreamSynchronize(c_->stream));  // staging buffers are reused by the next call
+    if (int rc = run_aux(s, plan::launch_roadmap_init)) return rc;
     slots_.push_back(s);
     s->t_kernel_start = std::chrono::steady_clock::now();
     if (int rc = start_pass(s)) return rc;
@@ -411,13 +407,32 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     return MRPOP_OK;
 }
 
+void Engine::release_pool() {
+    (void)hipGetLastError();
+    for (auto &kv : pool_) (void)hipFree(kv.second);
+    pool_.clear();
+    pool_bytes_ = 0;
+}
+
 void Engine::release(Slot *s) {
     auto it = std::find(slots_.begin(), slots_.end(), s);
     if (it == slots_.end()) return;
+    if (s->lane >= 0) {
+        // its lane may still run kernels on the slot's memory; its report is dropped
+        Lane &L = lanes_[s->lane];
+        (void)hipStreamSynchronize(L.stream);
+        for (Slot *&t : L.table)
+            if (t == s) t = nullptr;
+    }
     (void)hipStreamSynchronize(c_->stream);
     if (s->mem) {
-        (void)hipFree(s->mem);
         bytes_in_use_ -= s->mem_bytes;
+        if (pool_bytes_ + s->mem_bytes <= kPoolBytes) {
+            pool_.emplace(s->mem_bytes, s->mem);  // kept for the next problem of this size
+            pool_bytes_ += s->mem_bytes;
+        } else {
+            (void)hipFree(s->mem);
+        }
     }
     slots_.erase(it);
     delete s;
@@ -465,253 +480,306 @@ int Engine::start_pass(Slot *s) {
     s->dr_round = 0;
     s->dr_budget = s->pass == 0 ? s->st.max_iters : s->st.optimize_iters;
     if (s->dv.optimize && s->st.phs) compute_phs(s);
-    h_slots_[s->id] = s->dv;
-    MRPOP_HIP(c_, hipMemcpyAsync(d_slots_ + s->id, &h_slots_[s->id], sizeof(SlotDev), hipMemcpyHostToDevice, c_->stream));
-    h_ids_[0] = s->id;
-    MRPOP_HIP(c_, hipMemcpyAsync(d_ids_, h_ids_, sizeof(int), hipMemcpyHostToDevice, c_->stream));
-    plan::launch_tree_reset(d_slots_, d_ids_, 1, c_->stream);
-    MRPOP_HIP(c_, hipStreamSynchronize(c_->stream));
+    return run_aux(s, plan::launch_tree_reset);
+}
+
+// Set-up kernels of one slot that is on no lane, on the context stream.
+int Engine::run_aux(Slot *s, void (*launch)(SlotDev *, const int *, int, hipStream_t)) {
+    *h_aux_slot_ = s->dv;
+    MRPOP_HIP(c_, hipMemcpyAsync(d_aux_slot_, h_aux_slot_, sizeof(SlotDev), hipMemcpyHostToDevice, c_->stream));
+    launch(d_aux_slot_, d_aux_id_, 1, c_->stream);
+    MRPOP_HIP(c_, hipGetLastError());
+    MRPOP_HIP(c_, hipStreamSynchronize(c_->stream));  // the staging slot is reused by the next call
     return MRPOP_OK;
+}
+
+int *Engine::upload_ids(Lane &L, const std::vector<Slot *> &list) {
+    int *h = L.h_ids + L.id_cursor;
+    for (size_t i = 0; i < list.size(); ++i) {
+        int k = 0;
+        while (L.table[k] != list[i]) ++k;  // entry of the slot in the lane table
+        h[i] = k;
+    }
+    int *d = L.d_ids + L.id_cursor;
+    (void)hipMemcpyAsync(d, h, sizeof(int) * list.size(), hipMemcpyHostToDevice, L.stream);
+    L.id_cursor += list.size();
+    return d;
 }
 
 // Copy the edge-group descriptors of `list` (per_slot groups each) to the
 // device, bucketed by arm count for the multi-arm modes.  The descriptors
 // hold device pointers and counts only, so one copy serves every round of a
 // step.
-int Engine::stage_groups(int mode, const std::vector<const dev::EdgeGroup *> &groups, int per_slot,
-                         const std::vector<Slot *> &list, size_t &cursor, Staged &out) {
+int Engine::stage_groups(Lane &L, int mode, const std::vector<const dev::EdgeGroup *> &groups, int per_slot,
+                         const std::vector<Slot *> &list, Staged &out) {
     out.mode = mode;
     out.buckets.clear();
-    const size_t begin = cursor;
+    const size_t begin = L.group_cursor;
     for (int A = (mode == MRPOP_MODE_SINGLE ? 1 : 2); A <= (mode == MRPOP_MODE_SINGLE ? 1 : 5); ++A) {
-        Staged::Bucket bk{A, cursor, 0, 0};
+        Staged::Bucket bk{A, L.group_cursor, 0, 0};
         const int S = 64 / A;
         for (size_t i = 0; i < list.size(); ++i) {
             if (mode != MRPOP_MODE_SINGLE && list[i]->A != A) continue;
             const int n = mode == MRPOP_MODE_SINGLE ? list[i]->A : 1;
             for (int k = 0; k < n && k < per_slot; ++k) {
-                if (cursor >= group_cap_) return c_->fail(MRPOP_ERR_BUDGET, "edge-group staging full");
+                if (L.group_cursor >= group_cap_) return c_->fail(MRPOP_ERR_BUDGET, "edge-group staging full");
                 const dev::EdgeGroup &g = groups[i][k];
-                h_groups_[cursor++] = g;
+                L.h_groups[L.group_cursor++] = g;
                 bk.max_waves += ((size_t)g.n_edges * g.G + S - 1) / S;
             }
         }
-        bk.n = (int)(cursor - bk.first);
+        bk.n = (int)(L.group_cursor - bk.first);
         if (bk.n) out.buckets.push_back(bk);
     }
-    if (cursor > begin)
-        MRPOP_HIP(c_, hipMemcpyAsync(d_groups_ + begin, h_groups_ + begin, sizeof(dev::EdgeGroup) * (cursor - begin),
-                                     hipMemcpyHostToDevice, c_->stream));
+    if (L.group_cursor > begin)
+        MRPOP_HIP(c_, hipMemcpyAsync(L.d_groups + begin, L.h_groups + begin,
+                                     sizeof(dev::EdgeGroup) * (L.group_cursor - begin), hipMemcpyHostToDevice, L.stream));
     return MRPOP_OK;
 }
 
 // One batched check launch per arm-count bucket.  Every producer of planner
 // edges clears their verdict words, so the launches skip the zeroing pass.
-int Engine::launch_staged(const Staged &st) {
+int Engine::launch_staged(Lane &L, const Staged &st) {
     for (const Staged::Bucket &bk : st.buckets)
-        if (int rc = launch_check_groups(c_, st.mode, bk.A, d_worlds_, d_groups_ + bk.first, bk.n, d_wave_, bk.max_waves,
-                                         false, c_->stream))
+        if (int rc = launch_check_groups(c_, st.mode, bk.A, d_worlds_, L.d_groups + bk.first, bk.n, L.d_wave,
+                                         bk.max_waves, false, L.stream))
             return rc;
     return MRPOP_OK;
 }
 
-int Engine::step() {
-    std::vector<Slot *> rm, dr, sp;
-    for (Slot *s : slots_) {
-        if (s->phase == Phase::Done) continue;
-        if (s->opt.time_budget_s > 0 &&
-            std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t_submit).count() > s->opt.time_budget_s) {
-            s->phase = Phase::Done;
-            s->res.status = MRPOP_ERR_TIMEOUT;
-            s->res.wall_ns = ns_since(s->t_submit);
-            s->res.kernel_ns = ns_since(s->t_kernel_start);
-            continue;
-        }
-        if (s->phase == Phase::Roadmap) rm.push_back(s);
-        else if (s->phase == Phase::Search) dr.push_back(s);
-        else if (s->phase == Phase::Simplify) sp.push_back(s);
+// Enqueue one step of a lane for the slots in L.rm / L.dr / L.sp.
+int Engine::launch_lane(Lane &L) {
+    L.table.clear();
+    for (auto *lst : {&L.rm, &L.dr, &L.sp}) L.table.insert(L.table.end(), lst->begin(), lst->end());
+    if (L.table.empty()) return MRPOP_OK;
+    hipStream_t st = L.stream;
+    L.id_cursor = 0;
+    L.group_cursor = 0;
+    for (size_t k = 0; k < L.table.size(); ++k) {
+        L.h_slots[k] = L.table[k]->dv;
+        L.table[k]->lane = (int)(&L - lanes_);
     }
-    if (rm.empty() && dr.empty() && sp.empty()) return MRPOP_OK;
-    hipStream_t st = c_->stream;
-    // per-slot round state to the device
-    for (Slot *s : slots_) h_slots_[s->id] = s->dv;
-    MRPOP_HIP(c_, hipMemcpyAsync(d_slots_, h_slots_, sizeof(SlotDev) * cap_, hipMemcpyHostToDevice, st));
-    size_t cursor = 0, id_cursor = 0;
-    // every list gets its own staging range: the copies run after the host has moved on
-    auto upload_ids = [&](const std::vector<Slot *> &list, int) -> int * {
-        if (id_cursor + list.size() > (size_t)cap_ * 32) return nullptr;
-        int *h = h_ids_ + id_cursor;
-        for (size_t i = 0; i < list.size(); ++i) h[i] = list[i]->id;
-        (void)hipMemcpyAsync(d_ids_ + id_cursor, h, sizeof(int) * list.size(), hipMemcpyHostToDevice, st);
-        int *d = d_ids_ + id_cursor;
-        id_cursor += list.size();
-        return d;
-    };
+    MRPOP_HIP(c_, hipMemcpyAsync(L.d_slots, L.h_slots, sizeof(SlotDev) * L.table.size(), hipMemcpyHostToDevice, st));
+    SlotDev *ds = L.d_slots;
     // ---- roadmap round (roadmap_interior.cuh:94-339)
-    if (!rm.empty()) {
-        int *d = upload_ids(rm, 0);
-        const int n = (int)rm.size();
+    if (!L.rm.empty()) {
+        int *d = upload_ids(L, L.rm);
+        const int n = (int)L.rm.size();
         int max_am = 0;
-        for (Slot *s : rm) max_am = std::max(max_am, s->A * s->dv.M);
-        plan::launch_rm_sample(d_slots_, d, n, max_am, st);
+        for (Slot *s : L.rm) max_am = std::max(max_am, s->A * s->dv.M);
+        plan::launch_rm_sample(ds, d, n, max_am, st);
         std::vector<const dev::EdgeGroup *> gv, gr;
-        for (Slot *s : rm) {
+        for (Slot *s : L.rm) {
             gv.push_back(s->g_valid);
             gr.push_back(s->g_rm);
         }
         Staged sv, sr;
-        if (int rc = stage_groups(MRPOP_MODE_SINGLE, gv, geom::kMaxArms, rm, cursor, sv)) return rc;
-        if (int rc = stage_groups(MRPOP_MODE_SINGLE, gr, geom::kMaxArms, rm, cursor, sr)) return rc;
-        if (int rc = launch_staged(sv)) return rc;
-        plan::launch_rm_assign(d_slots_, d, n, st);
-        plan::launch_rm_radius(d_slots_, d, n, max_am, st);
-        plan::launch_rm_compact(d_slots_, d, n, st);
-        if (int rc = launch_staged(sr)) return rc;
-        plan::launch_rm_link(d_slots_, d, n, max_am, st);
-        plan::launch_rm_connected(d_slots_, d, n, st);
+        if (int rc = stage_groups(L, MRPOP_MODE_SINGLE, gv, geom::kMaxArms, L.rm, sv)) return rc;
+        if (int rc = stage_groups(L, MRPOP_MODE_SINGLE, gr, geom::kMaxArms, L.rm, sr)) return rc;
+        if (int rc = launch_staged(L, sv)) return rc;
+        plan::launch_rm_assign(ds, d, n, st);
+        plan::launch_rm_radius(ds, d, n, max_am, st);
+        plan::launch_rm_compact(ds, d, n, max_am, st);
+        if (int rc = launch_staged(L, sr)) return rc;
+        plan::launch_rm_link(ds, d, n, max_am, st);
+        plan::launch_rm_connected(ds, d, n, st);
     }
     // ---- dRRT rounds (pop_planner.cu:350-736).  The round index lives on the
     // device; a slot whose pass is solved or whose launch is out of rounds
     // skips the remaining rounds of the step.
-    if (!dr.empty()) {
-        int *d = upload_ids(dr, 1);
-        const int n = (int)dr.size();
+    if (!L.dr.empty()) {
+        int *d = upload_ids(L, L.dr);
+        const int n = (int)L.dr.size();
         int max_b = 0, rounds = 1 << 30;
         bool reset = false;
-        for (Slot *s : dr) {
+        for (Slot *s : L.dr) {
             max_b = std::max(max_b, s->dv.B);
             rounds = std::min(rounds, std::max(1, s->opt.rounds_per_step));
             reset |= s->dv.round_reset != 0;
             s->dv.round_reset = 0;  // applied by this step's k_dr_begin
         }
         std::vector<const dev::EdgeGroup *> g1, g2;
-        for (Slot *s : dr) {
+        for (Slot *s : L.dr) {
             g1.push_back(&s->g_dr);
             g2.push_back(&s->g_cn);
         }
         Staged s1, s2;
-        if (int rc = stage_groups(MRPOP_MODE_DRRT, g1, 1, dr, cursor, s1)) return rc;
-        if (int rc = stage_groups(MRPOP_MODE_FULL, g2, 1, dr, cursor, s2)) return rc;
-        if (reset) plan::launch_dr_begin(d_slots_, d, n, st);
+        if (int rc = stage_groups(L, MRPOP_MODE_DRRT, g1, 1, L.dr, s1)) return rc;
+        if (int rc = stage_groups(L, MRPOP_MODE_FULL, g2, 1, L.dr, s2)) return rc;
+        if (reset) plan::launch_dr_begin(ds, d, n, st);
         for (int r = 0; r < rounds; ++r) {
-            plan::launch_dr_expand(d_slots_, d, n, max_b, st);
-            if (int rc = launch_staged(s1)) return rc;
-            plan::launch_dr_commit(d_slots_, d, n, st);
-            plan::launch_dr_connect(d_slots_, d, n, max_b, st);
-            if (int rc = launch_staged(s2)) return rc;
-            plan::launch_dr_finish(d_slots_, d, n, st);
+            plan::launch_dr_expand(ds, d, n, max_b, st);
+            if (int rc = launch_staged(L, s1)) return rc;
+            plan::launch_dr_commit(ds, d, n, st);
+            plan::launch_dr_connect(ds, d, n, max_b, st);
+            if (int rc = launch_staged(L, s2)) return rc;
+            plan::launch_dr_finish(ds, d, n, st);
         }
     }
-    // ---- simplification stage (path_simplification.cu:41-238)
-    if (!sp.empty()) {
-        // group slots by stage: smoothing stages share one launch sequence, the shortcut another
-        std::vector<Slot *> smooth, cut;
-        for (Slot *s : sp) (s->sp_stage < s->st.bspline_step ? smooth : cut).push_back(s);
-        int which = 2;
-        for (auto *lst : {&smooth, &cut}) {
-            if (lst->empty()) continue;
-            // stages may differ between slots: launch per distinct stage
-            std::vector<int> stages;
-            for (Slot *s : *lst) stages.push_back(s->sp_stage);
-            std::sort(stages.begin(), stages.end());
-            stages.erase(std::unique(stages.begin(), stages.end()), stages.end());
-            for (int stage : stages) {
-                std::vector<Slot *> part;
-                for (Slot *s : *lst)
-                    if (s->sp_stage == stage) part.push_back(s);
-                // settings may differ per slot too: group by (bspline_step, range, min_change)
-                while (!part.empty()) {
-                    Slot *s0 = part.front();
-                    std::vector<Slot *> same, rest;
-                    for (Slot *s : part)
-                        ((s->st.bspline_step == s0->st.bspline_step && s->st.range == s0->st.range &&
-                          s->st.bspline_min_change == s0->st.bspline_min_change)
-                             ? same
-                             : rest)
-                            .push_back(s);
-                    part.swap(rest);
-                    int *d = upload_ids(same, which++);
-                    if (!d) return c_->fail(MRPOP_ERR_BUDGET, "slot-id staging full");
-                    const int n = (int)same.size();
-                    plan::launch_sp_edges(d_slots_, d, n, stage, s0->st.bspline_step, s0->st.range,
-                                          s0->st.bspline_min_change, st);
-                    std::vector<const dev::EdgeGroup *> gs;
-                    for (Slot *s : same) gs.push_back(&s->g_sp);
-                    Staged ss;
-                    if (int rc = stage_groups(MRPOP_MODE_FULL, gs, 1, same, cursor, ss)) return rc;
-                    if (int rc = launch_staged(ss)) return rc;
-                    plan::launch_sp_collect(d_slots_, d, n, st);
-                    if (stage < s0->st.bspline_step) plan::launch_sp_smooth_apply(d_slots_, d, n, st);
-                    else plan::launch_sp_shortcut(d_slots_, d, n, st);
-                }
-            }
+    // ---- simplification stage (path_simplification.cu:41-238): slots at the
+    // same stage with the same settings share one launch sequence
+    if (!L.sp.empty()) {
+        std::vector<Slot *> part = L.sp;
+        while (!part.empty()) {
+            Slot *s0 = part.front();
+            std::vector<Slot *> same, rest;
+            for (Slot *s : part)
+                ((s->sp_stage == s0->sp_stage && s->st.bspline_step == s0->st.bspline_step &&
+                  s->st.range == s0->st.range && s->st.bspline_min_change == s0->st.bspline_min_change)
+                     ? same
+                     : rest)
+                    .push_back(s);
+            part.swap(rest);
+            int *d = upload_ids(L, same);
+            const int n = (int)same.size();
+            const int stage = s0->sp_stage;
+            plan::launch_sp_edges(ds, d, n, stage, s0->st.bspline_step, s0->st.range, s0->st.bspline_min_change, st);
+            std::vector<const dev::EdgeGroup *> gs;
+            for (Slot *s : same) gs.push_back(&s->g_sp);
+            Staged ss;
+            if (int rc = stage_groups(L, MRPOP_MODE_FULL, gs, 1, same, ss)) return rc;
+            if (int rc = launch_staged(L, ss)) return rc;
+            plan::launch_sp_collect(ds, d, n, st);
+            if (stage < s0->st.bspline_step) plan::launch_sp_smooth_apply(ds, d, n, st);
+            else plan::launch_sp_shortcut(ds, d, n, st);
         }
     }
     // ---- reports
-    std::vector<Slot *> all;
-    all.insert(all.end(), rm.begin(), rm.end());
-    all.insert(all.end(), dr.begin(), dr.end());
-    all.insert(all.end(), sp.begin(), sp.end());
     {
-        int *d = upload_ids(all, 7);
-        plan::launch_report(d_slots_, d, (int)all.size(), d_report_, st);
-        MRPOP_HIP(c_, hipMemcpyAsync(h_report_, d_report_, sizeof(SlotReport) * cap_, hipMemcpyDeviceToHost, st));
-        MRPOP_HIP(c_, hipStreamSynchronize(st));
+        int *d = upload_ids(L, L.table);
+        plan::launch_report(ds, d, (int)L.table.size(), L.d_report, st);
+        MRPOP_HIP(c_, hipMemcpyAsync(L.h_report, L.d_report, sizeof(SlotReport) * L.table.size(),
+                                     hipMemcpyDeviceToHost, st));
     }
-    // ---- state machines
-    for (Slot *s : rm) {
-        const SlotReport &r = h_report_[s->id];
-        s->dv.rm_round++;
-        s->res.roadmap_rounds++;
-        bool all_connected = true, at_cap = false;
-        for (int a = 0; a < s->A; ++a) {
-            all_connected &= r.rm_connected[a] != 0;
-            at_cap |= r.rm_n[a] >= s->dv.rm_cap;
-            s->res.roadmap_size[a] = r.rm_n[a];
-        }
-        if (!s->rm_until_connected || all_connected || at_cap) {
-            s->rm_until_connected = false;
-            s->phase = Phase::Search;
-            s->dr_round = 0;
-            s->dv.round_reset = 1;  // a fresh dRRT launch
-            s->dv.round_limit = s->dr_budget;
-        }
-    }
-    for (Slot *s : dr) {
-        const SlotReport &r = h_report_[s->id];
-        s->dr_round = r.ctr[plan_ctr_round];
-        s->res.rounds = r.ctr[plan_ctr_run];
-        if (r.status > 0) {
-            s->pass_solved = true;
-            s->best_cost = r.best_cost;
-            s->res.start_tree_size = r.path_meta[0];
-            s->res.goal_tree_size = r.path_meta[1];
-            s->res.iters = r.path_meta[2];
-            if (s->st.path_simplify) {
-                s->phase = Phase::Simplify;
-                s->sp_stage = 0;
-            } else if (int rc = finish_pass(s, r)) {
-                return rc;
+    MRPOP_HIP(c_, hipGetLastError());
+    MRPOP_HIP(c_, hipEventRecord(L.done, st));
+    L.busy = true;
+    return MRPOP_OK;
+}
+
+// Advance the state machines of a lane's slots from its reports.
+int Engine::process_lane(Lane &L) {
+    L.busy = false;
+    const auto now = std::chrono::steady_clock::now();
+    for (size_t k = 0; k < L.table.size(); ++k) {
+        Slot *s = L.table[k];
+        if (!s) continue;  // released while running
+        s->lane = -1;
+        if (s->phase == Phase::Done) continue;  // cancelled while running
+        const SlotReport &r = L.h_report[k];
+        const bool in_rm = std::find(L.rm.begin(), L.rm.end(), s) != L.rm.end();
+        const bool in_dr = !in_rm && std::find(L.dr.begin(), L.dr.end(), s) != L.dr.end();
+        if (in_rm) {
+            s->dv.rm_round++;
+            s->res.roadmap_rounds++;
+            bool all_connected = true, at_cap = false;
+            for (int a = 0; a < s->A; ++a) {
+                all_connected &= r.rm_connected[a] != 0;
+                at_cap |= r.rm_n[a] >= s->dv.rm_cap;
+                s->res.roadmap_size[a] = r.rm_n[a];
             }
-        } else if (r.status < 0) {
-            // tree capacity reached: the pass ends unsolved
-            if (int rc = finish_pass(s, r)) return rc;
-        } else if (s->dr_round >= s->dr_budget) {
-            if (s->pass == 0) {
-                s->phase = Phase::Roadmap;  // one more roadmap round, then a fresh dRRT launch
-            } else if (int rc = finish_pass(s, r)) {
-                return rc;
+            if (!s->rm_until_connected || all_connected || at_cap) {
+                s->rm_until_connected = false;
+                s->phase = Phase::Search;
+                s->dr_round = 0;
+                s->dv.round_reset = 1;  // a fresh dRRT launch
+                s->dv.round_limit = s->dr_budget;
+            }
+        } else if (in_dr) {
+            s->dr_round = r.ctr[plan_ctr_round];
+            s->res.rounds = r.ctr[plan_ctr_run];
+            if (r.status > 0) {
+                s->pass_solved = true;
+                s->best_cost = r.best_cost;
+                s->res.start_tree_size = r.path_meta[0];
+                s->res.goal_tree_size = r.path_meta[1];
+                s->res.iters = r.path_meta[2];
+                if (s->st.path_simplify) {
+                    s->phase = Phase::Simplify;
+                    s->sp_stage = 0;
+                } else if (int rc = finish_pass(s, r)) {
+                    return rc;
+                }
+            } else if (r.status < 0) {
+                // tree capacity reached: the pass ends unsolved
+                if (int rc = finish_pass(s, r)) return rc;
+            } else if (s->dr_round >= s->dr_budget) {
+                if (s->pass == 0) {
+                    s->phase = Phase::Roadmap;  // one more roadmap round, then a fresh dRRT launch
+                } else if (int rc = finish_pass(s, r)) {
+                    return rc;
+                }
+            }
+        } else {
+            s->sp_stage++;
+            if (s->sp_stage > s->st.bspline_step) {
+                s->best_cost = r.best_cost;
+                if (int rc = finish_pass(s, r)) return rc;
             }
         }
-    }
-    for (Slot *s : sp) {
-        const SlotReport &r = h_report_[s->id];
-        s->sp_stage++;
-        if (s->sp_stage > s->st.bspline_step) {
-            s->best_cost = r.best_cost;
-            if (int rc = finish_pass(s, r)) return rc;
+        if (s->phase != Phase::Done && s->opt.time_budget_s > 0 &&
+            std::chrono::duration<double>(now - s->t_submit).count() > s->opt.time_budget_s) {
+            s->phase = Phase::Done;
+            s->res.status = MRPOP_ERR_TIMEOUT;
+            s->res.wall_ns = ns_since(s->t_submit);
+            s->res.kernel_ns = ns_since(s->t_kernel_start);
         }
     }
+    L.table.clear();
+    L.rm.clear();
+    L.dr.clear();
+    L.sp.clear();
+    return MRPOP_OK;
+}
+
+int Engine::drain() {
+    for (Lane &L : lanes_)
+        if (L.busy) {
+            MRPOP_HIP(c_, hipEventSynchronize(L.done));
+            if (int rc = process_lane(L)) return rc;
+        }
+    return MRPOP_OK;
+}
+
+int Engine::step() {
+    if (int rc = ensure_lanes()) return rc;
+    const auto now = std::chrono::steady_clock::now();
+    for (Slot *s : slots_) {
+        if (s->phase == Phase::Done || s->lane >= 0) continue;
+        if (s->opt.time_budget_s > 0 && std::chrono::duration<double>(now - s->t_submit).count() > s->opt.time_budget_s) {
+            s->phase = Phase::Done;
+            s->res.status = MRPOP_ERR_TIMEOUT;
+            s->res.wall_ns = ns_since(s->t_submit);
+            s->res.kernel_ns = ns_since(s->t_kernel_start);
+        }
+    }
+    Lane &R = lanes_[0], &S = lanes_[1];
+    if (!R.busy) {
+        // slots still looking for their first solution go first; the roadmap
+        // rounds of optimising passes wait for a step without them
+        for (Slot *s : slots_)
+            if (s->lane < 0 && s->phase == Phase::Roadmap && !s->res.solved) R.rm.push_back(s);
+        if (R.rm.empty())
+            for (Slot *s : slots_)
+                if (s->lane < 0 && s->phase == Phase::Roadmap) R.rm.push_back(s);
+        if (int rc = launch_lane(R)) return rc;
+    }
+    if (!S.busy) {
+        for (Slot *s : slots_) {
+            if (s->lane >= 0) continue;
+            if (s->phase == Phase::Search) S.dr.push_back(s);
+            else if (s->phase == Phase::Simplify) S.sp.push_back(s);
+        }
+        if (int rc = launch_lane(S)) return rc;
+    }
+    // wait for the search lane (short steps) when it runs, else for the roadmap lane
+    Lane *first = S.busy ? &S : (R.busy ? &R : nullptr);
+    if (!first) return MRPOP_OK;
+    MRPOP_HIP(c_, hipEventSynchronize(first->done));
+    if (int rc = process_lane(*first)) return rc;
+    Lane &other = first == &S ? R : S;
+    if (other.busy && hipEventQuery(other.done) == hipSuccess)
+        if (int rc = process_lane(other)) return rc;
     return MRPOP_OK;
 }
 

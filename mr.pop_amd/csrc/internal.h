@@ -62,6 +62,12 @@ struct World {
 
 int ensure(Context *c, DevBuf &b, size_t bytes);
 
+// Lay out a scene and the arms' base poses in one float buffer (obstacle
+// records by type, world-axis cuboids as centre + half extents, base poses,
+// cull boxes) and fill the offsets of d except d.obs.  Returns the number of
+// obstacles, or -1 if a count is negative.
+long pack_world(const mrpop_scene &s, const float *bases, int n_arms, dev::WorldDesc &d, std::vector<float> &host);
+
 // Enqueue the coarse + fine edge-check kernels (check.hip).
 int launch_check(Context *c, const World *w, int mode, int arm, int granularity, int n_edges, const float *d_from,
                  const float *d_to, uint32_t *d_out_edge, uint64_t *d_out_mask, hipStream_t stream);

@@ -170,7 +170,7 @@ void launch_tree_reset(SlotDev *d, const int *ids, int n, hipStream_t stream);
 void launch_rm_sample(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream);
 void launch_rm_assign(SlotDev *d, const int *ids, int n, hipStream_t stream);
 void launch_rm_radius(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream);
-void launch_rm_compact(SlotDev *d, const int *ids, int n, hipStream_t stream);
+void launch_rm_compact(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream);
 void launch_rm_link(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream);
 void launch_rm_connected(SlotDev *d, const int *ids, int n, hipStream_t stream);
 void launch_dr_begin(SlotDev *d, const int *ids, int n, hipStream_t stream);

@@ -192,9 +192,9 @@ __global__ __launch_bounds__(64) void k_rm_radius(SlotDev *slots, const int *ids
     if (lane == 0) cnt[i] = min(found, s.rm_Kc);
 }
 
-// Pack the staged candidates of each arm in sample order (prefix over
-// samples) into the edge list neighbour -> new node (roadmap_interior.cuh:
-// 214-234).  One block per (slot, arm).
+// First packed edge of each sample: the staged candidates of an arm form
+// one edge list neighbour -> new node in sample order (roadmap_interior.cuh:
+// 214-234); k_rm_pack writes it.  One block per (slot, arm).
 __global__ __launch_bounds__(1024) void k_rm_compact(SlotDev *slots, const int *ids) {
     SlotDev &s = slots[ids[blockIdx.y]];
     const int a = blockIdx.x;
@@ -215,25 +215,35 @@ __global__ __launch_bounds__(1024) void k_rm_compact(SlotDev *slots, const int *
         __syncthreads();
     }
     int e = sh[tid] - c;
-    const float *rq = s.rm_q + (size_t)a * s.rm_cap * 8;
-    const size_t ebase = (size_t)a * s.M * s.rm_Kc;
     for (int i = i0; i < min(i0 + per, s.M); ++i) {
         s.rs_first[(size_t)a * s.M + i] = e;
-        const int id = s.rs_id[(size_t)a * s.M + i];
-        const int *stage = s.rs_cand + ((size_t)a * s.M + i) * s.rm_Kc;
-        for (int k = 0; k < cnt[i]; ++k, ++e) {
-            const int n = stage[k];
-            const float *p = rq + (size_t)n * 8, *q = rq + (size_t)id * 8;
-            for (int j = 0; j < 7; ++j) {
-                s.re_from[(ebase + e) * 7 + j] = p[j];
-                s.re_to[(ebase + e) * 7 + j] = q[j];
-            }
-            s.re_pair[(ebase + e) * 2 + 0] = n;
-            s.re_pair[(ebase + e) * 2 + 1] = id;
-            s.re_coll[ebase + e] = 0u;
-        }
+        e += cnt[i];
     }
     if (tid == 1023) s.re_count[a] = sh[1023];
+}
+
+// Write the packed edges of one sample: lane k is candidate k.  One wave per
+// (slot, arm, sample).
+__global__ __launch_bounds__(64) void k_rm_pack(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int t = blockIdx.x;
+    if (t >= s.A * s.M) return;
+    const int a = t / s.M, i = t - a * s.M;
+    const int k = threadIdx.x;
+    if (k >= s.rs_ncand[(size_t)a * s.M + i]) return;
+    const float *rq = s.rm_q + (size_t)a * s.rm_cap * 8;
+    const size_t e = (size_t)a * s.M * s.rm_Kc + s.rs_first[(size_t)a * s.M + i] + k;
+    const int id = s.rs_id[(size_t)a * s.M + i];
+    const int n = s.rs_cand[((size_t)a * s.M + i) * s.rm_Kc + k];
+    const float *p = rq + (size_t)n * 8, *q = rq + (size_t)id * 8;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        s.re_from[e * 7 + j] = p[j];
+        s.re_to[e * 7 + j] = q[j];
+    }
+    s.re_pair[e * 2 + 0] = n;
+    s.re_pair[e * 2 + 1] = id;
+    s.re_coll[e] = 0u;  // the checker only sets bits
 }
 
 // Parents always have smaller ids than their children, so the walk ends; the
@@ -1134,8 +1144,9 @@ void launch_rm_assign(SlotDev *d, const int *ids, int n, hipStream_t stream) {
 void launch_rm_radius(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream) {
     MRPOP_L(k_rm_radius, dim3(max_am, n), dim3(64), d, ids);
 }
-void launch_rm_compact(SlotDev *d, const int *ids, int n, hipStream_t stream) {
+void launch_rm_compact(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream) {
     MRPOP_L(k_rm_compact, dim3(kMaxArms, n), dim3(1024), d, ids);
+    MRPOP_L(k_rm_pack, dim3(max_am, n), dim3(64), d, ids);
 }
 void launch_rm_link(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream) {
     MRPOP_L(k_rm_link_own, dim3(max_am, n), dim3(64), d, ids);
