@@ -74,6 +74,27 @@ def cpu_baseline(problem, frm, to, G, budget_s=12.0):
             "single_thread_value": rate1}
 
 
+TRAFFIC_PROFILE = REPO / "profiles" / "check_kernel_bin4_g32_pmc.md"
+
+
+def measured_traffic(n_edges, G):
+    """HBM bytes per launch of the timed check (FETCH_SIZE + WRITE_SIZE from
+    rocprofv3 --pmc, scaled by the counters' reading for a copy of known
+    size), as recorded in the profile summary for this workload; None when
+    the summary is absent or was taken on another batch."""
+    if not TRAFFIC_PROFILE.exists():
+        return None
+    vals = {}
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        if line.startswith("traffic:"):
+            for kv in line[len("traffic:"):].split(","):
+                k, v = kv.strip().split("=")
+                vals[k] = float(v)
+    if vals.get("edges") != n_edges or vals.get("granularity") != G:
+        return None
+    return vals.get("bytes_per_launch")
+
+
 PLAN_KINDS = ("plain2", "table3", "bin4", "shelf5")   # 2 / 3 / 4 / 5 arms, ratio 1:1:1:1
 
 
@@ -84,6 +105,8 @@ def plan_problems(checker, n):
 
 
 def _stats(sols, ttfs, wall):
+    if not sols:
+        return None
     solved = [s for s in sols if s.solved]
     t = np.array(ttfs, dtype=np.float64)
     costs = np.array([s.cost for s in solved]) if solved else np.array([np.nan])
@@ -117,7 +140,12 @@ def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_r
         if s.solved:
             plain_ttfs.append(start + s.ttfs_s)
             own_ttfs.append(s.ttfs_s)
-    plain = _stats(plain_sols, plain_ttfs, time.perf_counter() - t0)
+    plain = _stats(plain_sols, plain_ttfs, time.perf_counter() - t0) if plain_sols else None
+    if plain is None:
+        t0 = time.perf_counter()
+        batch_sols = _lib.solve_batch(ctx, probs[:n_batch], [st_batch] * n_batch, [op] * n_batch)
+        out["batched"] = _stats(batch_sols, [s.ttfs_s for s in batch_sols if s.solved], time.perf_counter() - t0)
+        return out
     plain["ttfs_own_p50_s"] = float(np.percentile(own_ttfs, 50)) if own_ttfs else None
     plain["ttfs_own_p99_s"] = float(np.percentile(own_ttfs, 99)) if own_ttfs else None
     print(f"[bench] plain: {plain}", file=sys.stderr, flush=True)
@@ -199,6 +227,9 @@ def main():
     ap.add_argument("--batch-nnc", type=int, default=256)
     ap.add_argument("--serve-rates", type=str, default="20,50")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--plan-only", action="store_true", help="only the planner measurements (profiling runs)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="copy 256 MiB three times and exit: the reference kernel that scales the HBM counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: problems are independent and this build runs on a single MI355X; "
@@ -209,8 +240,23 @@ def main():
     from mrpop import scenes, _lib
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        n = 64 * 1024 * 1024  # floats: 256 MiB
+        a = torch.ones(n, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "copies": 3}), flush=True)
+        return
     ctx = _lib.Context(0)
     checker = _lib.gpu_checker(ctx)
+    if args.plan_only:
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        print(json.dumps(planning_bench(ctx, checker, args.plain_problems, args.batch_problems, args.plain_nnc,
+                                        args.batch_nnc, rates, args.serve_seconds)), flush=True)
+        return
     problem = scenes.make_problem("bin4", 0, checker)
     G = args.granularity
     frm, to = make_edges(problem, checker, args.edges, seed=1)
@@ -241,7 +287,12 @@ def main():
     ms_events = ev0.elapsed_time(ev1) / args.steps
 
     colliding = float(d_out.cpu().numpy().astype(bool).mean())
-    ref = world.check_edges(_lib.MODE_FULL, G, frm, to)
+    # host-buffer entry point (copies over PCIe included): reported beside the value
+    world.check_edges(_lib.MODE_FULL, G, frm, to)
+    t_h = time.perf_counter()
+    for _ in range(3):
+        ref = world.check_edges(_lib.MODE_FULL, G, frm, to)
+    host_edges_per_s = 3 * args.edges / (time.perf_counter() - t_h)
     assert np.array_equal(ref, d_out.cpu().numpy().astype(bool)), "device and host entry points disagree"
 
     edges_per_s = args.edges * args.steps / wall
@@ -268,8 +319,9 @@ def main():
                                "obstacles, granularity 32, batch of edges resident in HBM",
                    "edges_per_step": args.edges, "granularity": G, "arms": problem.n_arms,
                    "obstacles": sc.n_obstacles, "colliding_fraction": round(colliding, 4)},
+        "host_buffers_edges_per_s": host_edges_per_s,
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": measured_traffic(args.edges, G),
                      "kernel": "check_kernel<4, FULL> (fused coarse + fine edge check)",
                      "ms_per_launch": ms_events, "flops_per_sample": w_sample},
     }
