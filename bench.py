@@ -98,10 +98,10 @@ def measured_traffic(n_edges, G):
 PLAN_KINDS = ("plain2", "table3", "bin4", "shelf5")   # 2 / 3 / 4 / 5 arms, ratio 1:1:1:1
 
 
-def plan_problems(checker, n):
-    """n seeded problems, arm counts 2/3/4/5 in turn (seeds 0, 0, 0, 0, 1, ...)."""
+def plan_problems(checker, n, first_seed=0):
+    """n seeded problems, arm counts 2/3/4/5 in turn (seeds s, s, s, s, s + 1, ...)."""
     from mrpop import scenes
-    return [scenes.make_problem(PLAN_KINDS[i % 4], i // 4, checker) for i in range(n)]
+    return [scenes.make_problem(PLAN_KINDS[i % 4], first_seed + i // 4, checker) for i in range(n)]
 
 
 def _stats(sols, ttfs, wall):
@@ -118,18 +118,30 @@ def _stats(sols, ttfs, wall):
             "timeouts": sum(s.info["status"] == 4 for s in sols)}
 
 
-def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_rates, serve_s):
-    """Plain vs batched planning on the same seeded problems, and serving."""
-    from mrpop import _lib
+def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_rates, serve_s, tune=True,
+                   n_budget=0):
+    """Plain vs batched planning on the same seeded problems, a budgeted
+    batch, and serving."""
+    from mrpop import _lib, autotune
     probs = plan_problems(checker, max(n_plain, n_batch))
     common = dict(granularity=32, range=2.0, rrtc_iter=6, optimize_iters=2000)
+    op_common = dict(time_budget_s=30.0)
     st_plain = _lib.settings(num_new_configs=plain_nnc, **common)
-    st_batch = _lib.settings(num_new_configs=batch_nnc, **common)
-    op = _lib.solve_options_default(time_budget_s=30.0)
-    out = {"settings": dict(common, plain_num_new_configs=plain_nnc, batch_num_new_configs=batch_nnc,
-                            time_budget_s=30.0),
+    op = _lib.solve_options_default(**op_common)
+    out = {"settings": dict(common, plain_num_new_configs=plain_nnc, time_budget_s=30.0),
            "problems": "seeded mixed 2/3/4/5-arm scenes (random cuboids, 3-arm table, 4-arm bin, 5-arm shelf), "
                        "one start and one goal each"}
+    # batched-form settings: autotuned on a calibration set of other seeds
+    tuned = {"num_new_configs": batch_nnc}
+    if tune:
+        calib = plan_problems(checker, 16, first_seed=1000)
+        res = autotune.tune(ctx, calib, common, op_common)
+        tuned = res.best
+        out["autotune"] = res.as_dict()
+        print(f"[bench] autotune: {tuned} in {res.seconds:.1f} s", file=sys.stderr, flush=True)
+    st_batch = _lib.settings(num_new_configs=tuned["num_new_configs"], **common)
+    op_batch = _lib.solve_options_default(**op_common, **{k: v for k, v in tuned.items() if k != "num_new_configs"})
+    out["settings"]["batched"] = tuned
     # plain: one problem at a time, all submitted at t = 0 (TTFS includes the queue)
     plain_sols, plain_ttfs, own_ttfs = [], [], []
     t0 = time.perf_counter()
@@ -143,7 +155,7 @@ def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_r
     plain = _stats(plain_sols, plain_ttfs, time.perf_counter() - t0) if plain_sols else None
     if plain is None:
         t0 = time.perf_counter()
-        batch_sols = _lib.solve_batch(ctx, probs[:n_batch], [st_batch] * n_batch, [op] * n_batch)
+        batch_sols = _lib.solve_batch(ctx, probs[:n_batch], [st_batch] * n_batch, [op_batch] * n_batch)
         out["batched"] = _stats(batch_sols, [s.ttfs_s for s in batch_sols if s.solved], time.perf_counter() - t0)
         return out
     plain["ttfs_own_p50_s"] = float(np.percentile(own_ttfs, 50)) if own_ttfs else None
@@ -151,7 +163,7 @@ def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_r
     print(f"[bench] plain: {plain}", file=sys.stderr, flush=True)
     # batched: every problem in one engine, all submitted at t = 0
     t0 = time.perf_counter()
-    batch_sols = _lib.solve_batch(ctx, probs[:n_batch], [st_batch] * n_batch, [op] * n_batch)
+    batch_sols = _lib.solve_batch(ctx, probs[:n_batch], [st_batch] * n_batch, [op_batch] * n_batch)
     wall = time.perf_counter() - t0
     batched = _stats(batch_sols, [s.ttfs_s for s in batch_sols if s.solved], wall)
     both = [(a, b) for a, b in zip(plain_sols, batch_sols) if a.solved and b.solved]
@@ -160,11 +172,23 @@ def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_r
     print(f"[bench] batched: {batched}", file=sys.stderr, flush=True)
     out.update(plain=plain, batched=batched,
                batched_vs_plain_problems_per_s=batched["problems_per_s"] / plain["problems_per_s"])
-    out["serving"] = [serving_bench(ctx, checker, rate, serve_s, st_batch) for rate in serve_rates]
+    if n_budget:
+        # the batched configuration with a fixed 2 s budget per problem and
+        # the long optimisation passes of the 4-/5-arm configurations
+        bprobs = plan_problems(checker, n_budget)
+        st_b = _lib.settings(num_new_configs=tuned["num_new_configs"], **dict(common, optimize_iters=400000))
+        op_b = _lib.solve_options_default(time_budget_s=2.0,
+                                          **{k: v for k, v in tuned.items() if k != "num_new_configs"})
+        t0 = time.perf_counter()
+        sols = _lib.solve_batch(ctx, bprobs, [st_b] * n_budget, [op_b] * n_budget)
+        out["batched_2s_budget"] = _stats(sols, [s.ttfs_s for s in sols if s.solved], time.perf_counter() - t0)
+        print(f"[bench] batched, 2 s budget: {out['batched_2s_budget']}", file=sys.stderr, flush=True)
+    op_kw = {k: v for k, v in tuned.items() if k != "num_new_configs"}
+    out["serving"] = [serving_bench(ctx, checker, rate, serve_s, st_batch, op_kw) for rate in serve_rates]
     return out
 
 
-def serving_bench(ctx, checker, rate, duration_s, settings, deadlines=(0.05, 0.25, 1.0), seed=7):
+def serving_bench(ctx, checker, rate, duration_s, settings, option_kw, deadlines=(0.05, 0.25, 1.0), seed=7):
     """Poisson arrivals at `rate` req/s for `duration_s`; each request has a
     budget of the last deadline.  Reports TTFS from arrival and the cost of
     the best solution available at each deadline."""
@@ -172,8 +196,8 @@ def serving_bench(ctx, checker, rate, duration_s, settings, deadlines=(0.05, 0.2
     rng = np.random.default_rng(seed)
     n = max(1, int(rate * duration_s))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
-    probs = plan_problems(checker, n)
-    op = _lib.solve_options_default(time_budget_s=deadlines[-1])
+    probs = plan_problems(checker, n, first_seed=2000)
+    op = _lib.solve_options_default(time_budget_s=deadlines[-1], **option_kw)
     srv = _lib.Server(ctx, hbm_budget_bytes=200 * 10**9)
     tickets, done = [], {}
     t0 = time.perf_counter()
@@ -227,6 +251,9 @@ def main():
     ap.add_argument("--batch-nnc", type=int, default=256)
     ap.add_argument("--serve-rates", type=str, default="20,50")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--budget-problems", type=int, default=256,
+                    help="problems of the batched run with a 2 s budget each (0: skip)")
+    ap.add_argument("--no-tune", action="store_true", help="batched form with --batch-nnc instead of autotuning")
     ap.add_argument("--plan-only", action="store_true", help="only the planner measurements (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 256 MiB three times and exit: the reference kernel that scales the HBM counters")
@@ -255,7 +282,8 @@ def main():
     if args.plan_only:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         print(json.dumps(planning_bench(ctx, checker, args.plain_problems, args.batch_problems, args.plain_nnc,
-                                        args.batch_nnc, rates, args.serve_seconds)), flush=True)
+                                        args.batch_nnc, rates, args.serve_seconds, not args.no_tune,
+                                        args.budget_problems)), flush=True)
         return
     problem = scenes.make_problem("bin4", 0, checker)
     G = args.granularity
@@ -330,7 +358,8 @@ def main():
     if not args.no_plan:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         out["planning"] = planning_bench(ctx, checker, args.plain_problems, args.batch_problems, args.plain_nnc,
-                                         args.batch_nnc, rates, args.serve_seconds)
+                                         args.batch_nnc, rates, args.serve_seconds, not args.no_tune,
+                                         args.budget_problems)
     print(json.dumps(out), flush=True)
 
 

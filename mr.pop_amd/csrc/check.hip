@@ -49,16 +49,6 @@ namespace dev {
 
 using u64 = unsigned long long;
 
-template <int N> struct IntC {
-    static constexpr int value = N;
-};
-template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&f) {
-    if constexpr (B < E) {
-        f(IntC<B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
 }  // namespace dev
 }  // namespace mrpop
 #include "check_groups.h"
@@ -163,7 +153,7 @@ __device__ __forceinline__ Frame frame_rt(const WorldDesc &w, const EdgeBatch &e
     for (int J = 1; J <= jt; ++J) {
         const float qj = __builtin_fmaf(kf, (t[J - 1] - f[J - 1]) / gf, f[J - 1]);
         float sq, cq;
-        sincosf(qj, &sq, &cq);
+        joint_sincos(qj, sq, cq);
         const float *F = kJT.tf[J];
         float S[12];
 #pragma unroll
@@ -285,8 +275,10 @@ __device__ __forceinline__ bool fine_walk(const WorldDesc &w, const Frame &base,
                 static_for<0, kFineSelfRanges>([&](auto rc) {
                     constexpr int r = decltype(rc)::value;
                     constexpr int ps = range_pos<MODE>(r);
-                    if constexpr (kFineJoint[ps] == J)
-                        centre(T, kFineSphere[ps][0], kFineSphere[ps][1], kFineSphere[ps][2], px[r], py[r], pz[r]);
+                    if constexpr (kFineJoint[ps] == J) {
+                        constexpr float ox = kFineSphere[ps][0], oy = kFineSphere[ps][1], oz = kFineSphere[ps][2];
+                        centre(T, ox, oy, oz, px[r], py[r], pz[r]);
+                    }
                 });
             }
         }
@@ -295,9 +287,10 @@ __device__ __forceinline__ bool fine_walk(const WorldDesc &w, const Frame &base,
             if (envbits & (1u << J)) {
                 static_for<1, kCoarseSpheres>([&](auto kc) {
                     constexpr int k = decltype(kc)::value;
-                    if constexpr (kCoarseJoint[k] == J)
-                        hit |= block_env(w, T, k, cs[k][0][lane], cs[k][1][lane], cs[k][2][lane],
-                                         kCoarseSphere[k][3] + kBlockInflate[k]);
+                    if constexpr (kCoarseJoint[k] == J) {
+                        constexpr float R = kCoarseSphere[k][3] + kBlockInflate[k];
+                        hit |= block_env(w, T, k, cs[k][0][lane], cs[k][1][lane], cs[k][2][lane], R);
+                    }
                 });
             }
         }
@@ -410,7 +403,10 @@ __global__ __launch_bounds__(64) void check_kernel(const WorldDesc *worlds, cons
         interp7(eb, e, k, dof_off, q);
         Frame T;
         load_base(w, base_arm, T);
-        centre(T, kCoarseSphere[0][0], kCoarseSphere[0][1], kCoarseSphere[0][2], cx[0], cy[0], cz[0]);
+        {
+            constexpr float ox = kCoarseSphere[0][0], oy = kCoarseSphere[0][1], oz = kCoarseSphere[0][2];
+            centre(T, ox, oy, oz, cx[0], cy[0], cz[0]);
+        }
         static_for<1, kJoints>([&](auto jc) {
             constexpr int J = decltype(jc)::value;
             Frame N;
@@ -418,7 +414,8 @@ __global__ __launch_bounds__(64) void check_kernel(const WorldDesc *worlds, cons
             T = N;
             static_for<kCoarseJointStart[J], kCoarseJointStart[J] + kCoarseJointCount[J]>([&](auto sc) {
                 constexpr int s = decltype(sc)::value;
-                centre(T, kCoarseSphere[s][0], kCoarseSphere[s][1], kCoarseSphere[s][2], cx[s], cy[s], cz[s]);
+                constexpr float ox = kCoarseSphere[s][0], oy = kCoarseSphere[s][1], oz = kCoarseSphere[s][2];
+                centre(T, ox, oy, oz, cx[s], cy[s], cz[s]);
             });
         });
     }
@@ -454,9 +451,11 @@ __global__ __launch_bounds__(64) void check_kernel(const WorldDesc *worlds, cons
                      lz > bb[5] + g);
         };
         auto test_all = [&](auto &&fn) {
-#pragma unroll
-            for (int s = 1; s < kCoarseSpheres; ++s)
-                if (fn(cx[s], cy[s], cz[s], kCoarseSphere[s][3]) < 0.f) hitmask |= 1u << s;
+            static_for<1, kCoarseSpheres>([&](auto sc) {
+                constexpr int s = decltype(sc)::value;
+                constexpr float r = kCoarseSphere[s][3];
+                if (fn(cx[s], cy[s], cz[s], r) < 0.f) hitmask |= 1u << s;
+            });
         };
         int ob = 0;
         for (int i = 0; i < w.n_sph; ++i, ++ob) {
@@ -492,21 +491,25 @@ __global__ __launch_bounds__(64) void check_kernel(const WorldDesc *worlds, cons
         }
     }
     unsigned envbits = 0;
-#pragma unroll
-    for (int s = 1; s < kCoarseSpheres; ++s)
-        if (hitmask & (1u << s)) envbits |= 1u << kCoarseJoint[s];
+    static_for<1, kCoarseSpheres>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr unsigned bit = 1u << kCoarseJoint[s];
+        if (hitmask & (1u << s)) envbits |= bit;
+    });
 
     // coarse intra-arm pairs -> self bit of the first sphere's joint (panda_four.cuh:144-176)
     unsigned selfbits = 0;
-#pragma unroll
-    for (int r = 0; r < kCoarseSelfRanges; ++r) {
-        const int s1 = kCoarseSelfRange[r][0];
-#pragma unroll
-        for (int j = kCoarseSelfRange[r][1]; j <= kCoarseSelfRange[r][2]; ++j)
-            if (sphere_sphere(cx[s1], cy[s1], cz[s1], kCoarseSphere[s1][3], cx[j], cy[j], cz[j], kCoarseSphere[j][3]) <
-                0.f)
-                selfbits |= 1u << kCoarseJoint[s1];
-    }
+    static_for<0, kCoarseSelfRanges>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        constexpr int s1 = kCoarseSelfRange[r][0];
+        constexpr float r1 = kCoarseSphere[s1][3];
+        static_for<kCoarseSelfRange[r][1], kCoarseSelfRange[r][2] + 1>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            constexpr float rj = kCoarseSphere[j][3];
+            constexpr unsigned bit = 1u << kCoarseJoint[s1];
+            if (sphere_sphere(cx[s1], cy[s1], cz[s1], r1, cx[j], cy[j], cz[j], rj) < 0.f) selfbits |= bit;
+        });
+    });
     __syncthreads();
 
     // coarse arm-arm pairs (panda_four.cuh:178-214), A-1 half-pairs per lane.
@@ -534,15 +537,16 @@ __global__ __launch_bounds__(64) void check_kernel(const WorldDesc *worlds, cons
                 const float rad = kTab.coarse_r[i], brad = kTab.block_r[i];
                 bool hit = false;
                 unsigned cand = 0;
-#pragma unroll
-                for (int j = 0; j < kCoarseSpheres; ++j) {
+                static_for<0, kCoarseSpheres>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    constexpr float rj = kCoarseSphere[j][3], Rj = kCoarseSphere[j][3] + kBlockInflate[j];
                     const float dx = ax - bx[j], dy = ay - by[j], dz = az - bz[j];
                     const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                    const float rs = rad + kCoarseSphere[j][3];
-                    const float Rs = brad + (kCoarseSphere[j][3] + kBlockInflate[j]);
+                    const float rs = rad + rj;
+                    const float Rs = brad + Rj;
                     hit |= d2 - rs * rs < 0.f;
                     if (d2 < Rs * Rs) cand |= 1u << j;
-                }
+                });
                 if (hit) mask |= 1u << (i < 7 ? i : 7);
                 if (t < 5)
                     clo |= (unsigned long long)cand << (11 * t);

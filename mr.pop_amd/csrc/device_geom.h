@@ -11,6 +11,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "panda_geometry.h"
 
 namespace mrpop {
@@ -22,7 +24,9 @@ struct Frame {
     float m[12];  // row-major [R | t]
 };
 
-// k * x for a compile-time constant k: exact for k in {0, 1, -1}.
+// k * x for a compile-time constant k: exact for k in {0, 1, -1}.  Callers
+// pass k as a constexpr value (a literal in the IR), so the tests fold away;
+// a value read from a table in memory would turn them into selects.
 __device__ __forceinline__ constexpr float cmul(float k, float x) {
     return k == 0.f ? 0.f : (k == 1.f ? x : (k == -1.f ? -x : k * x));
 }
@@ -30,40 +34,59 @@ __device__ __forceinline__ constexpr float cfma(float k, float x, float acc) {
     return k == 0.f ? acc : (k == 1.f ? acc + x : (k == -1.f ? acc - x : __builtin_fmaf(k, x, acc)));
 }
 
+template <int V> using IntC = std::integral_constant<int, V>;
+
+template <int B, int E, class F> __device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(IntC<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // out = P * FT_J * Rz(q) with cq = cos(q), sq = sin(q).
 template <int J>
 __device__ __forceinline__ void joint_step(const Frame &P, float cq, float sq, Frame &out) {
-    constexpr const float *F = kJointTf[J];
     // S = FT * Rz: S[r][0] = F[r][0] c + F[r][1] s; S[r][1] = F[r][1] c - F[r][0] s;
     //              S[r][2] = F[r][2]; S[r][3] = F[r][3].
-    float S[12];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        S[4 * r + 0] = cfma(F[4 * r + 1], sq, cmul(F[4 * r + 0], cq));
-        S[4 * r + 1] = cfma(-F[4 * r + 0], sq, cmul(F[4 * r + 1], cq));
-    }
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
+    float S[6];
+    static_for<0, 3>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        constexpr float f0 = kJointTf[J][4 * r + 0], f1 = kJointTf[J][4 * r + 1];
+        S[2 * r + 0] = cfma(f1, sq, cmul(f0, cq));
+        S[2 * r + 1] = cfma(-f0, sq, cmul(f1, cq));
+    });
+    static_for<0, 3>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             float v = P.m[4 * r + 0] * S[k];
-            v = __builtin_fmaf(P.m[4 * r + 1], S[4 + k], v);
-            v = __builtin_fmaf(P.m[4 * r + 2], S[8 + k], v);
+            v = __builtin_fmaf(P.m[4 * r + 1], S[2 + k], v);
+            v = __builtin_fmaf(P.m[4 * r + 2], S[4 + k], v);
             out.m[4 * r + k] = v;
         }
-        // column 2: P * F[:,2] (constant)
-        out.m[4 * r + 2] = cfma(F[10], P.m[4 * r + 2], cfma(F[6], P.m[4 * r + 1], cmul(F[2], P.m[4 * r + 0])));
-        // translation: P * F[:,3] + P_t
-        out.m[4 * r + 3] =
-            cfma(F[11], P.m[4 * r + 2], cfma(F[7], P.m[4 * r + 1], cfma(F[3], P.m[4 * r + 0], P.m[4 * r + 3])));
-    }
+        // column 2: P * F[:,2]; translation: P * F[:,3] + P_t (constants)
+        constexpr float f2 = kJointTf[J][2], f6 = kJointTf[J][6], f10 = kJointTf[J][10];
+        constexpr float f3 = kJointTf[J][3], f7 = kJointTf[J][7], f11 = kJointTf[J][11];
+        out.m[4 * r + 2] = cfma(f10, P.m[4 * r + 2], cfma(f6, P.m[4 * r + 1], cmul(f2, P.m[4 * r + 0])));
+        out.m[4 * r + 3] = cfma(f11, P.m[4 * r + 2], cfma(f7, P.m[4 * r + 1], cfma(f3, P.m[4 * r + 0], P.m[4 * r + 3])));
+    });
+}
+
+// Sine and cosine of a joint angle with the hardware instructions
+// (v_sin_f32 / v_cos_f32), as the reference uses the fast __sinf / __cosf
+// (utils.cuh:334-335).  Over the joint range their error moves a sphere
+// centre by far less than the 1e-5 m tolerance of the checker, and they
+// avoid the large-argument branch of the full-range library sincos.
+__device__ __forceinline__ void joint_sincos(float q, float &s, float &c) {
+    s = __sinf(q);
+    c = __cosf(q);
 }
 
 // T_J from T_{J-1} and the joint angle q_{J-1}.
 template <int J>
 __device__ __forceinline__ void joint_frame(const Frame &P, float q, Frame &out) {
     float s, c;
-    sincosf(q, &s, &c);
+    joint_sincos(q, s, c);
     joint_step<J>(P, c, s, out);
 }
 
