@@ -361,3 +361,40 @@ int mrpop_server_cancel(mrpop_server *srv, uint64_t ticket) {
 size_t mrpop_server_bytes(const mrpop_server *srv) { return srv ? srv->engine->device_bytes() : 0; }
 
 }  // extern "C"
+
+// Diagnostics (not part of the public ABI): build the roadmaps of a problem
+// for `rounds` roadmap rounds and copy arm `arm`'s roadmap out (nodes as
+// [n][8] floats, degrees, neighbour lists [n][neighbour_capacity]); *n gets
+// the node count, at most `cap` nodes are copied.
+extern "C" int mrpop_diag_roadmap(mrpop_ctx *c, const mrpop_problem *p, const mrpop_settings *settings,
+                                  const mrpop_solve_options *options, int rounds, int arm, float *q, int *deg,
+                                  int *nbr, int *n, int cap) {
+    if (!c || !p || !n || rounds < 0) return MRPOP_ERR_INVALID;
+    Context *ctx = (Context *)c;
+    if (hipSetDevice(ctx->device) != hipSuccess) return MRPOP_ERR_HIP;
+    Request r;
+    if (int rc = copy_problem(*p, r)) return ctx->fail(rc, "invalid problem");
+    mrpop_settings st;
+    if (settings) st = *settings;
+    else mrpop_settings_default(&st);
+    Engine engine(ctx);
+    mrpop_scene sc{(int)(r.spheres.size() / 4), (int)(r.capsules.size() / 8), (int)(r.zcaps.size() / 8),
+                   (int)(r.cuboids.size() / 15), (int)(r.zcubs.size() / 15),
+                   r.spheres.data(), r.capsules.data(), r.zcaps.data(), r.cuboids.data(), r.zcubs.data()};
+    Slot *s = nullptr;
+    if (int rc = engine.add(sc, r.n_arms, r.bases.empty() ? nullptr : r.bases.data(), r.start.data(), r.n_goals,
+                            r.goals.data(), r.seed, st, to_options(options), &s))
+        return rc;
+    if (int rc = engine.debug_roadmap(s, rounds)) return rc;
+    return engine.debug_copy_roadmap(s, arm, q, deg, nbr, n, cap);
+}
+
+// Diagnostics (not part of the public ABI): the planner's counter-based
+// generator, Philox4x32-10, evaluated on the host.
+extern "C" void mrpop_diag_philox(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]) {
+    const uint4 r = philox4x32_10(make_uint4(counter[0], counter[1], counter[2], counter[3]), make_uint2(key[0], key[1]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+}

@@ -116,6 +116,10 @@ class Engine {
     size_t slot_bytes(int n_arms, const mrpop_settings &st, const SolveOptions &opt) const;
     const std::vector<Slot *> &slots() const { return slots_; }
     size_t device_bytes() const { return bytes_in_use_; }
+    // Diagnostics: run `rounds` roadmap rounds of slot s (no search) and copy
+    // arm a's roadmap out: nodes [n][8], degrees [n], neighbour lists [n][K].
+    int debug_roadmap(Slot *s, int rounds);
+    int debug_copy_roadmap(Slot *s, int arm, float *q, int *deg, int *nbr, int *n, int cap);
 
   private:
     struct Staged {

@@ -783,6 +783,35 @@ int Engine::step() {
     return MRPOP_OK;
 }
 
+int Engine::debug_roadmap(Slot *s, int rounds) {
+    if (int rc = ensure_lanes()) return rc;
+    if (int rc = drain()) return rc;
+    Lane &L = lanes_[0];
+    for (int r = 0; r < rounds; ++r) {
+        s->phase = Phase::Roadmap;
+        L.rm.assign(1, s);
+        if (int rc = launch_lane(L)) return rc;
+        MRPOP_HIP(c_, hipEventSynchronize(L.done));
+        if (int rc = process_lane(L)) return rc;
+    }
+    return MRPOP_OK;
+}
+
+int Engine::debug_copy_roadmap(Slot *s, int arm, float *q, int *deg, int *nbr, int *n, int cap) {
+    if (arm < 0 || arm >= s->A) return c_->fail(MRPOP_ERR_INVALID, "arm %d outside the problem's arms", arm);
+    int count = 0;
+    MRPOP_HIP(c_, hipMemcpy(&count, s->dv.rm_n + arm, sizeof(int), hipMemcpyDeviceToHost));
+    *n = count;
+    const int m = std::min(count, cap);
+    const size_t off = (size_t)arm * s->dv.rm_cap;
+    if (q && m) MRPOP_HIP(c_, hipMemcpy(q, s->dv.rm_q + off * 8, sizeof(float) * 8 * m, hipMemcpyDeviceToHost));
+    if (deg && m) MRPOP_HIP(c_, hipMemcpy(deg, s->dv.rm_deg + off, sizeof(int) * m, hipMemcpyDeviceToHost));
+    if (nbr && m)
+        MRPOP_HIP(c_, hipMemcpy(nbr, s->dv.rm_nbr + off * s->dv.rm_K, sizeof(int) * (size_t)m * s->dv.rm_K,
+                                hipMemcpyDeviceToHost));
+    return MRPOP_OK;
+}
+
 // End of an AO-x pass: keep an improving path (pop_planner.cu:1000-1022),
 // then start the next pass or finish.
 int Engine::finish_pass(Slot *s, const SlotReport &r) {
