@@ -353,8 +353,10 @@ constexpr float env_radius_max() {
 }
 constexpr float kEnvRadiusMax = env_radius_max();
 
+// Three waves per SIMD: the register budget (168 VGPRs) costs a few spills
+// but hides more latency; measured faster than two (no spills) and four.
 template <int A, int MODE>
-__global__ __launch_bounds__(64) void check_kernel(const WorldDesc *worlds, const EdgeGroup *groups, EdgeGroup g0,
+__global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, const EdgeGroup *groups, EdgeGroup g0,
                                                    int n_groups, const unsigned *wave_start, FineQueue fq) {
     constexpr int S = 64 / A;  // samples per wave
     constexpr bool kEnv = MODE != MRPOP_MODE_DRRT;
