@@ -367,17 +367,45 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
     __shared__ unsigned result[64];
     __shared__ unsigned long long cand_lo[kHalf][64];  // candidate blocks, rows 0..4: bit t*11 + k
     __shared__ unsigned cand_hi[kHalf][64];            // row 5: bit k
+    __shared__ unsigned edge_done[64];                 // per edge of the tile: a sample collides
 
     const int lane = threadIdx.x;
     const int slot = lane / A;
     const int arm = lane - slot * A;
     const int sbase = slot < S ? slot * A : 0;  // spare lanes (64 % A) shadow slot 0 and discard
-    const unsigned total_waves =
-        groups ? wave_start[n_groups] : (unsigned)(((long)g0.n_edges * g0.G + S - 1) / S);
+    // First wave of each group.  Up to 64 groups the wave computes the prefix
+    // itself, one group per lane (wave_start is then null); more groups come
+    // with a prefix from wave_prefix_kernel.
+    unsigned lane_start = 0, total_waves;
+    if (!groups) {
+        total_waves = (unsigned)(((long)g0.n_edges * g0.G + S - 1) / S);
+    } else if (!wave_start) {
+        unsigned wcount = 0;
+        if (lane < n_groups) {
+            const EdgeGroup &g = groups[lane];
+            const unsigned long long samples = (unsigned long long)(g.count ? *g.count : g.n_edges) * g.G;
+            wcount = (unsigned)((samples + S - 1) / S);
+        }
+        unsigned incl = wcount;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+        }
+        lane_start = incl - wcount;
+        total_waves = __shfl(incl, 63);
+    } else {
+        total_waves = wave_start[n_groups];
+    }
 
     for (unsigned wv = blockIdx.x; wv < total_waves; wv += gridDim.x) {
     int gi = 0;
-    if (groups) {  // last group whose first wave is <= wv
+    unsigned gstart = 0;
+    if (groups && !wave_start) {  // last group whose first wave is <= wv (empty groups share starts)
+        const unsigned long long m = __ballot(lane < n_groups && lane_start <= wv);
+        gi = 63 - __builtin_clzll(m);
+        gstart = __shfl(lane_start, gi);
+    } else if (groups) {
         int lo = 0, hi = n_groups - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -385,10 +413,11 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
             else hi = mid - 1;
         }
         gi = lo;
+        gstart = wave_start[gi];
     }
     const EdgeGroup eb = groups ? groups[gi] : g0;
     const WorldDesc w = worlds[eb.world];
-    const unsigned lw = groups ? wv - wave_start[gi] : wv;
+    const unsigned lw = wv - gstart;
     const long total = (long)(eb.count ? *eb.count : eb.n_edges) * eb.G;
     const long n = (long)lw * S + slot;
     const bool active = slot < S && n < total;
@@ -566,6 +595,16 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
     result[lane] = selfbits;
     __syncthreads();
 
+    // Edge-only verdicts (no per-sample mask, no diagnostics): once one
+    // sample of an edge collides, the fine tests left for that edge's lanes
+    // cannot change its verdict and are skipped.  A tile holds the samples
+    // of at most S consecutive edges; le is this lane's edge within it.
+    const bool early_out = !eb.out_mask && !fq.diag;
+    const int le = e - (int)(((long)lw * S) / eb.G);
+    volatile unsigned *done = edge_done;
+    edge_done[lane] = 0u;
+    __syncthreads();
+
     // ---- B. fine intra-arm pairs of this lane's arm
     bool finehit = false;
     if (active && (selfbits | envbits)) {
@@ -576,11 +615,13 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
         if (selfbits & 0x1fu) finehit = fine_walk<MODE, 0, 4, false>(w, base, q, selfbits, 0u, cs, lane);
         if ((selfbits & 0x20u) || envbits) finehit |= fine_walk<MODE, 5, 5, kEnv>(w, base, q, selfbits, envbits, cs, lane);
     }
+    if (active && finehit) done[le] = 1u;
 
     // ---- C. fine arm-arm pairs for candidate blocks of this lane's half-pairs
     if constexpr (kInter) {
 #pragma unroll 1
         for (int m = 0; m < A - 1; ++m) {
+            if (early_out && active && done[le]) break;
             int a, b, i0;
             half_pair<A>(arm + m * A, a, b, i0);
             const unsigned gate = result[sbase + a];  // arm a's self bits
@@ -620,6 +661,8 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
                         finehit |= sphere_sphere(ax, ay, az, ar, x, y, z, kTab.r[j]) < 0.f;
                     }
                 }
+                if (finehit) done[le] = 1u;
+                if (early_out && done[le]) break;
             }
         }
     }
@@ -743,8 +786,10 @@ void enqueue_check(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_group
                            dim3(d_groups ? 4 : std::max(1, std::min(1024, (g0.n_edges + 255) / 256)),
                                 std::min(n_groups, 1024)),
                            dim3(256), 0, stream, d_groups, g0, n_groups);
-    if (d_groups)
+    // up to 64 groups the check kernel computes the group prefix itself
+    if (d_groups && n_groups > 64)
         hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups, S, d_wave_start);
+    if (n_groups <= 64) d_wave_start = nullptr;
     hipLaunchKernelGGL((dev::check_kernel<A, MODE>), dim3(check_blocks), dim3(64), 0, stream, d_worlds, d_groups, g0,
                        n_groups, d_wave_start, aux);
 }
