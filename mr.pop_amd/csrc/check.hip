@@ -353,71 +353,37 @@ constexpr float env_radius_max() {
 }
 constexpr float kEnvRadiusMax = env_radius_max();
 
-// Three waves per SIMD: the register budget (168 VGPRs) costs a few spills
-// but hides more latency; measured faster than two (no spills) and four.
-template <int A, int MODE>
-__global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, const EdgeGroup *groups, EdgeGroup g0,
-                                                   int n_groups, const unsigned *wave_start, FineQueue fq) {
+// Shared memory of one wave tile; KH = arm-arm half-pairs per lane.
+template <int KH> struct TileLds {
+    float cs[kCoarseSpheres][3][64];
+    unsigned inter_flags[64];
+    unsigned result[64];
+    unsigned long long cand_lo[KH][64];  // candidate blocks, rows 0..4: bit t*11 + k
+    unsigned cand_hi[KH][64];            // row 5: bit k
+    unsigned edge_done[64];              // per edge of the tile: a sample collides
+};
+
+// Waves of a group: ceil(samples / (64 / A)), A from the group's dimension
+// (one arm in SINGLE mode).
+__device__ __forceinline__ unsigned group_waves(const EdgeGroup &g, int single) {
+    const int S = single ? 64 : 64 / (g.dim / 7);
+    const unsigned long long samples = (unsigned long long)(g.count ? *g.count : g.n_edges) * g.G;
+    return (unsigned)((samples + S - 1) / S);
+}
+
+// One wave tile: samples lw*S .. lw*S + S - 1 of group eb, every (sample,
+// arm) on its own lane.
+template <int A, int MODE, int KH>
+__device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *worlds, const EdgeGroup &eb,
+                                           unsigned lw, const FineQueue &fq) {
     constexpr int S = 64 / A;  // samples per wave
     constexpr bool kEnv = MODE != MRPOP_MODE_DRRT;
     constexpr bool kInter = A > 1;
-    constexpr int kHalf = kInter ? A - 1 : 1;
-    __shared__ float cs[kCoarseSpheres][3][64];
-    __shared__ unsigned inter_flags[64];
-    __shared__ unsigned result[64];
-    __shared__ unsigned long long cand_lo[kHalf][64];  // candidate blocks, rows 0..4: bit t*11 + k
-    __shared__ unsigned cand_hi[kHalf][64];            // row 5: bit k
-    __shared__ unsigned edge_done[64];                 // per edge of the tile: a sample collides
-
     const int lane = threadIdx.x;
     const int slot = lane / A;
     const int arm = lane - slot * A;
     const int sbase = slot < S ? slot * A : 0;  // spare lanes (64 % A) shadow slot 0 and discard
-    // First wave of each group.  Up to 64 groups the wave computes the prefix
-    // itself, one group per lane (wave_start is then null); more groups come
-    // with a prefix from wave_prefix_kernel.
-    unsigned lane_start = 0, total_waves;
-    if (!groups) {
-        total_waves = (unsigned)(((long)g0.n_edges * g0.G + S - 1) / S);
-    } else if (!wave_start) {
-        unsigned wcount = 0;
-        if (lane < n_groups) {
-            const EdgeGroup &g = groups[lane];
-            const unsigned long long samples = (unsigned long long)(g.count ? *g.count : g.n_edges) * g.G;
-            wcount = (unsigned)((samples + S - 1) / S);
-        }
-        unsigned incl = wcount;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const unsigned y = __shfl_up(incl, off);
-            if (lane >= off) incl += y;
-        }
-        lane_start = incl - wcount;
-        total_waves = __shfl(incl, 63);
-    } else {
-        total_waves = wave_start[n_groups];
-    }
-
-    for (unsigned wv = blockIdx.x; wv < total_waves; wv += gridDim.x) {
-    int gi = 0;
-    unsigned gstart = 0;
-    if (groups && !wave_start) {  // last group whose first wave is <= wv (empty groups share starts)
-        const unsigned long long m = __ballot(lane < n_groups && lane_start <= wv);
-        gi = 63 - __builtin_clzll(m);
-        gstart = __shfl(lane_start, gi);
-    } else if (groups) {
-        int lo = 0, hi = n_groups - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (wave_start[mid] <= wv) lo = mid;
-            else hi = mid - 1;
-        }
-        gi = lo;
-        gstart = wave_start[gi];
-    }
-    const EdgeGroup eb = groups ? groups[gi] : g0;
     const WorldDesc w = worlds[eb.world];
-    const unsigned lw = wv - gstart;
     const long total = (long)(eb.count ? *eb.count : eb.n_edges) * eb.G;
     const long n = (long)lw * S + slot;
     const bool active = slot < S && n < total;
@@ -452,11 +418,11 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
     }
 #pragma unroll
     for (int s = 0; s < kCoarseSpheres; ++s) {
-        cs[s][0][lane] = cx[s];
-        cs[s][1][lane] = cy[s];
-        cs[s][2][lane] = cz[s];
+        sm.cs[s][0][lane] = cx[s];
+        sm.cs[s][1][lane] = cy[s];
+        sm.cs[s][2][lane] = cz[s];
     }
-    inter_flags[lane] = 0u;
+    sm.inter_flags[lane] = 0u;
 
     // coarse environment: spheres 1..10 -> env bit of their joint (panda_four.cuh:241-261)
     unsigned hitmask = 0;
@@ -554,9 +520,9 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
             float bx[kCoarseSpheres], by[kCoarseSpheres], bz[kCoarseSpheres];
 #pragma unroll
             for (int j = 0; j < kCoarseSpheres; ++j) {
-                bx[j] = cs[j][0][sbase + b];
-                by[j] = cs[j][1][sbase + b];
-                bz[j] = cs[j][2][sbase + b];
+                bx[j] = sm.cs[j][0][sbase + b];
+                by[j] = sm.cs[j][1][sbase + b];
+                bz[j] = sm.cs[j][2][sbase + b];
             }
             unsigned mask = 0;
             unsigned long long clo = 0;
@@ -564,7 +530,7 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
 #pragma unroll 1
             for (int t = 0; t < 6; ++t) {
                 const int i = min(i0 + t, kCoarseSpheres - 1);  // rows 6..10 are 5: repeat row 10
-                const float ax = cs[i][0][sbase + a], ay = cs[i][1][sbase + a], az = cs[i][2][sbase + a];
+                const float ax = sm.cs[i][0][sbase + a], ay = sm.cs[i][1][sbase + a], az = sm.cs[i][2][sbase + a];
                 const float rad = kTab.coarse_r[i], brad = kTab.block_r[i];
                 bool hit = false;
                 unsigned cand = 0;
@@ -584,15 +550,15 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
                 else if (i0 == 0)
                     chi = cand;
             }
-            cand_lo[m][lane] = active ? clo : 0ull;
-            cand_hi[m][lane] = active ? chi : 0u;
-            if (mask && slot < S) atomicOr(&inter_flags[sbase + a], mask);
+            sm.cand_lo[m][lane] = active ? clo : 0ull;
+            sm.cand_hi[m][lane] = active ? chi : 0u;
+            if (mask && slot < S) atomicOr(&sm.inter_flags[sbase + a], mask);
         }
     }
     __syncthreads();
-    if constexpr (kInter) selfbits |= inter_flags[lane];
+    if constexpr (kInter) selfbits |= sm.inter_flags[lane];
     // publish the final self bits of every arm for the arm-arm fine stage
-    result[lane] = selfbits;
+    sm.result[lane] = selfbits;
     __syncthreads();
 
     // Edge-only verdicts (no per-sample mask, no diagnostics): once one
@@ -601,8 +567,8 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
     // of at most S consecutive edges; le is this lane's edge within it.
     const bool early_out = !eb.out_mask && !fq.diag;
     const int le = e - (int)(((long)lw * S) / eb.G);
-    volatile unsigned *done = edge_done;
-    edge_done[lane] = 0u;
+    volatile unsigned *done = sm.edge_done;
+    sm.edge_done[lane] = 0u;
     __syncthreads();
 
     // ---- B. fine intra-arm pairs of this lane's arm
@@ -612,8 +578,8 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
         interp7(eb, e, k, dof_off, q);
         Frame base;
         load_base(w, base_arm, base);
-        if (selfbits & 0x1fu) finehit = fine_walk<MODE, 0, 4, false>(w, base, q, selfbits, 0u, cs, lane);
-        if ((selfbits & 0x20u) || envbits) finehit |= fine_walk<MODE, 5, 5, kEnv>(w, base, q, selfbits, envbits, cs, lane);
+        if (selfbits & 0x1fu) finehit = fine_walk<MODE, 0, 4, false>(w, base, q, selfbits, 0u, sm.cs, lane);
+        if ((selfbits & 0x20u) || envbits) finehit |= fine_walk<MODE, 5, 5, kEnv>(w, base, q, selfbits, envbits, sm.cs, lane);
     }
     if (active && finehit) done[le] = 1u;
 
@@ -624,9 +590,9 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
             if (early_out && active && done[le]) break;
             int a, b, i0;
             half_pair<A>(arm + m * A, a, b, i0);
-            const unsigned gate = result[sbase + a];  // arm a's self bits
-            unsigned long long clo = cand_lo[m][lane];
-            unsigned chi = cand_hi[m][lane];
+            const unsigned gate = sm.result[sbase + a];  // arm a's self bits
+            unsigned long long clo = sm.cand_lo[m][lane];
+            unsigned chi = sm.cand_hi[m][lane];
             // keep rows whose block's joint is self-flagged on arm a
 #pragma unroll
             for (int t = 0; t < 5; ++t) {
@@ -671,12 +637,12 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
     if (fq.diag && active)
         fq.diag[(size_t)n * 8 + arm] = envbits | (selfbits << 8) | (finehit ? 1u << 16 : 0u) | (1u << 17);
     __syncthreads();
-    result[lane] = finehit ? 1u : 0u;
+    sm.result[lane] = finehit ? 1u : 0u;
     __syncthreads();
     if (active && arm == 0) {
         bool coll = false;
 #pragma unroll
-        for (int a = 0; a < A; ++a) coll |= result[slot * A + a] != 0u;
+        for (int a = 0; a < A; ++a) coll |= sm.result[slot * A + a] != 0u;
         if (coll) {
             atomicOr(&eb.out_edge[e], 1u);
             if (eb.out_mask) atomicOr(&eb.out_mask[e], 1ull << (k - 1));
@@ -684,7 +650,84 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
         if (fq.diag) fq.diag[(size_t)n * 8 + 7] = (coll ? 1u : 0u) | (1u << 17);
     }
     __syncthreads();  // LDS is reused by the next wave-tile
+}
+
+// Persistent loop over the wave tiles of n_groups edge groups (or of the
+// single group g0 when groups is null).  Up to 64 groups the wave computes
+// the groups' first waves itself, one group per lane (wave_start is then
+// null); more groups come with a prefix from wave_prefix_kernel.  A == 0:
+// groups of different arm counts (multi-arm modes), dispatched per tile.
+template <int MODE, class Tile>
+__device__ __forceinline__ void check_loop(const EdgeGroup *groups, const EdgeGroup &g0, int n_groups,
+                                           const unsigned *wave_start, Tile &&tile) {
+    const int lane = threadIdx.x;
+    unsigned lane_start = 0, total_waves;
+    if (!groups) {
+        total_waves = group_waves(g0, MODE == MRPOP_MODE_SINGLE);
+    } else if (!wave_start) {
+        const unsigned wcount = lane < n_groups ? group_waves(groups[lane], MODE == MRPOP_MODE_SINGLE) : 0u;
+        unsigned incl = wcount;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+        }
+        lane_start = incl - wcount;
+        total_waves = __shfl(incl, 63);
+    } else {
+        total_waves = wave_start[n_groups];
     }
+    for (unsigned wv = blockIdx.x; wv < total_waves; wv += gridDim.x) {
+        int gi = 0;
+        unsigned gstart = 0;
+        if (groups && !wave_start) {  // last group whose first wave is <= wv (empty groups share starts)
+            const unsigned long long m = __ballot(lane < n_groups && lane_start <= wv);
+            gi = 63 - __builtin_clzll(m);
+            gstart = __shfl(lane_start, gi);
+        } else if (groups) {
+            int lo = 0, hi = n_groups - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (wave_start[mid] <= wv) lo = mid;
+                else hi = mid - 1;
+            }
+            gi = lo;
+            gstart = wave_start[gi];
+        }
+        const EdgeGroup eb = groups ? groups[gi] : g0;
+        tile(eb, wv - gstart);
+    }
+}
+
+// Three waves per SIMD: the register budget (168 VGPRs) costs a few spills
+// but hides more latency; measured faster than two (no spills) and four.
+template <int A, int MODE>
+__global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, const EdgeGroup *groups, EdgeGroup g0,
+                                                   int n_groups, const unsigned *wave_start, FineQueue fq) {
+    constexpr int KH = A > 1 ? A - 1 : 1;
+    __shared__ TileLds<KH> sm;
+    check_loop<MODE>(groups, g0, n_groups, wave_start, [&](const EdgeGroup &eb, unsigned lw) {
+        check_tile<A, MODE, KH>(sm, worlds, eb, lw, fq);
+    });
+}
+
+// Multi-arm groups of 2..5 arms in one launch: each tile runs the code of
+// its group's arm count, so a planner round checks all its problems in one
+// launch per mode instead of one per arm count.
+template <int MODE>
+__global__ __launch_bounds__(64, 3) void check_kernel_mixed(const WorldDesc *worlds, const EdgeGroup *groups,
+                                                         int n_groups, const unsigned *wave_start) {
+    __shared__ TileLds<4> sm;
+    const EdgeGroup none{};
+    const FineQueue fq{nullptr};
+    check_loop<MODE>(groups, none, n_groups, wave_start, [&](const EdgeGroup &eb, unsigned lw) {
+        switch (eb.dim / 7) {
+            case 2: check_tile<2, MODE, 4>(sm, worlds, eb, lw, fq); break;
+            case 3: check_tile<3, MODE, 4>(sm, worlds, eb, lw, fq); break;
+            case 4: check_tile<4, MODE, 4>(sm, worlds, eb, lw, fq); break;
+            default: check_tile<5, MODE, 4>(sm, worlds, eb, lw, fq); break;
+        }
+    });
 }
 
 // ---------------------------------------------------------- centres kernel --
@@ -748,19 +791,14 @@ __global__ void zero_outputs_kernel(const dev::EdgeGroup *groups, dev::EdgeGroup
     }
 }
 
-// Waves of each group (ceil(samples / S)) -> exclusive prefix; wave_start[n] = total.
-__global__ void wave_prefix_kernel(const dev::EdgeGroup *groups, int n_groups, int S, unsigned *wave_start) {
+// Waves of each group -> exclusive prefix; wave_start[n] = total.
+__global__ void wave_prefix_kernel(const dev::EdgeGroup *groups, int n_groups, int single, unsigned *wave_start) {
     __shared__ unsigned sh[256];
     const int tid = threadIdx.x;
     unsigned carry = 0;
     for (int base = 0; base < n_groups; base += 256) {
         const int i = base + tid;
-        unsigned v = 0;
-        if (i < n_groups) {
-            const dev::EdgeGroup &g = groups[i];
-            const unsigned long long samples = (unsigned long long)(g.count ? *g.count : g.n_edges) * g.G;
-            v = (unsigned)((samples + S - 1) / S);
-        }
+        const unsigned v = i < n_groups ? dev::group_waves(groups[i], single) : 0u;
         sh[tid] = v;
         __syncthreads();
         for (int off = 1; off < 256; off <<= 1) {
@@ -780,7 +818,6 @@ template <int A, int MODE>
 void enqueue_check(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups, const dev::EdgeGroup &g0,
                    int n_groups, unsigned *d_wave_start, const FineQueue &aux, int check_blocks, bool zero,
                    hipStream_t stream) {
-    constexpr int S = 64 / A;
     if (zero)
         hipLaunchKernelGGL(zero_outputs_kernel,
                            dim3(d_groups ? 4 : std::max(1, std::min(1024, (g0.n_edges + 255) / 256)),
@@ -788,10 +825,25 @@ void enqueue_check(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_group
                            dim3(256), 0, stream, d_groups, g0, n_groups);
     // up to 64 groups the check kernel computes the group prefix itself
     if (d_groups && n_groups > 64)
-        hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups, S, d_wave_start);
+        hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups,
+                           MODE == MRPOP_MODE_SINGLE ? 1 : 0, d_wave_start);
     if (n_groups <= 64) d_wave_start = nullptr;
     hipLaunchKernelGGL((dev::check_kernel<A, MODE>), dim3(check_blocks), dim3(64), 0, stream, d_worlds, d_groups, g0,
                        n_groups, d_wave_start, aux);
+}
+
+// Groups of several arm counts (multi-arm modes) in one launch.
+template <int MODE>
+void enqueue_mixed(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups, int n_groups,
+                   unsigned *d_wave_start, int check_blocks, bool zero, hipStream_t stream) {
+    const dev::EdgeGroup none{};
+    if (zero)
+        hipLaunchKernelGGL(zero_outputs_kernel, dim3(4, std::min(n_groups, 1024)), dim3(256), 0, stream, d_groups,
+                           none, n_groups);
+    if (n_groups > 64)
+        hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups, 0, d_wave_start);
+    hipLaunchKernelGGL((dev::check_kernel_mixed<MODE>), dim3(check_blocks), dim3(64), 0, stream, d_worlds, d_groups,
+                       n_groups, n_groups > 64 ? d_wave_start : nullptr);
 }
 
 int dispatch(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
@@ -811,6 +863,14 @@ int dispatch(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const 
         MRPOP_CASE(MRPOP_MODE_DRRT, 3)
         MRPOP_CASE(MRPOP_MODE_DRRT, 4)
         MRPOP_CASE(MRPOP_MODE_DRRT, 5)
+        case MRPOP_MODE_FULL * 8 + 0:  // groups of several arm counts
+            if (!d_groups) return c->fail(MRPOP_ERR_INVALID, "a mixed-arm check needs edge groups");
+            enqueue_mixed<MRPOP_MODE_FULL>(d_worlds, d_groups, n_groups, d_wave_start, check_blocks, zero, stream);
+            break;
+        case MRPOP_MODE_DRRT * 8 + 0:
+            if (!d_groups) return c->fail(MRPOP_ERR_INVALID, "a mixed-arm check needs edge groups");
+            enqueue_mixed<MRPOP_MODE_DRRT>(d_worlds, d_groups, n_groups, d_wave_start, check_blocks, zero, stream);
+            break;
         default: return c->fail(MRPOP_ERR_INVALID, "unsupported mode %d with %d arms", mode, A);
     }
 #undef MRPOP_CASE

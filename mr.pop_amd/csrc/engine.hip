@@ -507,7 +507,7 @@ int *Engine::upload_ids(Lane &L, const std::vector<Slot *> &list) {
 }
 
 // Copy the edge-group descriptors of `list` (per_slot groups each) to the
-// device, bucketed by arm count for the multi-arm modes.  The descriptors
+// device as one bucket (one launch).  The descriptors
 // hold device pointers and counts only, so one copy serves every round of a
 // step.
 int Engine::stage_groups(Lane &L, int mode, const std::vector<const dev::EdgeGroup *> &groups, int per_slot,
@@ -515,29 +515,32 @@ int Engine::stage_groups(Lane &L, int mode, const std::vector<const dev::EdgeGro
     out.mode = mode;
     out.buckets.clear();
     const size_t begin = L.group_cursor;
-    for (int A = (mode == MRPOP_MODE_SINGLE ? 1 : 2); A <= (mode == MRPOP_MODE_SINGLE ? 1 : 5); ++A) {
-        Staged::Bucket bk{A, L.group_cursor, 0, 0};
-        const int S = 64 / A;
-        for (size_t i = 0; i < list.size(); ++i) {
-            if (mode != MRPOP_MODE_SINGLE && list[i]->A != A) continue;
-            const int n = mode == MRPOP_MODE_SINGLE ? list[i]->A : 1;
-            for (int k = 0; k < n && k < per_slot; ++k) {
-                if (L.group_cursor >= group_cap_) return c_->fail(MRPOP_ERR_BUDGET, "edge-group staging full");
-                const dev::EdgeGroup &g = groups[i][k];
-                L.h_groups[L.group_cursor++] = g;
-                bk.max_waves += ((size_t)g.n_edges * g.G + S - 1) / S;
-            }
+    // one launch: SINGLE mode checks one arm per group; the multi-arm modes
+    // use the kernel of the arm count when all problems share it, else the
+    // kernel that mixes arm counts (check_kernel_mixed)
+    int A_all = mode == MRPOP_MODE_SINGLE ? 1 : (list.empty() ? 0 : list[0]->A);
+    for (Slot *s : list)
+        if (mode != MRPOP_MODE_SINGLE && s->A != A_all) A_all = 0;
+    Staged::Bucket bk{A_all, L.group_cursor, 0, 0};
+    for (size_t i = 0; i < list.size(); ++i) {
+        const int n = mode == MRPOP_MODE_SINGLE ? list[i]->A : 1;
+        const int S = mode == MRPOP_MODE_SINGLE ? 64 : 64 / list[i]->A;
+        for (int k = 0; k < n && k < per_slot; ++k) {
+            if (L.group_cursor >= group_cap_) return c_->fail(MRPOP_ERR_BUDGET, "edge-group staging full");
+            const dev::EdgeGroup &g = groups[i][k];
+            L.h_groups[L.group_cursor++] = g;
+            bk.max_waves += ((size_t)g.n_edges * g.G + S - 1) / S;
         }
-        bk.n = (int)(L.group_cursor - bk.first);
-        if (bk.n) out.buckets.push_back(bk);
     }
+    bk.n = (int)(L.group_cursor - bk.first);
+    if (bk.n) out.buckets.push_back(bk);
     if (L.group_cursor > begin)
         MRPOP_HIP(c_, hipMemcpyAsync(L.d_groups + begin, L.h_groups + begin,
                                      sizeof(dev::EdgeGroup) * (L.group_cursor - begin), hipMemcpyHostToDevice, L.stream));
     return MRPOP_OK;
 }
 
-// One batched check launch per arm-count bucket.  Every producer of planner
+// One batched check launch per bucket.  Every producer of planner
 // edges clears their verdict words, so the launches skip the zeroing pass.
 int Engine::launch_staged(Lane &L, const Staged &st) {
     for (const Staged::Bucket &bk : st.buckets)

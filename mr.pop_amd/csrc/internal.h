@@ -76,7 +76,8 @@ int launch_check(Context *c, const World *w, int mode, int arm, int granularity,
 // of one launch uses the same mode and arm count A; d_wave_start holds
 // n_groups + 1 entries of workspace.  max_waves bounds the persistent grid
 // (an upper bound of the waves the groups can need); zero = false when the
-// producers of the edges already cleared the verdict words.
+// producers of the edges already cleared the verdict words.  A == 0 in a
+// multi-arm mode: groups of 2..5 arms (from each group's dim) in one launch.
 namespace dev {
 struct EdgeGroup;
 }
