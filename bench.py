@@ -119,7 +119,7 @@ def _stats(sols, ttfs, wall):
 
 
 def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_rates, serve_s, tune=True,
-                   n_budget=0, batch_kc=16):
+                   n_budget=0, batch_kc=16, batch_probe=0):
     """Plain vs batched planning on the same seeded problems, a budgeted
     batch, and serving."""
     from mrpop import _lib, autotune
@@ -133,6 +133,8 @@ def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_r
                        "one start and one goal each"}
     # batched-form settings: autotuned on a calibration set of other seeds
     tuned = {"num_new_configs": batch_nnc, "neighbour_candidates": batch_kc}
+    if batch_probe:
+        tuned["probe_samples"] = batch_probe
     if tune:
         calib = plan_problems(checker, 16, first_seed=1000)
         res = autotune.tune(ctx, calib, common, op_common)
@@ -256,6 +258,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="batched form with --batch-nnc / --batch-candidates instead of autotuning")
     ap.add_argument("--batch-candidates", type=int, default=16, help="roadmap candidates per node, untuned runs")
+    ap.add_argument("--batch-probe", type=int, default=0, help="probe samples per planner edge, untuned runs")
     ap.add_argument("--plan-only", action="store_true", help="only the planner measurements (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 256 MiB three times and exit: the reference kernel that scales the HBM counters")
@@ -285,7 +288,8 @@ def main():
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         print(json.dumps(planning_bench(ctx, checker, args.plain_problems, args.batch_problems, args.plain_nnc,
                                         args.batch_nnc, rates, args.serve_seconds, not args.no_tune,
-                                        args.budget_problems, args.batch_candidates)), flush=True)
+                                        args.budget_problems, args.batch_candidates, args.batch_probe)),
+              flush=True)
         return
     problem = scenes.make_problem("bin4", 0, checker)
     G = args.granularity
@@ -361,7 +365,7 @@ def main():
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         out["planning"] = planning_bench(ctx, checker, args.plain_problems, args.batch_problems, args.plain_nnc,
                                          args.batch_nnc, rates, args.serve_seconds, not args.no_tune,
-                                         args.budget_problems, args.batch_candidates)
+                                         args.budget_problems, args.batch_candidates, args.batch_probe)
     print(json.dumps(out), flush=True)
 
 

@@ -189,6 +189,8 @@ typedef struct mrpop_solve_options {
     int neighbour_capacity;    /* adjacency entries per roadmap node (default 128) */
     int neighbour_candidates;  /* radius neighbours tried per new roadmap node, 1..64 (default 64) */
     int rounds_per_step;       /* dRRT rounds queued between host checks (default 8) */
+    int probe_samples;         /* > 0: planner edges are first checked at this many samples spread along
+                                  them, then in full only if still free; same verdicts (default 0) */
 } mrpop_solve_options;
 
 int mrpop_solve_options_default(mrpop_solve_options *o);

@@ -416,6 +416,19 @@ extern "C" int mrpop_diag_check_edges(mrpop_ctx *c, const mrpop_world *w, int mo
     return rc;
 }
 
+// Diagnostics (not part of the public ABI): a host-buffer edge check (edge
+// verdicts only) run as a probe pass of `probe` samples per edge followed by
+// the remaining samples of the edges the probe left free, as the planner
+// checks its edges.
+extern "C" int mrpop_diag_check_probe(mrpop_ctx *c, const mrpop_world *w, int mode, int arm, int granularity,
+                                      int n_edges, const float *from, const float *to, int probe, uint8_t *out_edge) {
+    if (!c || !w || probe < 1 || probe > 64) return MRPOP_ERR_INVALID;
+    c->probe = probe;
+    const int rc = mrpop_check_edges(c, w, mode, arm, granularity, n_edges, from, to, out_edge, nullptr);
+    c->probe = 0;
+    return rc;
+}
+
 // Diagnostics (not part of the public ABI): the cull boxes the checker uses
 // for a scene, 8 floats per obstacle in the checker's order (spheres,
 // capsules, z-capsules, world-axis cuboids, other cuboids, z-cuboids).

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
@@ -74,6 +75,7 @@ SolveOptions to_options(const mrpop_solve_options *o) {
         so.neighbour_capacity = o->neighbour_capacity;
         so.neighbour_candidates = o->neighbour_candidates;
         so.rounds_per_step = o->rounds_per_step;
+        so.probe_samples = o->probe_samples;
     }
     return so;
 }
@@ -217,6 +219,7 @@ int mrpop_solve_options_default(mrpop_solve_options *o) {
     o->neighbour_capacity = d.neighbour_capacity;
     o->neighbour_candidates = d.neighbour_candidates;
     o->rounds_per_step = d.rounds_per_step;
+    o->probe_samples = d.probe_samples;
     return MRPOP_OK;
 }
 
@@ -397,4 +400,106 @@ extern "C" void mrpop_diag_philox(const uint32_t counter[4], const uint32_t key[
     out[1] = r.y;
     out[2] = r.z;
     out[3] = r.w;
+}
+
+// Diagnostics (not part of the public ABI): one problem's slot, driven round
+// by round, with its device arrays readable by name.  The planner tests use
+// it to restate roadmap and search rounds on the CPU.
+namespace {
+struct DiagSlot {
+    Context *ctx = nullptr;
+    std::unique_ptr<Engine> engine;
+    Request r;
+    Slot *s = nullptr;
+};
+}  // namespace
+
+extern "C" int mrpop_diag_slot_open(mrpop_ctx *c, const mrpop_problem *p, const mrpop_settings *settings,
+                                    const mrpop_solve_options *options, void **out) {
+    if (!c || !p || !out) return MRPOP_ERR_INVALID;
+    *out = nullptr;
+    Context *ctx = (Context *)c;
+    if (hipSetDevice(ctx->device) != hipSuccess) return MRPOP_ERR_HIP;
+    auto h = std::make_unique<DiagSlot>();
+    h->ctx = ctx;
+    if (int rc = copy_problem(*p, h->r)) return ctx->fail(rc, "invalid problem");
+    mrpop_settings st;
+    if (settings) st = *settings;
+    else mrpop_settings_default(&st);
+    h->engine = std::make_unique<Engine>(ctx);
+    const Request &r = h->r;
+    mrpop_scene sc{(int)(r.spheres.size() / 4), (int)(r.capsules.size() / 8), (int)(r.zcaps.size() / 8),
+                   (int)(r.cuboids.size() / 15), (int)(r.zcubs.size() / 15),
+                   r.spheres.data(), r.capsules.data(), r.zcaps.data(), r.cuboids.data(), r.zcubs.data()};
+    if (int rc = h->engine->add(sc, r.n_arms, r.bases.empty() ? nullptr : r.bases.data(), r.start.data(), r.n_goals,
+                                r.goals.data(), r.seed, st, to_options(options), &h->s))
+        return rc;
+    *out = h.release();
+    return MRPOP_OK;
+}
+
+extern "C" int mrpop_diag_slot_roadmap(void *handle, int rounds) {
+    auto *h = (DiagSlot *)handle;
+    if (!h || rounds < 0) return MRPOP_ERR_INVALID;
+    return h->engine->debug_roadmap(h->s, rounds);
+}
+
+extern "C" int mrpop_diag_slot_search(void *handle, int rounds, int keep_round) {
+    auto *h = (DiagSlot *)handle;
+    if (!h) return MRPOP_ERR_INVALID;
+    return h->engine->debug_search(h->s, rounds, keep_round != 0);
+}
+
+// Sizes: info[0..9] = A, D, B, M, rm_cap, rm_K, tr_cap, ce_cap, G, path_cap.
+extern "C" int mrpop_diag_slot_info(void *handle, int *info) {
+    auto *h = (DiagSlot *)handle;
+    if (!h || !info) return MRPOP_ERR_INVALID;
+    const SlotDev &v = h->s->dv;
+    const int vals[10] = {v.A, v.D, v.B, v.M, v.rm_cap, v.rm_K, v.tr_cap, v.ce_cap, v.G, v.path_cap};
+    for (int i = 0; i < 10; ++i) info[i] = vals[i];
+    return MRPOP_OK;
+}
+
+// Copies the named device array (at most `cap` bytes); returns its size in
+// bytes, or -1 for an unknown name or a failed copy.  "counters" is the
+// slot's block of 64 ints: tr_n[2], ed_count, ce_count, status, best_cost
+// (float), path_cost (float), path_n, path_meta[4], sp_count, ..., then the
+// round counters at 16..19.
+extern "C" long long mrpop_diag_slot_read(void *handle, const char *name, void *out, long long cap) {
+    auto *h = (DiagSlot *)handle;
+    if (!h || !name) return -1;
+    const SlotDev &v = h->s->dv;
+    const size_t A = v.A, D = v.D, B = v.B, rc = v.rm_cap, K = v.rm_K, tc = v.tr_cap, cc = v.ce_cap;
+    struct F {
+        const char *n;
+        const void *p;
+        size_t bytes;
+    };
+    const F fields[] = {
+        {"rm_q", v.rm_q, 4 * A * rc * 8},      {"rm_nbr", v.rm_nbr, 4 * A * rc * K}, {"rm_deg", v.rm_deg, 4 * A * rc},
+        {"rm_n", v.rm_n, 4 * A},               {"tr_q", v.tr_q, 4 * 2 * tc * D},     {"tr_parent", v.tr_parent, 4 * 2 * tc},
+        {"tr_cost", v.tr_cost, 4 * 2 * tc},    {"tr_rid", v.tr_rid, 4 * 2 * tc * A}, {"counters", v.tr_n, 4 * 64},
+        {"ex_tree", v.ex_tree, 4 * B},         {"ex_state", v.ex_state, 4 * B},      {"ex_near", v.ex_near, 4 * B},
+        {"ex_exist", v.ex_exist, 4 * B},       {"ex_index", v.ex_index, 4 * B},      {"ex_dup", v.ex_dup, 4 * B},
+        {"ex_cost", v.ex_cost, 4 * B},         {"ex_q", v.ex_q, 4 * B * D},          {"ex_rid", v.ex_rid, 4 * B * A},
+        {"ed_b", v.ed_b, 4 * B},               {"ed_coll", v.ed_coll, 4 * B},        {"ed_from", v.ed_from, 4 * B * D},
+        {"ed_to", v.ed_to, 4 * B * D},         {"cn_near", v.cn_near, 4 * B},        {"cn_next", v.cn_next, 4 * B},
+        {"cn_d", v.cn_d, 4 * B},               {"cn_fail", v.cn_fail, 4 * B},        {"ce_b", v.ce_b, 4 * cc},
+        {"ce_coll", v.ce_coll, 4 * cc},        {"ce_from", v.ce_from, 4 * cc * D},   {"ce_to", v.ce_to, 4 * cc * D},
+        {"path", v.path, 4 * (size_t)v.path_cap * D},
+    };
+    for (const F &f : fields) {
+        if (std::strcmp(f.n, name) != 0) continue;
+        const size_t n = std::min<size_t>(f.bytes, cap > 0 ? (size_t)cap : 0);
+        if (out && n && hipMemcpy(out, f.p, n, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+        return (long long)f.bytes;
+    }
+    return -1;
+}
+
+extern "C" void mrpop_diag_slot_close(void *handle) {
+    auto *h = (DiagSlot *)handle;
+    if (!h) return;
+    if (h->s) h->engine->release(h->s);
+    delete h;
 }

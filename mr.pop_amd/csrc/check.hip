@@ -365,9 +365,18 @@ template <int KH> struct TileLds {
 
 // Waves of a group: ceil(samples / (64 / A)), A from the group's dimension
 // (one arm in SINGLE mode).
-__device__ __forceinline__ unsigned group_waves(const EdgeGroup &g, int single) {
+// Samples per edge in a launch of the given pass.  Pass 0 checks every
+// sample k = 1..G; a probe pass P > 0 checks min(P, G) of them spread along
+// the edge, the last being k = G; pass -1 checks every sample of the edges
+// a probe pass left free.  The edge verdict, the OR over its samples, is
+// the same either way.
+__device__ __forceinline__ int pass_samples(const EdgeGroup &g, int pass) {
+    return pass > 0 ? min(pass, g.G) : g.G;
+}
+
+__device__ __forceinline__ unsigned group_waves(const EdgeGroup &g, int single, int pass) {
     const int S = single ? 64 : 64 / (g.dim / 7);
-    const unsigned long long samples = (unsigned long long)(g.count ? *g.count : g.n_edges) * g.G;
+    const unsigned long long samples = (unsigned long long)(g.count ? *g.count : g.n_edges) * pass_samples(g, pass);
     return (unsigned)((samples + S - 1) / S);
 }
 
@@ -375,7 +384,7 @@ __device__ __forceinline__ unsigned group_waves(const EdgeGroup &g, int single) 
 // arm) on its own lane.
 template <int A, int MODE, int KH>
 __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *worlds, const EdgeGroup &eb,
-                                           unsigned lw, const FineQueue &fq) {
+                                           unsigned lw, const FineQueue &fq, int pass) {
     constexpr int S = 64 / A;  // samples per wave
     constexpr bool kEnv = MODE != MRPOP_MODE_DRRT;
     constexpr bool kInter = A > 1;
@@ -384,12 +393,19 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
     const int arm = lane - slot * A;
     const int sbase = slot < S ? slot * A : 0;  // spare lanes (64 % A) shadow slot 0 and discard
     const WorldDesc w = worlds[eb.world];
-    const long total = (long)(eb.count ? *eb.count : eb.n_edges) * eb.G;
+    const int per = pass_samples(eb, pass);
+    const long total = (long)(eb.count ? *eb.count : eb.n_edges) * per;
     const long n = (long)lw * S + slot;
-    const bool active = slot < S && n < total;
+    bool active = slot < S && n < total;
     const long nn = active ? n : 0;
-    const int e = (int)(nn / eb.G);
-    const int k = (int)(nn - (long)e * eb.G) + 1;
+    const int e = (int)(nn / per);
+    const int j = (int)(nn - (long)e * per);
+    const int k = per == eb.G ? j + 1 : ((j + 1) * eb.G) / per;  // sample k of k = 1..G
+    if (pass < 0) {
+        // the remaining samples of edges still free after the probe pass
+        if (active && eb.out_edge[e] != 0u) active = false;
+        if (__ballot(active) == 0ull) return;
+    }
     const int base_arm = MODE == MRPOP_MODE_SINGLE ? eb.arm : arm;
     const int dof_off = MODE == MRPOP_MODE_SINGLE ? 0 : 7 * arm;
 
@@ -566,7 +582,7 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
     // cannot change its verdict and are skipped.  A tile holds the samples
     // of at most S consecutive edges; le is this lane's edge within it.
     const bool early_out = !eb.out_mask && !fq.diag;
-    const int le = e - (int)(((long)lw * S) / eb.G);
+    const int le = e - (int)(((long)lw * S) / per);
     volatile unsigned *done = sm.edge_done;
     sm.edge_done[lane] = 0u;
     __syncthreads();
@@ -659,13 +675,13 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
 // groups of different arm counts (multi-arm modes), dispatched per tile.
 template <int MODE, class Tile>
 __device__ __forceinline__ void check_loop(const EdgeGroup *groups, const EdgeGroup &g0, int n_groups,
-                                           const unsigned *wave_start, Tile &&tile) {
+                                           const unsigned *wave_start, int pass, Tile &&tile) {
     const int lane = threadIdx.x;
     unsigned lane_start = 0, total_waves;
     if (!groups) {
-        total_waves = group_waves(g0, MODE == MRPOP_MODE_SINGLE);
+        total_waves = group_waves(g0, MODE == MRPOP_MODE_SINGLE, pass);
     } else if (!wave_start) {
-        const unsigned wcount = lane < n_groups ? group_waves(groups[lane], MODE == MRPOP_MODE_SINGLE) : 0u;
+        const unsigned wcount = lane < n_groups ? group_waves(groups[lane], MODE == MRPOP_MODE_SINGLE, pass) : 0u;
         unsigned incl = wcount;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -703,11 +719,12 @@ __device__ __forceinline__ void check_loop(const EdgeGroup *groups, const EdgeGr
 // but hides more latency; measured faster than two (no spills) and four.
 template <int A, int MODE>
 __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, const EdgeGroup *groups, EdgeGroup g0,
-                                                   int n_groups, const unsigned *wave_start, FineQueue fq) {
+                                                   int n_groups, const unsigned *wave_start, FineQueue fq,
+                                                   int pass) {
     constexpr int KH = A > 1 ? A - 1 : 1;
     __shared__ TileLds<KH> sm;
-    check_loop<MODE>(groups, g0, n_groups, wave_start, [&](const EdgeGroup &eb, unsigned lw) {
-        check_tile<A, MODE, KH>(sm, worlds, eb, lw, fq);
+    check_loop<MODE>(groups, g0, n_groups, wave_start, pass, [&](const EdgeGroup &eb, unsigned lw) {
+        check_tile<A, MODE, KH>(sm, worlds, eb, lw, fq, pass);
     });
 }
 
@@ -716,16 +733,16 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
 // launch per mode instead of one per arm count.
 template <int MODE>
 __global__ __launch_bounds__(64, 3) void check_kernel_mixed(const WorldDesc *worlds, const EdgeGroup *groups,
-                                                         int n_groups, const unsigned *wave_start) {
+                                                         int n_groups, const unsigned *wave_start, int pass) {
     __shared__ TileLds<4> sm;
     const EdgeGroup none{};
     const FineQueue fq{nullptr};
-    check_loop<MODE>(groups, none, n_groups, wave_start, [&](const EdgeGroup &eb, unsigned lw) {
+    check_loop<MODE>(groups, none, n_groups, wave_start, pass, [&](const EdgeGroup &eb, unsigned lw) {
         switch (eb.dim / 7) {
-            case 2: check_tile<2, MODE, 4>(sm, worlds, eb, lw, fq); break;
-            case 3: check_tile<3, MODE, 4>(sm, worlds, eb, lw, fq); break;
-            case 4: check_tile<4, MODE, 4>(sm, worlds, eb, lw, fq); break;
-            default: check_tile<5, MODE, 4>(sm, worlds, eb, lw, fq); break;
+            case 2: check_tile<2, MODE, 4>(sm, worlds, eb, lw, fq, pass); break;
+            case 3: check_tile<3, MODE, 4>(sm, worlds, eb, lw, fq, pass); break;
+            case 4: check_tile<4, MODE, 4>(sm, worlds, eb, lw, fq, pass); break;
+            default: check_tile<5, MODE, 4>(sm, worlds, eb, lw, fq, pass); break;
         }
     });
 }
@@ -792,13 +809,14 @@ __global__ void zero_outputs_kernel(const dev::EdgeGroup *groups, dev::EdgeGroup
 }
 
 // Waves of each group -> exclusive prefix; wave_start[n] = total.
-__global__ void wave_prefix_kernel(const dev::EdgeGroup *groups, int n_groups, int single, unsigned *wave_start) {
+__global__ void wave_prefix_kernel(const dev::EdgeGroup *groups, int n_groups, int single, int pass,
+                                   unsigned *wave_start) {
     __shared__ unsigned sh[256];
     const int tid = threadIdx.x;
     unsigned carry = 0;
     for (int base = 0; base < n_groups; base += 256) {
         const int i = base + tid;
-        const unsigned v = i < n_groups ? dev::group_waves(groups[i], single) : 0u;
+        const unsigned v = i < n_groups ? dev::group_waves(groups[i], single, pass) : 0u;
         sh[tid] = v;
         __syncthreads();
         for (int off = 1; off < 256; off <<= 1) {
@@ -817,7 +835,7 @@ __global__ void wave_prefix_kernel(const dev::EdgeGroup *groups, int n_groups, i
 template <int A, int MODE>
 void enqueue_check(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups, const dev::EdgeGroup &g0,
                    int n_groups, unsigned *d_wave_start, const FineQueue &aux, int check_blocks, bool zero,
-                   hipStream_t stream) {
+                   int pass, hipStream_t stream) {
     if (zero)
         hipLaunchKernelGGL(zero_outputs_kernel,
                            dim3(d_groups ? 4 : std::max(1, std::min(1024, (g0.n_edges + 255) / 256)),
@@ -826,32 +844,33 @@ void enqueue_check(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_group
     // up to 64 groups the check kernel computes the group prefix itself
     if (d_groups && n_groups > 64)
         hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups,
-                           MODE == MRPOP_MODE_SINGLE ? 1 : 0, d_wave_start);
+                           MODE == MRPOP_MODE_SINGLE ? 1 : 0, pass, d_wave_start);
     if (n_groups <= 64) d_wave_start = nullptr;
     hipLaunchKernelGGL((dev::check_kernel<A, MODE>), dim3(check_blocks), dim3(64), 0, stream, d_worlds, d_groups, g0,
-                       n_groups, d_wave_start, aux);
+                       n_groups, d_wave_start, aux, pass);
 }
 
 // Groups of several arm counts (multi-arm modes) in one launch.
 template <int MODE>
 void enqueue_mixed(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups, int n_groups,
-                   unsigned *d_wave_start, int check_blocks, bool zero, hipStream_t stream) {
+                   unsigned *d_wave_start, int check_blocks, bool zero, int pass, hipStream_t stream) {
     const dev::EdgeGroup none{};
     if (zero)
         hipLaunchKernelGGL(zero_outputs_kernel, dim3(4, std::min(n_groups, 1024)), dim3(256), 0, stream, d_groups,
                            none, n_groups);
     if (n_groups > 64)
-        hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups, 0, d_wave_start);
+        hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups, 0, pass,
+                           d_wave_start);
     hipLaunchKernelGGL((dev::check_kernel_mixed<MODE>), dim3(check_blocks), dim3(64), 0, stream, d_worlds, d_groups,
-                       n_groups, n_groups > 64 ? d_wave_start : nullptr);
+                       n_groups, n_groups > 64 ? d_wave_start : nullptr, pass);
 }
 
 int dispatch(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
              const dev::EdgeGroup &g0, int n_groups, unsigned *d_wave_start, const FineQueue &aux, int check_blocks,
-             bool zero, hipStream_t stream) {
+             bool zero, int pass, hipStream_t stream) {
 #define MRPOP_CASE(M, AA)                                                                                         \
     case M * 8 + AA:                                                                                              \
-        enqueue_check<AA, M>(d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, zero, stream);    \
+        enqueue_check<AA, M>(d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, zero, pass, stream); \
         break;
     switch (mode * 8 + A) {
         MRPOP_CASE(MRPOP_MODE_SINGLE, 1)
@@ -865,11 +884,13 @@ int dispatch(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const 
         MRPOP_CASE(MRPOP_MODE_DRRT, 5)
         case MRPOP_MODE_FULL * 8 + 0:  // groups of several arm counts
             if (!d_groups) return c->fail(MRPOP_ERR_INVALID, "a mixed-arm check needs edge groups");
-            enqueue_mixed<MRPOP_MODE_FULL>(d_worlds, d_groups, n_groups, d_wave_start, check_blocks, zero, stream);
+            enqueue_mixed<MRPOP_MODE_FULL>(d_worlds, d_groups, n_groups, d_wave_start, check_blocks, zero, pass,
+                                           stream);
             break;
         case MRPOP_MODE_DRRT * 8 + 0:
             if (!d_groups) return c->fail(MRPOP_ERR_INVALID, "a mixed-arm check needs edge groups");
-            enqueue_mixed<MRPOP_MODE_DRRT>(d_worlds, d_groups, n_groups, d_wave_start, check_blocks, zero, stream);
+            enqueue_mixed<MRPOP_MODE_DRRT>(d_worlds, d_groups, n_groups, d_wave_start, check_blocks, zero, pass,
+                                           stream);
             break;
         default: return c->fail(MRPOP_ERR_INVALID, "unsupported mode %d with %d arms", mode, A);
     }
@@ -897,16 +918,27 @@ int launch_check(Context *c, const World *w, int mode, int arm, int granularity,
     const int S = 64 / A;
     const size_t waves = (total + S - 1) / S;
     const int check_blocks = (int)std::min<size_t>(waves, (size_t)c->num_cus * 32);
-    return dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux, check_blocks, true, stream);
+    const int probe = d_out_mask || c->diag ? 0 : std::min(c->probe, granularity);
+    if (probe > 0 && probe < granularity) {
+        const size_t pw = ((size_t)n_edges * probe + S - 1) / S;
+        if (int rc = dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux,
+                              (int)std::min<size_t>(pw, (size_t)c->num_cus * 32), true, probe, stream))
+            return rc;
+        return dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux, check_blocks, false, -1, stream);
+    }
+    return dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux, check_blocks, true, 0, stream);
 }
 
 int launch_check_groups(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
-                        int n_groups, unsigned *d_wave_start, size_t max_waves, bool zero, hipStream_t stream) {
+                        int n_groups, unsigned *d_wave_start, size_t max_waves, bool zero, int pass,
+                        hipStream_t stream) {
     if (n_groups <= 0) return MRPOP_OK;
+    if (pass < -1) return c->fail(MRPOP_ERR_INVALID, "check pass %d", pass);
     const FineQueue aux{nullptr};
     dev::EdgeGroup g0{};
     const int check_blocks = (int)std::max<size_t>(1, std::min<size_t>(max_waves, (size_t)c->num_cus * 32));
-    return dispatch(c, mode, A, d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, zero, stream);
+    return dispatch(c, mode, A, d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, zero, pass,
+                    stream);
 }
 
 int launch_centres(Context *c, const World *w, int n_cfg, const float *d_cfg, float *d_coarse, float *d_fine,

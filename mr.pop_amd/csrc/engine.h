@@ -29,6 +29,7 @@ struct SolveOptions {
     int neighbour_capacity = 128;  // adjacency entries per roadmap node
     int neighbour_candidates = 64; // radius neighbours examined per new roadmap node
     int rounds_per_step = 8;       // dRRT rounds queued per host step
+    int probe_samples = 0;         // > 0: planner edges are checked at this many samples first (see launch_staged)
 };
 
 struct SeedPath {
@@ -119,6 +120,7 @@ class Engine {
     // Diagnostics: run `rounds` roadmap rounds of slot s (no search) and copy
     // arm a's roadmap out: nodes [n][8], degrees [n], neighbour lists [n][K].
     int debug_roadmap(Slot *s, int rounds);
+    int debug_search(Slot *s, int rounds, bool keep_round);
     int debug_copy_roadmap(Slot *s, int arm, float *q, int *deg, int *nbr, int *n, int cap);
 
   private:
@@ -129,7 +131,9 @@ class Engine {
             size_t first;
             int n;
             size_t max_waves;
+            size_t probe_waves;  // grid bound of the probe pass
         };
+        int probe = 0;  // probe pass samples per edge (0: one pass)
         std::vector<Bucket> buckets;
     };
     struct Lane {

@@ -20,6 +20,14 @@ constexpr size_t kWorldFloats = (size_t)MRPOP_MAX_OBSTACLES * (15 + 8) + 5 * 12 
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Entries of the per-round edge table: a power of two of at least 2 B, so
+// linear probing stays short with B insertions.
+size_t edge_hash_size(int B) {
+    size_t h = 64;
+    while (h < 2 * (size_t)B) h <<= 1;
+    return h;
+}
+
 
 uint64_t ns_since(std::chrono::steady_clock::time_point t) {
     return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t)
@@ -162,6 +170,7 @@ size_t Engine::slot_bytes(int A, const mrpop_settings &st, const SolveOptions &o
     L.add(sizeof(int) * B * 6 + sizeof(float) * B);                         // ex_* ints + cost
     L.add(sizeof(float) * (size_t)B * D);                                   // ex_q
     L.add(sizeof(int) * B * A);                                             // ex_rid
+    L.add(sizeof(unsigned long long) * edge_hash_size(B));                  // ex_hash
     L.add(sizeof(float) * (size_t)B * D * 2 + sizeof(int) * B * 2);         // ed_from/to, ed_b, ed_coll
     L.add(sizeof(int) * B * 3 + sizeof(float) * B);                         // cn_*
     L.add(sizeof(float) * ce_cap * D * 2);                                  // ce_from/to
@@ -188,8 +197,9 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     if (!(st.range > 0)) return c_->fail(MRPOP_ERR_INVALID, "range must be positive");
     if (opt.block_per_robot < 1 || opt.roadmap_capacity < 1 + n_goals + 1 || opt.neighbour_capacity < 1 ||
         opt.neighbour_candidates < 1 || opt.neighbour_candidates > 64 || opt.rounds_per_step < 1 ||
-        opt.rounds_per_step > 1024)
-        return c_->fail(MRPOP_ERR_INVALID, "bad solve options (neighbour_candidates 1..64, rounds_per_step 1..1024)");
+        opt.rounds_per_step > 1024 || opt.probe_samples < 0 || opt.probe_samples > 64)
+        return c_->fail(MRPOP_ERR_INVALID,
+                        "bad solve options (neighbour_candidates 1..64, rounds_per_step 1..1024, probe_samples 0..64)");
     const int counts[5] = {scene.n_spheres, scene.n_capsules, scene.n_z_capsules, scene.n_cuboids, scene.n_z_cuboids};
     long n_obs = 0;
     for (int k = 0; k < 5; ++k) {
@@ -313,9 +323,12 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     v.ex_near = ex + 2 * B;
     v.ex_exist = ex + 3 * B;
     v.ex_index = ex + 4 * B;
+    v.ex_dup = ex + 5 * B;
     v.ex_cost = (float *)(ex + 6 * B);
     v.ex_q = (float *)take(sizeof(float) * (size_t)B * D);
     v.ex_rid = (int *)take(sizeof(int) * B * A);
+    v.ex_hash = (unsigned long long *)take(sizeof(unsigned long long) * edge_hash_size(B));
+    v.ex_hmask = (int)edge_hash_size(B) - 1;
     float *ed = (float *)take(sizeof(float) * (size_t)B * D * 2 + sizeof(int) * B * 2);
     v.ed_from = ed;
     v.ed_to = ed + (size_t)B * D;
@@ -375,6 +388,7 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     MRPOP_HIP(c_, hipMemcpyAsync(v.sg, s->sg.data(), s->sg.size() * sizeof(float), hipMemcpyHostToDevice, c_->stream));
     MRPOP_HIP(c_, hipMemsetAsync(v.rw_key, 0xff, sizeof(unsigned long long) * 2 * (size_t)tr_cap, c_->stream));
     MRPOP_HIP(c_, hipMemsetAsync(ctr, 0, sizeof(int) * 64, c_->stream));
+    MRPOP_HIP(c_, hipMemsetAsync(v.ex_hash, 0, sizeof(unsigned long long) * edge_hash_size(B), c_->stream));
     MRPOP_HIP(c_, hipMemsetAsync(v.rl_slot, 0xff, sizeof(int) * (size_t)A * rm_cap, c_->stream));
     MRPOP_HIP(c_, hipMemsetAsync(v.rl_bits, 0, sizeof(unsigned) * (size_t)A * rl_cap * rl_words, c_->stream));
     {
@@ -521,7 +535,11 @@ int Engine::stage_groups(Lane &L, int mode, const std::vector<const dev::EdgeGro
     int A_all = mode == MRPOP_MODE_SINGLE ? 1 : (list.empty() ? 0 : list[0]->A);
     for (Slot *s : list)
         if (mode != MRPOP_MODE_SINGLE && s->A != A_all) A_all = 0;
-    Staged::Bucket bk{A_all, L.group_cursor, 0, 0};
+    Staged::Bucket bk{A_all, L.group_cursor, 0, 0, 0};
+    // probe pass: the largest probe any of the problems asks for; it is used
+    // only if some group has more samples per edge than the probe
+    int probe = 0, max_G = 0;
+    for (Slot *s : list) probe = std::max(probe, s->opt.probe_samples);
     for (size_t i = 0; i < list.size(); ++i) {
         const int n = mode == MRPOP_MODE_SINGLE ? list[i]->A : 1;
         const int S = mode == MRPOP_MODE_SINGLE ? 64 : 64 / list[i]->A;
@@ -530,8 +548,11 @@ int Engine::stage_groups(Lane &L, int mode, const std::vector<const dev::EdgeGro
             const dev::EdgeGroup &g = groups[i][k];
             L.h_groups[L.group_cursor++] = g;
             bk.max_waves += ((size_t)g.n_edges * g.G + S - 1) / S;
+            if (probe > 0) bk.probe_waves += ((size_t)g.n_edges * std::min(probe, g.G) + S - 1) / S;
+            max_G = std::max(max_G, g.G);
         }
     }
+    out.probe = max_G > probe ? probe : 0;
     bk.n = (int)(L.group_cursor - bk.first);
     if (bk.n) out.buckets.push_back(bk);
     if (L.group_cursor > begin)
@@ -542,11 +563,21 @@ int Engine::stage_groups(Lane &L, int mode, const std::vector<const dev::EdgeGro
 
 // One batched check launch per bucket.  Every producer of planner
 // edges clears their verdict words, so the launches skip the zeroing pass.
+// With a probe, a first launch checks `probe` samples per edge and a second
+// checks every sample of the edges the first left free: most planner edges
+// that collide are found by the probe, and their remaining samples, which
+// cannot change the verdict, are not checked.
 int Engine::launch_staged(Lane &L, const Staged &st) {
-    for (const Staged::Bucket &bk : st.buckets)
+    for (const Staged::Bucket &bk : st.buckets) {
+        if (st.probe > 0) {
+            if (int rc = launch_check_groups(c_, st.mode, bk.A, d_worlds_, L.d_groups + bk.first, bk.n, L.d_wave,
+                                             bk.probe_waves, false, st.probe, L.stream))
+                return rc;
+        }
         if (int rc = launch_check_groups(c_, st.mode, bk.A, d_worlds_, L.d_groups + bk.first, bk.n, L.d_wave,
-                                         bk.max_waves, false, L.stream))
+                                         bk.max_waves, false, st.probe > 0 ? -1 : 0, L.stream))
             return rc;
+    }
     return MRPOP_OK;
 }
 
@@ -797,6 +828,42 @@ int Engine::debug_roadmap(Slot *s, int rounds) {
         MRPOP_HIP(c_, hipEventSynchronize(L.done));
         if (int rc = process_lane(L)) return rc;
     }
+    return MRPOP_OK;
+}
+
+// Exactly `rounds` dRRT rounds of one slot in one launch (fewer if the pass
+// is solved), without the host state machine: the slot's device state is
+// left as the last round's kernels wrote it.  keep_round: the rounds
+// continue the previous launch's round count (as the later rounds of one
+// launch do) instead of starting a launch at round 0.
+int Engine::debug_search(Slot *s, int rounds, bool keep_round) {
+    if (rounds < 1) return c_->fail(MRPOP_ERR_INVALID, "rounds must be positive");
+    if (int rc = ensure_lanes()) return rc;
+    if (int rc = drain()) return rc;
+    Lane &L = lanes_[0];
+    const int rps = s->opt.rounds_per_step;
+    s->opt.rounds_per_step = rounds;
+    if (keep_round) {
+        int ctr[4];
+        MRPOP_HIP(c_, hipMemcpy(ctr, s->dv.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+        ctr[kCtrLimit] = ctr[kCtrRound] + rounds;
+        MRPOP_HIP(c_, hipMemcpy(s->dv.ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice));
+        s->dv.round_reset = 0;
+    } else {
+        s->dv.round_reset = 1;
+        s->dv.round_limit = rounds;
+    }
+    s->phase = Phase::Search;
+    L.rm.clear();
+    L.sp.clear();
+    L.dr.assign(1, s);
+    const int rc = launch_lane(L);
+    s->opt.rounds_per_step = rps;
+    L.dr.clear();
+    if (rc) return rc;
+    MRPOP_HIP(c_, hipEventSynchronize(L.done));
+    L.busy = false;
+    s->lane = -1;
     return MRPOP_OK;
 }
 

@@ -35,6 +35,7 @@ struct Context {
     DevBuf from, to, out_edge, out_mask, cfg, out_a, out_b;
     std::vector<DevBuf *> owned;
     void *diag = nullptr;  // when set, edge checks record per-sample phase results (diagnostics)
+    int probe = 0;         // > 0: edge-only checks run a probe pass of this many samples first (diagnostics)
 
     int fail(int code, const char *fmt, ...) {
         char buf[512];
@@ -78,11 +79,16 @@ int launch_check(Context *c, const World *w, int mode, int arm, int granularity,
 // (an upper bound of the waves the groups can need); zero = false when the
 // producers of the edges already cleared the verdict words.  A == 0 in a
 // multi-arm mode: groups of 2..5 arms (from each group's dim) in one launch.
+// pass: 0 checks every sample; P > 0 checks min(P, G) samples per edge
+// spread along it (k = floor(j G / P), j = 1..P); -1 then checks every
+// sample of the edges still free.  Probe + completion give the verdicts of
+// pass 0 and are for groups without a per-sample mask.
 namespace dev {
 struct EdgeGroup;
 }
 int launch_check_groups(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
-                        int n_groups, unsigned *d_wave_start, size_t max_waves, bool zero, hipStream_t stream);
+                        int n_groups, unsigned *d_wave_start, size_t max_waves, bool zero, int pass,
+                        hipStream_t stream);
 
 // Sphere centres kernel (check.hip).
 int launch_centres(Context *c, const World *w, int n_cfg, const float *d_cfg, float *d_coarse, float *d_fine,

@@ -94,6 +94,9 @@ struct SlotDev {
     float *ex_q;              // [B][D]
     int *ex_rid;              // [B][A]
     int *ex_index;            // [B] node index after insertion (-1 none)
+    int *ex_dup;              // [B] expansion whose identical edge is checked for b (-1 none)
+    unsigned long long *ex_hash;  // [ex_hmask+1] edges of the round: (round tag << 32) | b
+    int ex_hmask;
     unsigned long long *rw_key;  // [tr_cap*2] rewire arbitration (cost bits << 32 | b)
     // extension edges (DRRT check)
     int *ed_count;

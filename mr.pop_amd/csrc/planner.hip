@@ -447,6 +447,39 @@ __device__ __forceinline__ float edge_angle(const float *v, const float *q, cons
     return acosf(c);
 }
 
+// Enters expansion b's edge (tree t, tree node near, roadmap nodes rid[]) in
+// the round's edge table.  Returns -1 if b is the first expansion of the
+// round with this edge, else that expansion.  An entry belongs to the
+// round whose tag it carries; entries of earlier rounds count as free.  b's
+// own fields are published before its entry: ex_tree and ex_near were
+// stored by this thread, ex_rid by lanes of this wave before the caller's
+// barrier.
+__device__ int claim_edge(const SlotDev &s, unsigned tag, int b, int t, int near, const int *rid) {
+    unsigned h = (unsigned)near * 0x9E3779B1u ^ (unsigned)t * 0x85EBCA77u;
+    for (int a = 0; a < s.A; ++a) h = (h ^ (unsigned)rid[a]) * 0xC2B2AE3Du + 0x27D4EB2Fu;
+    h ^= h >> 15;
+    const unsigned long long mine = ((unsigned long long)tag << 32) | (unsigned)b;
+    __threadfence();  // release b's fields before its entry
+    for (int p = 0; p <= s.ex_hmask; ++p) {
+        unsigned long long *e = s.ex_hash + ((h + (unsigned)p) & (unsigned)s.ex_hmask);
+        unsigned long long cur = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while ((unsigned)(cur >> 32) != tag) {  // free: claim it
+            const unsigned long long prev = atomicCAS(e, cur, mine);
+            if (prev == cur) return -1;
+            cur = prev;
+        }
+        const int w = (int)(unsigned)cur;
+        __threadfence();  // acquire w's fields
+        bool same = __hip_atomic_load(s.ex_tree + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == t &&
+                    __hip_atomic_load(s.ex_near + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == near;
+        for (int a = 0; a < s.A && same; ++a)
+            same = __hip_atomic_load(s.ex_rid + (size_t)w * s.A + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                   rid[a];
+        if (same) return w;
+    }
+    return -1;  // not reached: the table has more entries than a round has expansions
+}
+
 // One dRRT expansion per wave (pop_planner.cu:350-622 up to the edge check):
 // pick the tree, sample (uniform or PHS), nearest tree node, per-arm roadmap
 // neighbour of minimum angle, duplicate test, and the arm-arm edge to check.
@@ -474,6 +507,7 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
     if (lane == 0) {
         s.ex_state[b] = 0;
         s.ex_index[b] = -1;
+        s.ex_dup[b] = -1;
     }
     const int o = 1 - t;
     const float best = *s.best_cost;
@@ -617,6 +651,17 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
     }
     if (lane < A) s.ex_rid[b * A + lane] = nrid[lane];
     if (lane < D) s.ex_q[(size_t)b * D + lane] = nq[lane];
+    __syncthreads();
+    // The edge is fixed by (t, near, nrid[]): an expansion of this round with
+    // the same triple has the same edge and so the same verdict, and the
+    // edge is checked once.
+    int dup = -1;
+    if (lane == 0) dup = claim_edge(s, rng_round + 1u, b, t, near, nrid);
+    dup = __shfl(dup, 0);
+    if (dup >= 0) {
+        if (lane == 0) s.ex_dup[b] = dup;
+        return;
+    }
     int slot = 0;
     if (lane == 0) slot = atomicAdd(s.ed_count, 1);
     slot = __shfl(slot, 0);
@@ -642,6 +687,14 @@ __global__ __launch_bounds__(256) void k_dr_commit(SlotDev *slots, const int *id
     for (int e = tid; e < ne; e += 256) {
         const int b = s.ed_b[e];
         if (s.ed_coll[e]) s.ex_state[b] = 0;
+    }
+    __syncthreads();
+    // an expansion whose edge was checked for another takes that verdict
+    // (ex_dup only names expansions that own an edge, so no state read here
+    // is written in this loop)
+    for (int b = tid; b < s.B; b += 256) {
+        const int w = s.ex_dup[b];
+        if (w >= 0 && s.ex_state[w] == 0) s.ex_state[b] = 0;
     }
     __syncthreads();
     // rewire arbitration

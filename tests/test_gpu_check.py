@@ -18,6 +18,21 @@ pytestmark = pytest.mark.gpu
 FK_TOL = 1e-5  # metres
 
 
+def _check_probe(w, mode, G, frm, to, probe, arm=0):
+    """Edge verdicts of the two-pass schedule the planner uses: `probe`
+    samples per edge, then every sample of the edges still free."""
+    import ctypes
+    from mrpop._lib import lib
+    f = lib().mrpop_diag_check_probe
+    f.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_int] * 4 + [ctypes.c_void_p] * 2 + [ctypes.c_int, ctypes.c_void_p]
+    frm = np.ascontiguousarray(frm, np.float32)
+    to = np.ascontiguousarray(to, np.float32)
+    out = np.zeros(len(frm), np.uint8)
+    w.ctx.check(f(w.ctx.handle, w._h, mode, arm, G, len(frm), frm.ctypes.data, to.ctypes.data, probe,
+                  out.ctypes.data))
+    return out.astype(bool)
+
+
 def _scene_from_npz(z):
     from mrpop.scenes import Scene
     return Scene(z["spheres"], z["capsules"], z["z_capsules"], z["cuboids"], z["z_cuboids"])
@@ -50,6 +65,8 @@ def test_golden_edges(ctx, name):
     # without the mask output the kernel may skip samples of known-colliding edges
     edge2 = w.check_edges(mode, G, z["frm"], z["to"], arm=arm)
     assert np.array_equal(edge2, edge)
+    for probe in (1, 4, 7):
+        assert np.array_equal(_check_probe(w, mode, G, z["frm"], z["to"], probe, arm=arm), edge), probe
 
 
 def _random_edges(rng, starts, n, dim, lo, hi):
@@ -77,6 +94,8 @@ def test_live_oracle_parity(ctx, kind, G):
         edge, mask = w.check_edges(mode, G, frm, to, with_mask=True)
         r = O.check_edges(p.scene, mode, p.bases, G, frm, to)
         _assert_masks(mask, edge, r["mask_lo"], r["mask_hi"], f"{kind} mode {mode} G {G}")
+        for probe in (1, 4, 5):
+            assert np.array_equal(_check_probe(w, mode, G, frm, to, probe), edge), (mode, probe)
     for arm in range(p.n_arms):
         f1, t1 = frm[:, 7 * arm:7 * arm + 7], to[:, 7 * arm:7 * arm + 7]
         edge, mask = w.check_edges(O.MODE_SINGLE, G, f1, t1, arm=arm, with_mask=True)

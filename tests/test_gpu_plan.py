@@ -157,6 +157,23 @@ def test_same_alone_and_in_the_batch(ctx, alone):
         assert s.info["rounds"] == alone[p.name].info["rounds"]
 
 
+def test_probe_pass_changes_no_result(ctx, alone):
+    """Checking planner edges at a few samples first, and in full only while
+    free, gives the verdicts of one full pass: every result is unchanged."""
+    sols = _lib.solve_batch(ctx, PROBLEMS, [_settings()] * len(PROBLEMS),
+                            [_options(probe_samples=4)] * len(PROBLEMS))
+    for p, s in zip(PROBLEMS, sols):
+        assert s.info["status"] == _lib.OK
+        assert np.array_equal(s.path, alone[p.name].path), p.name
+        assert [c for c, _ in s.seed_paths] == [c for c, _ in alone[p.name].seed_paths]
+        assert s.info["rounds"] == alone[p.name].info["rounds"]
+    mixed = [scenes.make_problem(k, 3, _lib.gpu_checker(ctx)) for k in ("plain2", "table3", "bin4", "shelf5")]
+    one = _lib.solve_batch(ctx, mixed, [_settings()] * 4, [_options()] * 4)
+    two = _lib.solve_batch(ctx, mixed, [_settings()] * 4, [_options(probe_samples=4)] * 4)
+    for a, b in zip(one, two):
+        assert np.array_equal(a.path, b.path) and a.info["rounds"] == b.info["rounds"]
+
+
 @pytest.mark.parametrize("kind,seed", [("plain2", 3), ("table3", 5)])
 def test_generated_two_and_three_arm(ctx, kind, seed):
     p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))

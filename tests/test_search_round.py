@@ -1,0 +1,327 @@
+"""dRRT search rounds (P3-P8 of the hot path): the device's rounds against
+the CPU restatement in oracle/search.py.
+
+A problem's slot is driven round by round through the library's diagnostic
+entry points; before each round its trees, roadmaps and counters are read,
+after it every expansion record, candidate edge, connection chunk and the
+trees again.  The restatement recomputes the round from the state before it:
+tree choice and balance, the Philox uniform sample (bit-exact), the nearest
+node (float32 distances, lowest index among equals), the roadmap step per
+arm (the device's pick must be admissible and of least angle within
+ANGLE_TOL; acosf is not reproduced), the existing-node test, the new cost,
+one checked edge per distinct (tree, near, roadmap nodes), insertion in
+expansion order, the connection to the other tree in floor(d / range) + 1
+chunks, the connection node and the recorded path.  The edge verdicts the
+device used are checked against the FP64 oracle (near-contact edges may go
+either way) and then drive the restated commit.
+
+CPU tests pin fma32, the exact float32 fused multiply-add the restatement
+relies on."""
+import ctypes
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from mrpop import _lib, geometry, scenes
+from oracle import oracle as O
+from oracle import search as SR
+
+
+def _round_half_even_f32(x: Fraction) -> np.float32:
+    """Correct rounding of an exact rational to float32 (normal range)."""
+    if x == 0:
+        return np.float32(0)
+    sign = -1 if x < 0 else 1
+    x = abs(x)
+    e = x.numerator.bit_length() - x.denominator.bit_length()
+    while Fraction(2) ** e > x:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= x:
+        e += 1
+    m = x / Fraction(2) ** (e - 23)  # in [2^23, 2^24)
+    fl = m.numerator // m.denominator
+    rem = m - fl
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and fl % 2 == 1):
+        fl += 1
+    return np.float32(sign * float(Fraction(fl) * Fraction(2) ** (e - 23)))
+
+
+def test_fma32_is_one_rounding():
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal(4000).astype(np.float32)
+    b = rng.standard_normal(4000).astype(np.float32)
+    c = (rng.standard_normal(4000) * 10.0 ** rng.integers(-8, 3, 4000)).astype(np.float32)
+    got = SR.fma32(a, b, c)
+    want = [_round_half_even_f32(Fraction(float(x)) * Fraction(float(y)) + Fraction(float(z)))
+            for x, y, z in zip(a, b, c)]
+    assert np.array_equal(got, np.array(want, np.float32))
+    # x * x = 1 + 2^-11 + 2^-24 lies halfway between two floats; an addend
+    # below float64 resolution vanishes from the float64 sum, which then sits
+    # on the midpoint, but it decides the one correct rounding
+    x = np.float32(1 + 2.0 ** -12)
+    for c in (2.0 ** -60, -(2.0 ** -60), 0.0):
+        c = np.float32(c)
+        want = _round_half_even_f32(Fraction(float(x)) ** 2 + Fraction(float(c)))
+        assert SR.fma32(x, x, c) == want, c
+    assert SR.fma32(x, x, np.float32(2.0 ** -60)) > SR.fma32(x, x, np.float32(-(2.0 ** -60)))
+
+
+def test_sqdist_accumulates_with_fma():
+    rng = np.random.default_rng(1)
+    p = rng.standard_normal((50, 14)).astype(np.float32)
+    q = rng.standard_normal(14).astype(np.float32)
+    d = SR.sqdist(p, q)
+    for i in range(50):
+        s = np.float32(0)
+        for k in range(14):
+            x = np.float32(p[i, k] - q[k])
+            s = _round_half_even_f32(Fraction(float(x)) ** 2 + Fraction(float(s)))
+        assert d[i] == s
+
+
+# ---------------------------------------------------------------- GPU ----
+
+class _DiagSlot:
+    def __init__(self, ctx, problem, st, op):
+        self.lib = _lib.lib()
+        L = self.lib
+        L.mrpop_diag_slot_open.argtypes = [ctypes.c_void_p] * 5
+        L.mrpop_diag_slot_roadmap.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.mrpop_diag_slot_search.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.mrpop_diag_slot_info.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.mrpop_diag_slot_read.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_longlong]
+        L.mrpop_diag_slot_read.restype = ctypes.c_longlong
+        L.mrpop_diag_slot_close.argtypes = [ctypes.c_void_p]
+        L.mrpop_diag_slot_close.restype = None
+        self.ctx = ctx
+        self.ref = _lib._ProblemRef(problem)
+        self.st, self.op = st, op
+        self.h = ctypes.c_void_p()
+        ctx.check(L.mrpop_diag_slot_open(ctx.handle, ctypes.byref(self.ref.c), ctypes.byref(st), ctypes.byref(op),
+                                         ctypes.byref(self.h)))
+        info = np.zeros(10, np.int32)
+        ctx.check(L.mrpop_diag_slot_info(self.h, info.ctypes.data))
+        self.A, self.D, self.B, self.M, self.rm_cap, self.K, self.tr_cap, self.ce_cap, self.G, self.path_cap = \
+            (int(v) for v in info)
+
+    def close(self):
+        self.lib.mrpop_diag_slot_close(self.h)
+
+    def roadmap(self, rounds):
+        self.ctx.check(self.lib.mrpop_diag_slot_roadmap(self.h, rounds))
+
+    def search(self, rounds, keep_round):
+        self.ctx.check(self.lib.mrpop_diag_slot_search(self.h, rounds, int(keep_round)))
+
+    def read(self, name, dtype, shape):
+        n = int(np.prod(shape))
+        out = np.zeros(n, dtype)
+        got = self.lib.mrpop_diag_slot_read(self.h, name.encode(), out.ctypes.data, out.nbytes)
+        assert got >= out.nbytes, (name, got, out.nbytes)
+        return out.reshape(shape)
+
+    def state(self):
+        A, D, B, tc, rc, K = self.A, self.D, self.B, self.tr_cap, self.rm_cap, self.K
+        ctr = self.read("counters", np.int32, (64,))
+        return dict(
+            counters=ctr, tr_n=ctr[0:2].copy(), status=int(ctr[4]),
+            best=ctr[5:6].view(np.float32)[0], path_cost=ctr[6:7].view(np.float32)[0], path_n=int(ctr[7]),
+            round=int(ctr[16]), rng_round=int(ctr[17]),
+            tr_q=self.read("tr_q", np.float32, (2, tc, D)), tr_parent=self.read("tr_parent", np.int32, (2, tc)),
+            tr_cost=self.read("tr_cost", np.float32, (2, tc)), tr_rid=self.read("tr_rid", np.int32, (2, tc, A)),
+            ex_tree=self.read("ex_tree", np.int32, (B,)), ex_state=self.read("ex_state", np.int32, (B,)),
+            ex_near=self.read("ex_near", np.int32, (B,)), ex_dup=self.read("ex_dup", np.int32, (B,)),
+            ex_index=self.read("ex_index", np.int32, (B,)), ex_cost=self.read("ex_cost", np.float32, (B,)),
+            ex_q=self.read("ex_q", np.float32, (B, D)), ex_rid=self.read("ex_rid", np.int32, (B, A)),
+            ed_b=self.read("ed_b", np.int32, (B,)), ed_coll=self.read("ed_coll", np.uint32, (B,)),
+            ed_from=self.read("ed_from", np.float32, (B, D)), ed_to=self.read("ed_to", np.float32, (B, D)),
+            cn_near=self.read("cn_near", np.int32, (B,)), cn_next=self.read("cn_next", np.int32, (B,)),
+            cn_d=self.read("cn_d", np.float32, (B,)), cn_fail=self.read("cn_fail", np.uint32, (B,)),
+            ce_count=int(ctr[3]), ce_b=self.read("ce_b", np.int32, (self.ce_cap,)),
+            ce_coll=self.read("ce_coll", np.uint32, (self.ce_cap,)),
+            ce_from=self.read("ce_from", np.float32, (self.ce_cap, D)),
+            ce_to=self.read("ce_to", np.float32, (self.ce_cap, D)),
+            path=self.read("path", np.float32, (self.path_cap, D)),
+            rm_q=self.read("rm_q", np.float32, (A, rc, 8)), rm_nbr=self.read("rm_nbr", np.int32, (A, rc, K)),
+            rm_deg=self.read("rm_deg", np.int32, (A, rc)))
+
+
+def _verdicts_agree(problem, mode, G, frm, to, device):
+    """Device edge verdicts against the FP64 oracle; near-contact edges are
+    exempt.  Returns the number of exempt edges."""
+    if not len(frm):
+        return 0
+    r = O.check_edges(problem.scene, mode, problem.bases, G, np.array(frm), np.array(to))
+    device = np.asarray(device, bool)
+    robust = r["lo"] == r["hi"]
+    assert np.array_equal(device[robust], r["lo"][robust]), f"mode {mode}: edge verdicts differ from the oracle"
+    return int((~robust).sum())
+
+
+def _check_round(p, ds, before, after, settings):
+    """Restate one round from `before` and compare with `after`."""
+    A, D, B = ds.A, ds.D, ds.B
+    s = SR.Slot(A=A, D=D, B=B, K=ds.K, seed=p.seed, range=settings.range, balance=settings.balance,
+                tree_ratio=settings.tree_ratio, round=before["round"], rng_round=before["rng_round"],
+                scale=geometry.table("kScale"), low=geometry.table("kLow"), tr_n=before["tr_n"],
+                tr_q=before["tr_q"], tr_cost=before["tr_cost"], tr_rid=before["tr_rid"], ex_tree=before["ex_tree"],
+                rm_q=before["rm_q"], rm_nbr=before["rm_nbr"], rm_deg=before["rm_deg"])
+    assert before["status"] == 0 and before["best"] == SR.FLT_MAX  # first pass, still searching
+    cand = {}
+    for b in range(B):
+        t, near, dnear, cands = SR.expand(s, b)
+        assert after["ex_tree"][b] == t, f"expansion {b}: tree"
+        if cands is None:
+            continue
+        # least angle; among angles within ANGLE_TOL of it the device's pick
+        nrid = []
+        for a in range(A):
+            r0 = cands[a][0][1]
+            tie = len(cands[a]) > 1 and cands[a][1][0] <= cands[a][0][0] + SR.ANGLE_TOL
+            dev = int(after["ex_rid"][b][a])
+            nrid.append(dev if tie and SR.pick_ok(cands[a], dev) else r0)
+        nq, cost, exists = SR.compose(s, t, near, dnear, nrid)
+        if exists:
+            continue
+        cand[b] = dict(t=t, near=near, nrid=tuple(nrid), nq=nq, cost=cost)
+    # candidates: one edge per distinct (tree, near, roadmap nodes)
+    groups = {}
+    for b, c in cand.items():
+        groups.setdefault((c["t"], c["near"], c["nrid"]), []).append(b)
+    n_edges = len(groups)
+    emitted = after["ed_b"][:n_edges].tolist()
+    assert sorted(set(emitted)) == sorted(emitted) and len(emitted) == n_edges
+    for b in range(B):
+        if b not in cand:
+            assert after["ex_dup"][b] == -1 and after["ex_state"][b] == 0 and b not in emitted, \
+                f"expansion {b}: the device made a candidate of it"
+            assert after["ex_index"][b] == -1
+    verdict = {}
+    frm, to, dev_v = [], [], []
+    for key, members in groups.items():
+        owners = [b for b in members if b in emitted]
+        assert len(owners) == 1, f"group {key}: {len(owners)} edges"
+        w = owners[0]
+        for b in members:
+            assert after["ex_dup"][b] == (-1 if b == w else w), f"expansion {b}: duplicate link"
+            c = cand[b]
+            assert after["ex_near"][b] == c["near"] and tuple(after["ex_rid"][b]) == c["nrid"]
+            assert np.array_equal(after["ex_q"][b], c["nq"]) and after["ex_cost"][b] == c["cost"], f"expansion {b}"
+        e = emitted.index(w)
+        c = cand[w]
+        assert np.array_equal(after["ed_from"][e], before["tr_q"][c["t"], c["near"]])
+        assert np.array_equal(after["ed_to"][e], c["nq"])
+        frm.append(after["ed_from"][e]); to.append(after["ed_to"][e]); dev_v.append(after["ed_coll"][e] != 0)
+        for b in members:
+            verdict[b] = after["ed_coll"][e] != 0
+    exempt = _verdicts_agree(p, O.MODE_DRRT, ds.G, frm, to, dev_v)
+    # commit: free candidates inserted in expansion order per tree
+    tr_q, tr_parent = before["tr_q"].copy(), before["tr_parent"].copy()
+    tr_cost, tr_rid, tr_n = before["tr_cost"].copy(), before["tr_rid"].copy(), before["tr_n"].copy()
+    index = {}
+    for b in sorted(cand):
+        if verdict[b]:
+            assert after["ex_state"][b] == 0 and after["ex_index"][b] == -1, f"expansion {b}: colliding edge committed"
+            continue
+        c = cand[b]
+        t, idx = c["t"], int(tr_n[c["t"]])
+        assert idx < ds.tr_cap
+        tr_n[t] += 1
+        tr_q[t, idx], tr_parent[t, idx], tr_cost[t, idx], tr_rid[t, idx] = c["nq"], c["near"], c["cost"], c["nrid"]
+        index[b] = idx
+        assert after["ex_state"][b] == 1 and after["ex_index"][b] == idx, f"expansion {b}: insert"
+    # connection of each inserted node to the other tree (trees after the commit)
+    s2 = SR.Slot(**dict(s.__dict__, tr_n=tr_n, tr_q=tr_q))
+    plans = {}
+    cfrm, cto, cdev = [], [], []
+    ce_b = after["ce_b"][:after["ce_count"]].tolist()
+    for b in range(B):
+        if b not in index:
+            assert after["cn_near"][b] == -1
+            continue
+        t = cand[b]["t"]
+        plan = SR.connect_plan(s2, t, tr_q[t, index[b]], ds.ce_cap // B)
+        if plan is None:
+            assert after["cn_near"][b] == -1
+            continue
+        assert after["cn_near"][b] == plan["near"] and after["cn_next"][b] == plan["next"]
+        assert after["cn_d"][b] == plan["d"]
+        base = ce_b.index(b)
+        assert ce_b[base:base + plan["next"]] == [b] * plan["next"]
+        assert np.array_equal(after["ce_from"][base:base + plan["next"]], plan["frm"])
+        assert np.array_equal(after["ce_to"][base:base + plan["next"]], plan["to"])
+        chunks = after["ce_coll"][base:base + plan["next"]] != 0
+        cfrm += list(plan["frm"]); cto += list(plan["to"]); cdev += list(chunks)
+        plan["failed"] = bool(chunks.any())
+        assert bool(after["cn_fail"][b]) == plan["failed"]
+        plans[b] = plan
+    exempt += _verdicts_agree(p, O.MODE_FULL, ds.G, cfrm, cto, cdev)
+    assert len(ce_b) == sum(pl["next"] for pl in plans.values())
+    # connection nodes, in expansion order per tree; the lowest records the path
+    n_commit = tr_n.copy()
+    winner = None
+    for b in sorted(plans):
+        pl = plans[b]
+        if pl["failed"]:
+            continue
+        t, o = cand[b]["t"], 1 - cand[b]["t"]
+        idx = int(tr_n[t])
+        tr_n[t] += 1
+        tr_q[t, idx] = pl["end"]
+        tr_rid[t, idx] = tr_rid[o, pl["near"]]
+        tr_parent[t, idx] = index[b]
+        tr_cost[t, idx] = np.float32(tr_cost[t, index[b]] + np.float32(pl["d"] / np.float32(pl["next"])))
+        if winner is None:
+            winner = (b, t, idx, pl["near"])
+    for tr in (0, 1):
+        n = int(tr_n[tr])
+        assert after["tr_n"][tr] == n, f"tree {tr}: {after['tr_n'][tr]} nodes, restated {n}"
+        assert np.array_equal(after["tr_q"][tr, :n], tr_q[tr, :n]), f"tree {tr}: configurations"
+        assert np.array_equal(after["tr_parent"][tr, :n], tr_parent[tr, :n]), f"tree {tr}: parents"
+        assert np.array_equal(after["tr_cost"][tr, :n], tr_cost[tr, :n]), f"tree {tr}: costs"
+        assert np.array_equal(after["tr_rid"][tr, :n], tr_rid[tr, :n]), f"tree {tr}: roadmap ids"
+    assert after["rng_round"] == before["rng_round"] + 1
+    if winner is None:
+        assert after["status"] == 0
+    else:
+        b, t, wnode, onode = winner
+        assert after["status"] == b + 1, "solving expansion"
+        path, cost = SR.record_path(tr_q, tr_parent, t, wnode, onode, ds.path_cap)
+        assert after["path_n"] == len(path)
+        assert np.array_equal(after["path"][:len(path)], path), "recorded path"
+        assert after["path_cost"] == cost and after["best"] == cost
+    return dict(candidates=len(cand), edges=n_edges, inserted=len(index), connections=len(plans),
+                solved=winner is not None, exempt=exempt, n_commit=n_commit)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,seed", [("plain2", 4), ("table3", 2), ("bin4", 1), ("shelf5", 3)])
+def test_search_rounds_match_the_restatement(ctx, kind, seed):
+    p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
+    st = _lib.settings(num_new_configs=48, granularity=8, range=2.0)
+    op = _lib.solve_options_default(block_per_robot=1, neighbour_candidates=8, neighbour_capacity=16,
+                                    roadmap_capacity=4096, tree_capacity=4096)
+    ds = _DiagSlot(ctx, p, st, op)
+    try:
+        # roadmap rounds until the start and first goal node of every arm
+        # have a roadmap neighbour (else no expansion can move every arm)
+        for _ in range(12):
+            ds.roadmap(1)
+            deg = ds.read("rm_deg", np.int32, (ds.A, ds.rm_cap))
+            if (deg[:, :2] > 0).all():
+                break
+        stats = []
+        keep = False  # the first round of a launch (round 0), then rounds continuing it
+        for r in range(6):
+            before = ds.state()
+            if before["status"] != 0:
+                break
+            print(f"[search] {kind} seed {seed} round {before['round']} trees {before['tr_n'].tolist()}", flush=True)
+            ds.search(1, keep)
+            keep = True
+            stats.append(_check_round(p, ds, before, ds.state(), st))
+        print(f"[search] {kind}: {stats}", flush=True)
+        assert sum(x["edges"] for x in stats) > 0 and sum(x["inserted"] for x in stats) > 0
+    finally:
+        ds.close()
