@@ -114,17 +114,22 @@ __device__ __forceinline__ void load_base(const WorldDesc &w, int arm, Frame &T)
     for (int i = 0; i < 12; ++i) T.m[i] = b[i];
 }
 
+// (to - from) / G.  For G a power of two the quotient is exact, so it
+// equals the product with 1/G (one instruction instead of a correctly
+// rounded division); G is uniform over a wave tile.
+__device__ __forceinline__ float step_of(float t, float f, int G) {
+    if ((G & (G - 1)) == 0) return (t - f) * __builtin_amdgcn_ldexpf(1.0f, -__builtin_ctz(G));
+    return (t - f) / (float)G;
+}
+
 // Interpolated joint angles of one arm: q = from + k * ((to - from) / G)
 // (pop_planner.cu:553-560).
 __device__ __forceinline__ void interp7(const EdgeBatch &eb, int e, int k, int off, float q[7]) {
     const float *f = eb.from + (size_t)e * eb.dim + off;
     const float *t = eb.to + (size_t)e * eb.dim + off;
-    const float kf = (float)k, gf = (float)eb.G;
+    const float kf = (float)k;
 #pragma unroll
-    for (int j = 0; j < 7; ++j) {
-        float d = (t[j] - f[j]) / gf;
-        q[j] = __builtin_fmaf(kf, d, f[j]);
-    }
+    for (int j = 0; j < 7; ++j) q[j] = __builtin_fmaf(kf, step_of(t[j], f[j], eb.G), f[j]);
 }
 
 // Joint transforms for a rolled (runtime-depth) FK walk.
@@ -148,10 +153,10 @@ __device__ __forceinline__ Frame frame_rt(const WorldDesc &w, const EdgeBatch &e
     load_base(w, base_arm, T);
     const float *f = eb.from + (size_t)e * eb.dim + dof_off;
     const float *t = eb.to + (size_t)e * eb.dim + dof_off;
-    const float kf = (float)k, gf = (float)eb.G;
+    const float kf = (float)k;
 #pragma unroll 1
     for (int J = 1; J <= jt; ++J) {
-        const float qj = __builtin_fmaf(kf, (t[J - 1] - f[J - 1]) / gf, f[J - 1]);
+        const float qj = __builtin_fmaf(kf, step_of(t[J - 1], f[J - 1], eb.G), f[J - 1]);
         float sq, cq;
         joint_sincos(qj, sq, cq);
         const float *F = kJT.tf[J];
@@ -353,9 +358,22 @@ constexpr float env_radius_max() {
 }
 constexpr float kEnvRadiusMax = env_radius_max();
 
+// Largest block radius (coarse radius + inflation) over the coarse spheres,
+// and the slack added to arm balls: 0.1 mm, orders of magnitude above the
+// float32 rounding of distances of a few metres.
+constexpr float max_block_radius() {
+    float r = 0.f;
+    for (int s = 0; s < kCoarseSpheres; ++s)
+        r = kCoarseSphere[s][3] + kBlockInflate[s] > r ? kCoarseSphere[s][3] + kBlockInflate[s] : r;
+    return r;
+}
+constexpr float kMaxBlockRadius = max_block_radius();
+constexpr float kBallMargin = 1e-4f;
+
 // Shared memory of one wave tile; KH = arm-arm half-pairs per lane.
 template <int KH> struct TileLds {
     float cs[kCoarseSpheres][3][64];
+    float ball[4][64];                   // per lane: its arm's bounding ball (centre, radius)
     unsigned inter_flags[64];
     unsigned result[64];
     unsigned long long cand_lo[KH][64];  // candidate blocks, rows 0..4: bit t*11 + k
@@ -439,6 +457,34 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
         sm.cs[s][2][lane] = cz[s];
     }
     sm.inter_flags[lane] = 0u;
+    if constexpr (kInter) {
+        // A ball around this arm's coarse block balls: centre = mean of the
+        // centres, radius = largest centre distance + largest block radius +
+        // a margin far above float rounding.  A sphere whose block ball
+        // misses it can neither hit nor be a candidate of any of the arm's
+        // spheres in the arm-arm stage.
+        float mx = cx[0], my = cy[0], mz = cz[0];
+#pragma unroll
+        for (int s = 1; s < kCoarseSpheres; ++s) {
+            mx += cx[s];
+            my += cy[s];
+            mz += cz[s];
+        }
+        constexpr float inv = 1.0f / kCoarseSpheres;
+        mx *= inv;
+        my *= inv;
+        mz *= inv;
+        float r2 = 0.f;
+#pragma unroll
+        for (int s = 0; s < kCoarseSpheres; ++s) {
+            const float dx = cx[s] - mx, dy = cy[s] - my, dz = cz[s] - mz;
+            r2 = fmaxf(r2, __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
+        }
+        sm.ball[0][lane] = mx;
+        sm.ball[1][lane] = my;
+        sm.ball[2][lane] = mz;
+        sm.ball[3][lane] = sqrtf(r2) + kMaxBlockRadius + kBallMargin;
+    }
 
     // coarse environment: spheres 1..10 -> env bit of their joint (panda_four.cuh:241-261)
     unsigned hitmask = 0;
@@ -533,6 +579,28 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
         for (int m = 0; m < A - 1; ++m) {
             int a, b, i0;
             half_pair<A>(arm + m * A, a, b, i0);
+            // rows whose block ball meets arm b's ball; rows no lane of the
+            // wave needs are skipped (their tests cannot hit)
+            unsigned need = 0;
+            {
+                const float ux = sm.ball[0][sbase + b], uy = sm.ball[1][sbase + b], uz = sm.ball[2][sbase + b];
+                const float ur = sm.ball[3][sbase + b];
+#pragma unroll
+                for (int t = 0; t < 6; ++t) {
+                    const int i = min(i0 + t, kCoarseSpheres - 1);
+                    const float dx = sm.cs[i][0][sbase + a] - ux, dy = sm.cs[i][1][sbase + a] - uy,
+                                dz = sm.cs[i][2][sbase + a] - uz;
+                    const float lim = ur + kTab.block_r[i];
+                    if (__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) < lim * lim) need |= 1u << t;
+                }
+            }
+            if (!active) need = 0;
+            const unsigned long long rows_any = __ballot(need != 0u);
+            if (!rows_any) {
+                sm.cand_lo[m][lane] = 0ull;
+                sm.cand_hi[m][lane] = 0u;
+                continue;
+            }
             float bx[kCoarseSpheres], by[kCoarseSpheres], bz[kCoarseSpheres];
 #pragma unroll
             for (int j = 0; j < kCoarseSpheres; ++j) {
@@ -545,6 +613,7 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
             unsigned chi = 0;
 #pragma unroll 1
             for (int t = 0; t < 6; ++t) {
+                if (!__ballot((need >> t) & 1u)) continue;
                 const int i = min(i0 + t, kCoarseSpheres - 1);  // rows 6..10 are 5: repeat row 10
                 const float ax = sm.cs[i][0][sbase + a], ay = sm.cs[i][1][sbase + a], az = sm.cs[i][2][sbase + a];
                 const float rad = kTab.coarse_r[i], brad = kTab.block_r[i];
