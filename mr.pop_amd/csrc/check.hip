@@ -261,9 +261,9 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
 //     with bounding-ball culling of the obstacles.
 // Run once for gates on joints 0..4 and once for joint 5 plus the
 // environment, so that at most 17 tested centres are live at a time.
-template <int MODE, int G0, int G1, bool WITH_ENV>
+template <int MODE, int G0, int G1, bool WITH_ENV, int NT>
 __device__ __forceinline__ bool fine_walk(const WorldDesc &w, const Frame &base, const float q[7], unsigned selfbits,
-                                          unsigned envbits, const float (*cs)[3][64], int lane) {
+                                          unsigned envbits, const float (*cs)[3][NT], int lane) {
     bool hit = false;
     float px[kFineSelfRanges], py[kFineSelfRanges], pz[kFineSelfRanges];
     Frame T = base;
@@ -370,19 +370,19 @@ constexpr float max_block_radius() {
 constexpr float kMaxBlockRadius = max_block_radius();
 constexpr float kBallMargin = 1e-4f;
 
-// Shared memory of one wave tile; KH = arm-arm half-pairs per lane.
-template <int KH> struct TileLds {
-    float cs[kCoarseSpheres][3][64];
-    float ball[4][64];                   // per lane: its arm's bounding ball (centre, radius)
-    unsigned inter_flags[64];
-    unsigned result[64];
-    unsigned long long cand_lo[KH][64];  // candidate blocks, rows 0..4: bit t*11 + k
-    unsigned cand_hi[KH][64];            // row 5: bit k
-    unsigned edge_done[64];              // per edge of the tile: a sample collides
+// Shared memory of one tile of NT items (sample, arm); KH = arm-arm
+// half-pairs per item.
+template <int KH, int NT> struct TileLds {
+    float cs[kCoarseSpheres][3][NT];
+    float ball[4][NT];                   // per item: its arm's bounding ball (centre, radius)
+    unsigned inter_flags[NT];
+    unsigned result[NT];
+    unsigned long long cand_lo[KH][NT];  // candidate blocks, rows 0..4: bit t*11 + k
+    unsigned cand_hi[KH][NT];            // row 5: bit k
+    unsigned edge_done[NT];              // per edge of the tile: a sample collides
+    unsigned fine[NT];                   // per item: its fine tests found a collision
 };
 
-// Waves of a group: ceil(samples / (64 / A)), A from the group's dimension
-// (one arm in SINGLE mode).
 // Samples per edge in a launch of the given pass.  Pass 0 checks every
 // sample k = 1..G; a probe pass P > 0 checks min(P, G) of them spread along
 // the edge, the last being k = G; pass -1 checks every sample of the edges
@@ -392,29 +392,34 @@ __device__ __forceinline__ int pass_samples(const EdgeGroup &g, int pass) {
     return pass > 0 ? min(pass, g.G) : g.G;
 }
 
-__device__ __forceinline__ unsigned group_waves(const EdgeGroup &g, int single, int pass) {
-    const int S = single ? 64 : 64 / (g.dim / 7);
+// Tiles of a group: a tile is W waves, 64 W / A samples of all A arms of
+// the group (one arm in SINGLE mode).
+__device__ __forceinline__ unsigned group_waves(const EdgeGroup &g, int single, int pass, int W) {
+    const int S = single ? 64 * W : (64 * W) / (g.dim / 7);
     const unsigned long long samples = (unsigned long long)(g.count ? *g.count : g.n_edges) * pass_samples(g, pass);
     return (unsigned)((samples + S - 1) / S);
 }
 
-// One wave tile: samples lw*S .. lw*S + S - 1 of group eb, every (sample,
-// arm) on its own lane.
-template <int A, int MODE, int KH>
-__device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *worlds, const EdgeGroup &eb,
+// One tile of NT / 64 waves: samples lw*S .. lw*S + S - 1 of group eb,
+// S = NT / A, thread arm * S + slot holding arm `arm` of sample `slot`.  The
+// waves are arm-major, so the environment and arm-pair culls of a wave see
+// one arm of 64 nearby samples.
+template <int A, int MODE, int KH, int NT>
+__device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc *worlds, const EdgeGroup &eb,
                                            unsigned lw, const FineQueue &fq, int pass) {
-    constexpr int S = 64 / A;  // samples per wave
+    constexpr int S = NT / A;  // samples per tile
     constexpr bool kEnv = MODE != MRPOP_MODE_DRRT;
     constexpr bool kInter = A > 1;
-    const int lane = threadIdx.x;
-    const int slot = lane / A;
-    const int arm = lane - slot * A;
-    const int sbase = slot < S ? slot * A : 0;  // spare lanes (64 % A) shadow slot 0 and discard
+    const int lane = threadIdx.x;  // the thread's own item
+    const bool item = lane < S * A;
+    // spare threads (NT % A) shadow arm 0 of sample 0 and discard
+    const int arm = item ? lane / S : 0;
+    const int slot = item ? lane - arm * S : 0;
     const WorldDesc w = worlds[eb.world];
     const int per = pass_samples(eb, pass);
     const long total = (long)(eb.count ? *eb.count : eb.n_edges) * per;
     const long n = (long)lw * S + slot;
-    bool active = slot < S && n < total;
+    bool active = item && n < total;
     const long nn = active ? n : 0;
     const int e = (int)(nn / per);
     const int j = (int)(nn - (long)e * per);
@@ -422,7 +427,7 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
     if (pass < 0) {
         // the remaining samples of edges still free after the probe pass
         if (active && eb.out_edge[e] != 0u) active = false;
-        if (__ballot(active) == 0ull) return;
+        if (!__syncthreads_or(active)) return;
     }
     const int base_arm = MODE == MRPOP_MODE_SINGLE ? eb.arm : arm;
     const int dof_off = MODE == MRPOP_MODE_SINGLE ? 0 : 7 * arm;
@@ -569,6 +574,13 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
             if (sphere_sphere(cx[s1], cy[s1], cz[s1], r1, cx[j], cy[j], cz[j], rj) < 0.f) selfbits |= bit;
         });
     });
+    // Barriers of a tile: (1) coarse centres, balls and intra-arm self bits
+    // published, (2) arm-arm flags complete, (3) fine results published.
+    // Every LDS entry is written by its own item before (1), or between (1)
+    // and (3) with all readers before (3); so a wave that has passed (3) may
+    // start the next tile while others finish this one.
+    sm.result[lane] = selfbits;  // intra-arm part of the self bits
+    sm.edge_done[lane] = 0u;
     __syncthreads();
 
     // coarse arm-arm pairs (panda_four.cuh:178-214), A-1 half-pairs per lane.
@@ -583,13 +595,13 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
             // wave needs are skipped (their tests cannot hit)
             unsigned need = 0;
             {
-                const float ux = sm.ball[0][sbase + b], uy = sm.ball[1][sbase + b], uz = sm.ball[2][sbase + b];
-                const float ur = sm.ball[3][sbase + b];
+                const float ux = sm.ball[0][b * S + slot], uy = sm.ball[1][b * S + slot], uz = sm.ball[2][b * S + slot];
+                const float ur = sm.ball[3][b * S + slot];
 #pragma unroll
                 for (int t = 0; t < 6; ++t) {
                     const int i = min(i0 + t, kCoarseSpheres - 1);
-                    const float dx = sm.cs[i][0][sbase + a] - ux, dy = sm.cs[i][1][sbase + a] - uy,
-                                dz = sm.cs[i][2][sbase + a] - uz;
+                    const float dx = sm.cs[i][0][a * S + slot] - ux, dy = sm.cs[i][1][a * S + slot] - uy,
+                                dz = sm.cs[i][2][a * S + slot] - uz;
                     const float lim = ur + kTab.block_r[i];
                     if (__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) < lim * lim) need |= 1u << t;
                 }
@@ -604,9 +616,9 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
             float bx[kCoarseSpheres], by[kCoarseSpheres], bz[kCoarseSpheres];
 #pragma unroll
             for (int j = 0; j < kCoarseSpheres; ++j) {
-                bx[j] = sm.cs[j][0][sbase + b];
-                by[j] = sm.cs[j][1][sbase + b];
-                bz[j] = sm.cs[j][2][sbase + b];
+                bx[j] = sm.cs[j][0][b * S + slot];
+                by[j] = sm.cs[j][1][b * S + slot];
+                bz[j] = sm.cs[j][2][b * S + slot];
             }
             unsigned mask = 0;
             unsigned long long clo = 0;
@@ -615,7 +627,7 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
             for (int t = 0; t < 6; ++t) {
                 if (!__ballot((need >> t) & 1u)) continue;
                 const int i = min(i0 + t, kCoarseSpheres - 1);  // rows 6..10 are 5: repeat row 10
-                const float ax = sm.cs[i][0][sbase + a], ay = sm.cs[i][1][sbase + a], az = sm.cs[i][2][sbase + a];
+                const float ax = sm.cs[i][0][a * S + slot], ay = sm.cs[i][1][a * S + slot], az = sm.cs[i][2][a * S + slot];
                 const float rad = kTab.coarse_r[i], brad = kTab.block_r[i];
                 bool hit = false;
                 unsigned cand = 0;
@@ -637,14 +649,11 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
             }
             sm.cand_lo[m][lane] = active ? clo : 0ull;
             sm.cand_hi[m][lane] = active ? chi : 0u;
-            if (mask && slot < S) atomicOr(&sm.inter_flags[sbase + a], mask);
+            if (mask && item) atomicOr(&sm.inter_flags[a * S + slot], mask);
         }
     }
     __syncthreads();
     if constexpr (kInter) selfbits |= sm.inter_flags[lane];
-    // publish the final self bits of every arm for the arm-arm fine stage
-    sm.result[lane] = selfbits;
-    __syncthreads();
 
     // Edge-only verdicts (no per-sample mask, no diagnostics): once one
     // sample of an edge collides, the fine tests left for that edge's lanes
@@ -653,8 +662,6 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
     const bool early_out = !eb.out_mask && !fq.diag;
     const int le = e - (int)(((long)lw * S) / per);
     volatile unsigned *done = sm.edge_done;
-    sm.edge_done[lane] = 0u;
-    __syncthreads();
 
     // ---- B. fine intra-arm pairs of this lane's arm
     bool finehit = false;
@@ -663,8 +670,9 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
         interp7(eb, e, k, dof_off, q);
         Frame base;
         load_base(w, base_arm, base);
-        if (selfbits & 0x1fu) finehit = fine_walk<MODE, 0, 4, false>(w, base, q, selfbits, 0u, sm.cs, lane);
-        if ((selfbits & 0x20u) || envbits) finehit |= fine_walk<MODE, 5, 5, kEnv>(w, base, q, selfbits, envbits, sm.cs, lane);
+        if (selfbits & 0x1fu) finehit = fine_walk<MODE, 0, 4, false, NT>(w, base, q, selfbits, 0u, sm.cs, lane);
+        if ((selfbits & 0x20u) || envbits)
+            finehit |= fine_walk<MODE, 5, 5, kEnv, NT>(w, base, q, selfbits, envbits, sm.cs, lane);
     }
     if (active && finehit) done[le] = 1u;
 
@@ -675,7 +683,7 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
             if (early_out && active && done[le]) break;
             int a, b, i0;
             half_pair<A>(arm + m * A, a, b, i0);
-            const unsigned gate = sm.result[sbase + a];  // arm a's self bits
+            const unsigned gate = sm.result[a * S + slot] | sm.inter_flags[a * S + slot];  // arm a's self bits
             unsigned long long clo = sm.cand_lo[m][lane];
             unsigned chi = sm.cand_hi[m][lane];
             // keep rows whose block's joint is self-flagged on arm a
@@ -721,20 +729,18 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
     // ---- D. per-sample verdict: OR over the sample's lanes
     if (fq.diag && active)
         fq.diag[(size_t)n * 8 + arm] = envbits | (selfbits << 8) | (finehit ? 1u << 16 : 0u) | (1u << 17);
-    __syncthreads();
-    sm.result[lane] = finehit ? 1u : 0u;
+    sm.fine[lane] = finehit ? 1u : 0u;
     __syncthreads();
     if (active && arm == 0) {
         bool coll = false;
 #pragma unroll
-        for (int a = 0; a < A; ++a) coll |= sm.result[slot * A + a] != 0u;
+        for (int a = 0; a < A; ++a) coll |= sm.fine[a * S + slot] != 0u;
         if (coll) {
             atomicOr(&eb.out_edge[e], 1u);
             if (eb.out_mask) atomicOr(&eb.out_mask[e], 1ull << (k - 1));
         }
         if (fq.diag) fq.diag[(size_t)n * 8 + 7] = (coll ? 1u : 0u) | (1u << 17);
     }
-    __syncthreads();  // LDS is reused by the next wave-tile
 }
 
 // Persistent loop over the wave tiles of n_groups edge groups (or of the
@@ -742,15 +748,18 @@ __device__ __forceinline__ void check_tile(TileLds<KH> &sm, const WorldDesc *wor
 // the groups' first waves itself, one group per lane (wave_start is then
 // null); more groups come with a prefix from wave_prefix_kernel.  A == 0:
 // groups of different arm counts (multi-arm modes), dispatched per tile.
-template <int MODE, class Tile>
+// Persistent loop of a block of W waves over the tiles of the groups.  Every
+// wave computes the same group lookup.
+template <int MODE, int W, class Tile>
 __device__ __forceinline__ void check_loop(const EdgeGroup *groups, const EdgeGroup &g0, int n_groups,
                                            const unsigned *wave_start, int pass, Tile &&tile) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     unsigned lane_start = 0, total_waves;
     if (!groups) {
-        total_waves = group_waves(g0, MODE == MRPOP_MODE_SINGLE, pass);
+        total_waves = group_waves(g0, MODE == MRPOP_MODE_SINGLE, pass, W);
     } else if (!wave_start) {
-        const unsigned wcount = lane < n_groups ? group_waves(groups[lane], MODE == MRPOP_MODE_SINGLE, pass) : 0u;
+        const unsigned wcount =
+            lane < n_groups ? group_waves(groups[lane], MODE == MRPOP_MODE_SINGLE, pass, W) : 0u;
         unsigned incl = wcount;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -784,16 +793,16 @@ __device__ __forceinline__ void check_loop(const EdgeGroup *groups, const EdgeGr
     }
 }
 
-// Three waves per SIMD: the register budget (168 VGPRs) costs a few spills
-// but hides more latency; measured faster than two (no spills) and four.
+// A waves per block, one per arm; three waves per SIMD: the register budget
+// (168 VGPRs) costs a few spills but hides more latency.
 template <int A, int MODE>
-__global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, const EdgeGroup *groups, EdgeGroup g0,
-                                                   int n_groups, const unsigned *wave_start, FineQueue fq,
-                                                   int pass) {
+__global__ __launch_bounds__(64 * A, 3) void check_kernel(const WorldDesc *worlds, const EdgeGroup *groups,
+                                                       EdgeGroup g0, int n_groups, const unsigned *wave_start,
+                                                       FineQueue fq, int pass) {
     constexpr int KH = A > 1 ? A - 1 : 1;
-    __shared__ TileLds<KH> sm;
-    check_loop<MODE>(groups, g0, n_groups, wave_start, pass, [&](const EdgeGroup &eb, unsigned lw) {
-        check_tile<A, MODE, KH>(sm, worlds, eb, lw, fq, pass);
+    __shared__ TileLds<KH, 64 * A> sm;
+    check_loop<MODE, A>(groups, g0, n_groups, wave_start, pass, [&](const EdgeGroup &eb, unsigned lw) {
+        check_tile<A, MODE, KH, 64 * A>(sm, worlds, eb, lw, fq, pass);
     });
 }
 
@@ -801,17 +810,19 @@ __global__ __launch_bounds__(64, 3) void check_kernel(const WorldDesc *worlds, c
 // its group's arm count, so a planner round checks all its problems in one
 // launch per mode instead of one per arm count.
 template <int MODE>
-__global__ __launch_bounds__(64, 3) void check_kernel_mixed(const WorldDesc *worlds, const EdgeGroup *groups,
-                                                         int n_groups, const unsigned *wave_start, int pass) {
-    __shared__ TileLds<4> sm;
+__global__ __launch_bounds__(64 * kMixedWaves, 3) void check_kernel_mixed(const WorldDesc *worlds,
+                                                                       const EdgeGroup *groups, int n_groups,
+                                                                       const unsigned *wave_start, int pass) {
+    constexpr int NT = 64 * kMixedWaves;
+    __shared__ TileLds<4, NT> sm;
     const EdgeGroup none{};
     const FineQueue fq{nullptr};
-    check_loop<MODE>(groups, none, n_groups, wave_start, pass, [&](const EdgeGroup &eb, unsigned lw) {
+    check_loop<MODE, kMixedWaves>(groups, none, n_groups, wave_start, pass, [&](const EdgeGroup &eb, unsigned lw) {
         switch (eb.dim / 7) {
-            case 2: check_tile<2, MODE, 4>(sm, worlds, eb, lw, fq, pass); break;
-            case 3: check_tile<3, MODE, 4>(sm, worlds, eb, lw, fq, pass); break;
-            case 4: check_tile<4, MODE, 4>(sm, worlds, eb, lw, fq, pass); break;
-            default: check_tile<5, MODE, 4>(sm, worlds, eb, lw, fq, pass); break;
+            case 2: check_tile<2, MODE, 4, NT>(sm, worlds, eb, lw, fq, pass); break;
+            case 3: check_tile<3, MODE, 4, NT>(sm, worlds, eb, lw, fq, pass); break;
+            case 4: check_tile<4, MODE, 4, NT>(sm, worlds, eb, lw, fq, pass); break;
+            default: check_tile<5, MODE, 4, NT>(sm, worlds, eb, lw, fq, pass); break;
         }
     });
 }
@@ -878,14 +889,14 @@ __global__ void zero_outputs_kernel(const dev::EdgeGroup *groups, dev::EdgeGroup
 }
 
 // Waves of each group -> exclusive prefix; wave_start[n] = total.
-__global__ void wave_prefix_kernel(const dev::EdgeGroup *groups, int n_groups, int single, int pass,
+__global__ void wave_prefix_kernel(const dev::EdgeGroup *groups, int n_groups, int single, int pass, int W,
                                    unsigned *wave_start) {
     __shared__ unsigned sh[256];
     const int tid = threadIdx.x;
     unsigned carry = 0;
     for (int base = 0; base < n_groups; base += 256) {
         const int i = base + tid;
-        const unsigned v = i < n_groups ? dev::group_waves(groups[i], single, pass) : 0u;
+        const unsigned v = i < n_groups ? dev::group_waves(groups[i], single, pass, W) : 0u;
         sh[tid] = v;
         __syncthreads();
         for (int off = 1; off < 256; off <<= 1) {
@@ -913,10 +924,10 @@ void enqueue_check(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_group
     // up to 64 groups the check kernel computes the group prefix itself
     if (d_groups && n_groups > 64)
         hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups,
-                           MODE == MRPOP_MODE_SINGLE ? 1 : 0, pass, d_wave_start);
+                           MODE == MRPOP_MODE_SINGLE ? 1 : 0, pass, A, d_wave_start);
     if (n_groups <= 64) d_wave_start = nullptr;
-    hipLaunchKernelGGL((dev::check_kernel<A, MODE>), dim3(check_blocks), dim3(64), 0, stream, d_worlds, d_groups, g0,
-                       n_groups, d_wave_start, aux, pass);
+    hipLaunchKernelGGL((dev::check_kernel<A, MODE>), dim3(check_blocks), dim3(64 * A), 0, stream, d_worlds, d_groups,
+                       g0, n_groups, d_wave_start, aux, pass);
 }
 
 // Groups of several arm counts (multi-arm modes) in one launch.
@@ -929,9 +940,9 @@ void enqueue_mixed(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_group
                            none, n_groups);
     if (n_groups > 64)
         hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups, 0, pass,
-                           d_wave_start);
-    hipLaunchKernelGGL((dev::check_kernel_mixed<MODE>), dim3(check_blocks), dim3(64), 0, stream, d_worlds, d_groups,
-                       n_groups, n_groups > 64 ? d_wave_start : nullptr, pass);
+                           dev::kMixedWaves, d_wave_start);
+    hipLaunchKernelGGL((dev::check_kernel_mixed<MODE>), dim3(check_blocks), dim3(64 * dev::kMixedWaves), 0, stream,
+                       d_worlds, d_groups, n_groups, n_groups > 64 ? d_wave_start : nullptr, pass);
 }
 
 int dispatch(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
@@ -984,8 +995,8 @@ int launch_check(Context *c, const World *w, int mode, int arm, int granularity,
     const FineQueue aux{(unsigned *)c->diag};
     dev::EdgeGroup g0{d_from, d_to, mode == MRPOP_MODE_SINGLE ? 7 : 7 * A, granularity, n_edges, arm, nullptr,
                       d_out_edge, (unsigned long long *)d_out_mask, 0, 0};
-    const int S = 64 / A;
-    const size_t waves = (total + S - 1) / S;
+    const int S = dev::tile_samples(A, false);
+    const size_t waves = (total + S - 1) / S;  // tiles
     const int check_blocks = (int)std::min<size_t>(waves, (size_t)c->num_cus * 32);
     const int probe = d_out_mask || c->diag ? 0 : std::min(c->probe, granularity);
     if (probe > 0 && probe < granularity) {

@@ -22,5 +22,11 @@ struct EdgeGroup {
     int pad_;
 };
 
+// A check tile is W waves holding all arms of 64 W / A samples: W = A for
+// the kernel of one arm count (64 samples per tile, one wave per arm),
+// kMixedWaves for the kernel that mixes arm counts.  SINGLE mode: one wave.
+constexpr int kMixedWaves = 5;
+constexpr int tile_samples(int A, bool mixed) { return mixed ? 64 * kMixedWaves / A : 64; }
+
 }  // namespace dev
 }  // namespace mrpop

@@ -542,7 +542,7 @@ int Engine::stage_groups(Lane &L, int mode, const std::vector<const dev::EdgeGro
     for (Slot *s : list) probe = std::max(probe, s->opt.probe_samples);
     for (size_t i = 0; i < list.size(); ++i) {
         const int n = mode == MRPOP_MODE_SINGLE ? list[i]->A : 1;
-        const int S = mode == MRPOP_MODE_SINGLE ? 64 : 64 / list[i]->A;
+        const int S = mode == MRPOP_MODE_SINGLE ? 64 : dev::tile_samples(list[i]->A, A_all == 0);
         for (int k = 0; k < n && k < per_slot; ++k) {
             if (L.group_cursor >= group_cap_) return c_->fail(MRPOP_ERR_BUDGET, "edge-group staging full");
             const dev::EdgeGroup &g = groups[i][k];
