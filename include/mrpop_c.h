@@ -191,6 +191,8 @@ typedef struct mrpop_solve_options {
     int rounds_per_step;       /* dRRT rounds queued between host checks (default 8) */
     int probe_samples;         /* > 0: planner edges are first checked at this many samples spread along
                                   them, then in full only if still free; same verdicts (default 0) */
+    int graph_rounds;          /* 1: replay each step's dRRT rounds from a captured HIP graph (default 0;
+                                  same results, no measured gain; not for runs under a kernel tracer) */
 } mrpop_solve_options;
 
 int mrpop_solve_options_default(mrpop_solve_options *o);

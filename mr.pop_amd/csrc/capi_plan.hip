@@ -76,6 +76,7 @@ SolveOptions to_options(const mrpop_solve_options *o) {
         so.neighbour_candidates = o->neighbour_candidates;
         so.rounds_per_step = o->rounds_per_step;
         so.probe_samples = o->probe_samples;
+        so.graph_rounds = o->graph_rounds;
     }
     return so;
 }
@@ -220,6 +221,7 @@ int mrpop_solve_options_default(mrpop_solve_options *o) {
     o->neighbour_candidates = d.neighbour_candidates;
     o->rounds_per_step = d.rounds_per_step;
     o->probe_samples = d.probe_samples;
+    o->graph_rounds = d.graph_rounds;
     return MRPOP_OK;
 }
 

@@ -945,6 +945,12 @@ void enqueue_mixed(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_group
                        d_worlds, d_groups, n_groups, n_groups > 64 ? d_wave_start : nullptr, pass);
 }
 
+// Blocks of W waves that stay resident at three waves per SIMD: the
+// persistent grid needs no more (idle blocks only cost dispatch).
+size_t resident_blocks(const Context *c, int W) {
+    return (size_t)c->num_cus * std::max(1, 12 / W);
+}
+
 int dispatch(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
              const dev::EdgeGroup &g0, int n_groups, unsigned *d_wave_start, const FineQueue &aux, int check_blocks,
              bool zero, int pass, hipStream_t stream) {
@@ -997,12 +1003,12 @@ int launch_check(Context *c, const World *w, int mode, int arm, int granularity,
                       d_out_edge, (unsigned long long *)d_out_mask, 0, 0};
     const int S = dev::tile_samples(A, false);
     const size_t waves = (total + S - 1) / S;  // tiles
-    const int check_blocks = (int)std::min<size_t>(waves, (size_t)c->num_cus * 32);
+    const int check_blocks = (int)std::min<size_t>(waves, resident_blocks(c, A));
     const int probe = d_out_mask || c->diag ? 0 : std::min(c->probe, granularity);
     if (probe > 0 && probe < granularity) {
         const size_t pw = ((size_t)n_edges * probe + S - 1) / S;
         if (int rc = dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux,
-                              (int)std::min<size_t>(pw, (size_t)c->num_cus * 32), true, probe, stream))
+                              (int)std::min<size_t>(pw, resident_blocks(c, A)), true, probe, stream))
             return rc;
         return dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux, check_blocks, false, -1, stream);
     }
@@ -1016,7 +1022,8 @@ int launch_check_groups(Context *c, int mode, int A, const dev::WorldDesc *d_wor
     if (pass < -1) return c->fail(MRPOP_ERR_INVALID, "check pass %d", pass);
     const FineQueue aux{nullptr};
     dev::EdgeGroup g0{};
-    const int check_blocks = (int)std::max<size_t>(1, std::min<size_t>(max_waves, (size_t)c->num_cus * 32));
+    const int W = mode == MRPOP_MODE_SINGLE ? 1 : (A == 0 ? dev::kMixedWaves : A);
+    const int check_blocks = (int)std::max<size_t>(1, std::min<size_t>(max_waves, resident_blocks(c, W)));
     return dispatch(c, mode, A, d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, zero, pass,
                     stream);
 }

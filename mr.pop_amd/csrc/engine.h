@@ -30,6 +30,7 @@ struct SolveOptions {
     int neighbour_candidates = 64; // radius neighbours examined per new roadmap node
     int rounds_per_step = 8;       // dRRT rounds queued per host step
     int probe_samples = 0;         // > 0: planner edges are checked at this many samples first (see launch_staged)
+    int graph_rounds = 0;          // 1: replay a step's dRRT rounds from a captured HIP graph
 };
 
 struct SeedPath {
@@ -149,7 +150,17 @@ class Engine {
         size_t group_cursor = 0;
         unsigned *d_wave = nullptr;
         SlotReport *d_report = nullptr, *h_report = nullptr;
+        // dRRT round sequences captured on this lane, keyed by every launch
+        // argument they bake in (buffers, counts, grid bounds)
+        struct Graph {
+            std::vector<uint64_t> key;
+            hipGraphExec_t exec = nullptr;
+            uint64_t used = 0;
+        };
+        std::vector<Graph> graphs;
+        uint64_t graph_clock = 0;
     };
+    void drop_graphs(Lane &L);
     int ensure_capacity(int n);
     int ensure_lanes();
     int finish_pass(Slot *s, const SlotReport &r);

@@ -16,6 +16,7 @@ constexpr int kPathCap = 2048;             // waypoints (MAX_PATH_SIZE floats in
 constexpr int kMaxConnectChunks = 64;      // chunks of one connection attempt
 constexpr float kFltMax = 3.402823466e38f;
 // obstacle records (at most 15 floats), their cull boxes (8), 5 base poses
+constexpr size_t kGraphsPerLane = 8;  // captured round sequences kept per lane
 constexpr size_t kWorldFloats = (size_t)MRPOP_MAX_OBSTACLES * (15 + 8) + 5 * 12 + 64;
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -60,6 +61,7 @@ Engine::~Engine() {
             if (p) (void)hipFree(p);
         for (void *p : {(void *)L.h_slots, (void *)L.h_ids, (void *)L.h_groups, (void *)L.h_report})
             if (p) (void)hipHostFree(p);
+        drop_graphs(L);
         if (L.done) (void)hipEventDestroy(L.done);
         if (L.stream) (void)hipStreamDestroy(L.stream);
     }
@@ -68,6 +70,11 @@ Engine::~Engine() {
         if (p) (void)hipFree(p);
     for (void *p : {(void *)h_worlds_, (void *)h_aux_slot_})
         if (p) (void)hipHostFree(p);
+}
+
+void Engine::drop_graphs(Lane &L) {
+    for (Lane::Graph &g : L.graphs) (void)hipGraphExecDestroy(g.exec);
+    L.graphs.clear();
 }
 
 bool Engine::idle() const {
@@ -119,6 +126,7 @@ int Engine::ensure_capacity(int n) {
     // per slot at most 2 * 5 roadmap groups, or dr + cn + sp; twice over for safety
     group_cap_ = (size_t)cap * 32;
     for (Lane &L : lanes_) {
+        drop_graphs(L);  // they bake in the buffers reallocated here
         if (int rc = realloc_pair((void **)&L.d_slots, (void **)&L.h_slots, sizeof(SlotDev) * cap, 0)) return rc;
         if (int rc = realloc_pair((void **)&L.d_report, (void **)&L.h_report, sizeof(SlotReport) * cap, 0)) return rc;
         if (int rc = realloc_pair((void **)&L.d_ids, (void **)&L.h_ids, sizeof(int) * cap * 32, 0)) return rc;
@@ -641,13 +649,64 @@ int Engine::launch_lane(Lane &L) {
         if (int rc = stage_groups(L, MRPOP_MODE_DRRT, g1, 1, L.dr, s1)) return rc;
         if (int rc = stage_groups(L, MRPOP_MODE_FULL, g2, 1, L.dr, s2)) return rc;
         if (reset) plan::launch_dr_begin(ds, d, n, st);
-        for (int r = 0; r < rounds; ++r) {
-            plan::launch_dr_expand(ds, d, n, max_b, st);
-            if (int rc = launch_staged(L, s1)) return rc;
-            plan::launch_dr_commit(ds, d, n, st);
-            plan::launch_dr_connect(ds, d, n, max_b, st);
-            if (int rc = launch_staged(L, s2)) return rc;
-            plan::launch_dr_finish(ds, d, n, st);
+        auto enqueue_rounds = [&]() -> int {
+            for (int r = 0; r < rounds; ++r) {
+                plan::launch_dr_expand(ds, d, n, max_b, st);
+                if (int rc = launch_staged(L, s1)) return rc;
+                plan::launch_dr_commit(ds, d, n, st);
+                plan::launch_dr_connect(ds, d, n, max_b, st);
+                if (int rc = launch_staged(L, s2)) return rc;
+                plan::launch_dr_finish(ds, d, n, st);
+            }
+            return MRPOP_OK;
+        };
+        bool graph = true;
+        for (Slot *s : L.dr) graph &= s->opt.graph_rounds != 0;
+        if (!graph) {
+            if (int rc = enqueue_rounds()) return rc;
+        } else {
+            // The rounds read every changing value (slot table, ids, group
+            // descriptors, counts) from device buffers refreshed above, so a
+            // sequence captured once replays for as long as the launch
+            // arguments it bakes in stay the same.  It holds kernel nodes only.
+            std::vector<uint64_t> key = {(uint64_t)(uintptr_t)ds, (uint64_t)(uintptr_t)d, (uint64_t)n,
+                                         (uint64_t)max_b, (uint64_t)rounds, (uint64_t)(uintptr_t)L.d_wave,
+                                         (uint64_t)(uintptr_t)d_worlds_};
+            for (const Staged *sg : {&s1, &s2}) {
+                key.push_back((uint64_t)sg->mode);
+                key.push_back((uint64_t)sg->probe);
+                for (const Staged::Bucket &bk : sg->buckets)
+                    key.insert(key.end(), {(uint64_t)bk.A, (uint64_t)bk.first, (uint64_t)bk.n, (uint64_t)bk.max_waves,
+                                           (uint64_t)bk.probe_waves});
+            }
+            Lane::Graph *hit = nullptr;
+            for (Lane::Graph &g : L.graphs)
+                if (g.key == key) hit = &g;
+            if (!hit) {
+                if (L.graphs.size() >= kGraphsPerLane) {
+                    auto lru = std::min_element(L.graphs.begin(), L.graphs.end(),
+                                                [](const Lane::Graph &a, const Lane::Graph &b) { return a.used < b.used; });
+                    (void)hipGraphExecDestroy(lru->exec);
+                    L.graphs.erase(lru);
+                }
+                MRPOP_HIP(c_, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+                const int rc = enqueue_rounds();
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(st, &g);
+                if (rc || e != hipSuccess) {
+                    if (g) (void)hipGraphDestroy(g);
+                    if (rc) return rc;
+                    return c_->fail(MRPOP_ERR_HIP, "round capture: %s", hipGetErrorString(e));
+                }
+                hipGraphExec_t exec = nullptr;
+                const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) return c_->fail(MRPOP_ERR_HIP, "round graph: %s", hipGetErrorString(ei));
+                L.graphs.push_back(Lane::Graph{std::move(key), exec, 0});
+                hit = &L.graphs.back();
+            }
+            hit->used = ++L.graph_clock;
+            MRPOP_HIP(c_, hipGraphLaunch(hit->exec, st));
         }
     }
     // ---- simplification stage (path_simplification.cu:41-238): slots at the

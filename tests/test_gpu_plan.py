@@ -174,6 +174,22 @@ def test_probe_pass_changes_no_result(ctx, alone):
         assert np.array_equal(a.path, b.path) and a.info["rounds"] == b.info["rounds"]
 
 
+def test_graph_replay_changes_no_result(ctx, alone):
+    """dRRT rounds replayed from captured HIP graphs and launched one kernel
+    at a time (the default) give the same results; a solve replays each
+    captured sequence many times, with other lanes' work between."""
+    replayed = [_lib.solve(ctx, p, _settings(), _options(graph_rounds=1)) for p in PROBLEMS[:3]]
+    for p, s in zip(PROBLEMS[:3], replayed):
+        assert np.array_equal(s.path, alone[p.name].path), p.name
+        assert s.info["rounds"] == alone[p.name].info["rounds"]
+    mixed = [scenes.make_problem(k, 5, _lib.gpu_checker(ctx)) for k in ("plain2", "table3", "bin4", "shelf5")]
+    one = _lib.solve_batch(ctx, mixed, [_settings()] * 4, [_options()] * 4)
+    two = _lib.solve_batch(ctx, mixed, [_settings()] * 4, [_options(graph_rounds=1)] * 4)
+    for a, b in zip(one, two):
+        assert np.array_equal(a.path, b.path) and a.info["rounds"] == b.info["rounds"]
+        assert [c for c, _ in a.seed_paths] == [c for c, _ in b.seed_paths]
+
+
 @pytest.mark.parametrize("kind,seed", [("plain2", 3), ("table3", 5)])
 def test_generated_two_and_three_arm(ctx, kind, seed):
     p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
