@@ -251,7 +251,7 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=64)
     ap.add_argument("--plain-nnc", type=int, default=1024)
     ap.add_argument("--batch-nnc", type=int, default=256)
-    ap.add_argument("--serve-rates", type=str, default="20,50")
+    ap.add_argument("--serve-rates", type=str, default="20,50,100,200")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--budget-problems", type=int, default=256,
                     help="problems of the batched run with a 2 s budget each (0: skip)")

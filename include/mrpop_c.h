@@ -193,6 +193,8 @@ typedef struct mrpop_solve_options {
                                   them, then in full only if still free; same verdicts (default 0) */
     int graph_rounds;          /* 1: replay each step's dRRT rounds from a captured HIP graph (default 0;
                                   same results, no measured gain; not for runs under a kernel tracer) */
+    int arm_split;             /* batches of several arm counts: 1 = one check launch per arm count (tiles of
+                                  64 samples), 0 = one launch of the mixed-arm kernel (default 0) */
 } mrpop_solve_options;
 
 int mrpop_solve_options_default(mrpop_solve_options *o);

@@ -77,6 +77,7 @@ SolveOptions to_options(const mrpop_solve_options *o) {
         so.rounds_per_step = o->rounds_per_step;
         so.probe_samples = o->probe_samples;
         so.graph_rounds = o->graph_rounds;
+        so.arm_split = o->arm_split;
     }
     return so;
 }
@@ -222,6 +223,7 @@ int mrpop_solve_options_default(mrpop_solve_options *o) {
     o->rounds_per_step = d.rounds_per_step;
     o->probe_samples = d.probe_samples;
     o->graph_rounds = d.graph_rounds;
+    o->arm_split = d.arm_split;
     return MRPOP_OK;
 }
 

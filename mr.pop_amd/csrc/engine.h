@@ -31,6 +31,7 @@ struct SolveOptions {
     int rounds_per_step = 8;       // dRRT rounds queued per host step
     int probe_samples = 0;         // > 0: planner edges are checked at this many samples first (see launch_staged)
     int graph_rounds = 0;          // 1: replay a step's dRRT rounds from a captured HIP graph
+    int arm_split = 0;             // 1: one check launch per arm count, not the mixed-arm kernel
 };
 
 struct SeedPath {

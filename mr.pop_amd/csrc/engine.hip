@@ -541,28 +541,41 @@ int Engine::stage_groups(Lane &L, int mode, const std::vector<const dev::EdgeGro
     // use the kernel of the arm count when all problems share it, else the
     // kernel that mixes arm counts (check_kernel_mixed)
     int A_all = mode == MRPOP_MODE_SINGLE ? 1 : (list.empty() ? 0 : list[0]->A);
-    for (Slot *s : list)
+    bool split = false;  // a launch per arm count instead of the mixed kernel
+    for (Slot *s : list) {
         if (mode != MRPOP_MODE_SINGLE && s->A != A_all) A_all = 0;
-    Staged::Bucket bk{A_all, L.group_cursor, 0, 0, 0};
+        split |= s->opt.arm_split != 0;
+    }
     // probe pass: the largest probe any of the problems asks for; it is used
     // only if some group has more samples per edge than the probe
     int probe = 0, max_G = 0;
     for (Slot *s : list) probe = std::max(probe, s->opt.probe_samples);
-    for (size_t i = 0; i < list.size(); ++i) {
-        const int n = mode == MRPOP_MODE_SINGLE ? list[i]->A : 1;
-        const int S = mode == MRPOP_MODE_SINGLE ? 64 : dev::tile_samples(list[i]->A, A_all == 0);
-        for (int k = 0; k < n && k < per_slot; ++k) {
-            if (L.group_cursor >= group_cap_) return c_->fail(MRPOP_ERR_BUDGET, "edge-group staging full");
-            const dev::EdgeGroup &g = groups[i][k];
-            L.h_groups[L.group_cursor++] = g;
-            bk.max_waves += ((size_t)g.n_edges * g.G + S - 1) / S;
-            if (probe > 0) bk.probe_waves += ((size_t)g.n_edges * std::min(probe, g.G) + S - 1) / S;
-            max_G = std::max(max_G, g.G);
+    auto bucket = [&](int A_bucket, bool mixed, auto &&take) -> int {
+        Staged::Bucket bk{A_bucket, L.group_cursor, 0, 0, 0};
+        for (size_t i = 0; i < list.size(); ++i) {
+            if (!take(list[i])) continue;
+            const int n = mode == MRPOP_MODE_SINGLE ? list[i]->A : 1;
+            const int S = mode == MRPOP_MODE_SINGLE ? 64 : dev::tile_samples(list[i]->A, mixed);
+            for (int k = 0; k < n && k < per_slot; ++k) {
+                if (L.group_cursor >= group_cap_) return c_->fail(MRPOP_ERR_BUDGET, "edge-group staging full");
+                const dev::EdgeGroup &g = groups[i][k];
+                L.h_groups[L.group_cursor++] = g;
+                bk.max_waves += ((size_t)g.n_edges * g.G + S - 1) / S;
+                if (probe > 0) bk.probe_waves += ((size_t)g.n_edges * std::min(probe, g.G) + S - 1) / S;
+                max_G = std::max(max_G, g.G);
+            }
         }
+        bk.n = (int)(L.group_cursor - bk.first);
+        if (bk.n) out.buckets.push_back(bk);
+        return MRPOP_OK;
+    };
+    if (A_all != 0 || !split) {
+        if (int rc = bucket(A_all, A_all == 0, [](Slot *) { return true; })) return rc;
+    } else {
+        for (int A = 2; A <= geom::kMaxArms; ++A)
+            if (int rc = bucket(A, false, [A](Slot *s) { return s->A == A; })) return rc;
     }
     out.probe = max_G > probe ? probe : 0;
-    bk.n = (int)(L.group_cursor - bk.first);
-    if (bk.n) out.buckets.push_back(bk);
     if (L.group_cursor > begin)
         MRPOP_HIP(c_, hipMemcpyAsync(L.d_groups + begin, L.h_groups + begin,
                                      sizeof(dev::EdgeGroup) * (L.group_cursor - begin), hipMemcpyHostToDevice, L.stream));

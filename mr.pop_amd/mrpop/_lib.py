@@ -37,7 +37,7 @@ class SolveOptions(ctypes.Structure):
     """mrpop_solve_options (include/mrpop_c.h)."""
     _fields_ = [("block_per_robot", I), ("time_budget_s", ctypes.c_double), ("roadmap_capacity", I),
                 ("tree_capacity", I), ("neighbour_capacity", I), ("neighbour_candidates", I), ("rounds_per_step", I),
-                ("probe_samples", I), ("graph_rounds", I)]
+                ("probe_samples", I), ("graph_rounds", I), ("arm_split", I)]
 
 
 class CProblem(ctypes.Structure):

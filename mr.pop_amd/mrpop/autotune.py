@@ -2,8 +2,10 @@
 
 The batched engine runs many problems per launch; its throughput depends on
 how much work each round creates: expansions per round (num_new_configs),
-roadmap edges checked per new roadmap node (neighbour_candidates) and dRRT
-rounds queued per host step (rounds_per_step).  The tuner runs short batches
+roadmap edges checked per new roadmap node (neighbour_candidates), dRRT
+rounds queued per host step (rounds_per_step) and the check tiling of a
+batch of several arm counts (arm_split: one mixed-arm launch with tiles of
+320/A samples, or one launch per arm count with tiles of 64 samples).  The tuner runs short batches
 of a calibration set for each candidate, discards candidates that leave a
 problem unsolved or raise the median cost by more than `cost_slack` over the
 best candidate, and returns the fastest of the rest.  Granularity is not
@@ -23,6 +25,7 @@ DEFAULT_GRID = {
     "num_new_configs": (128, 256, 512),
     "neighbour_candidates": (16, 32),
     "rounds_per_step": (8,),
+    "arm_split": (0, 1),
 }
 
 

@@ -172,6 +172,10 @@ def test_probe_pass_changes_no_result(ctx, alone):
     two = _lib.solve_batch(ctx, mixed, [_settings()] * 4, [_options(probe_samples=4)] * 4)
     for a, b in zip(one, two):
         assert np.array_equal(a.path, b.path) and a.info["rounds"] == b.info["rounds"]
+    # the other tiling: one check launch per arm count instead of the mixed kernel
+    three = _lib.solve_batch(ctx, mixed, [_settings()] * 4, [_options(arm_split=1)] * 4)
+    for a, b in zip(one, three):
+        assert np.array_equal(a.path, b.path) and a.info["rounds"] == b.info["rounds"]
 
 
 def test_graph_replay_changes_no_result(ctx, alone):
