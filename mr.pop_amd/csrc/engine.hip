@@ -37,13 +37,12 @@ uint64_t ns_since(std::chrono::steady_clock::time_point t) {
 
 // Sizes of one slot's buffers, in the order they are carved.
 struct Layout {
-    size_t off[64];
-    int n = 0;
+    std::vector<size_t> off;
     size_t total = 0;
     size_t add(size_t bytes) {
-        off[n++] = total;
+        off.push_back(total);
         total += align256(bytes);
-        return off[n - 1];
+        return off.back();
     }
 };
 
