@@ -448,19 +448,25 @@ extern "C" int mrpop_diag_slot_roadmap(void *handle, int rounds) {
     return h->engine->debug_roadmap(h->s, rounds);
 }
 
+extern "C" int mrpop_diag_slot_simplify(void *handle, int stage) {
+    auto *h = (DiagSlot *)handle;
+    if (!h) return MRPOP_ERR_INVALID;
+    return h->engine->debug_simplify(h->s, stage);
+}
+
 extern "C" int mrpop_diag_slot_search(void *handle, int rounds, int keep_round) {
     auto *h = (DiagSlot *)handle;
     if (!h) return MRPOP_ERR_INVALID;
     return h->engine->debug_search(h->s, rounds, keep_round != 0);
 }
 
-// Sizes: info[0..9] = A, D, B, M, rm_cap, rm_K, tr_cap, ce_cap, G, path_cap.
+// Sizes: info[0..10] = A, D, B, M, rm_cap, rm_K, tr_cap, ce_cap, G, path_cap, sp_cap.
 extern "C" int mrpop_diag_slot_info(void *handle, int *info) {
     auto *h = (DiagSlot *)handle;
     if (!h || !info) return MRPOP_ERR_INVALID;
     const SlotDev &v = h->s->dv;
-    const int vals[10] = {v.A, v.D, v.B, v.M, v.rm_cap, v.rm_K, v.tr_cap, v.ce_cap, v.G, v.path_cap};
-    for (int i = 0; i < 10; ++i) info[i] = vals[i];
+    const int vals[11] = {v.A, v.D, v.B, v.M, v.rm_cap, v.rm_K, v.tr_cap, v.ce_cap, v.G, v.path_cap, v.sp_cap};
+    for (int i = 0; i < 11; ++i) info[i] = vals[i];
     return MRPOP_OK;
 }
 
@@ -474,6 +480,7 @@ extern "C" long long mrpop_diag_slot_read(void *handle, const char *name, void *
     if (!h || !name) return -1;
     const SlotDev &v = h->s->dv;
     const size_t A = v.A, D = v.D, B = v.B, rc = v.rm_cap, K = v.rm_K, tc = v.tr_cap, cc = v.ce_cap;
+    const size_t pc = v.path_cap, sc = v.sp_cap;
     struct F {
         const char *n;
         const void *p;
@@ -490,7 +497,9 @@ extern "C" long long mrpop_diag_slot_read(void *handle, const char *name, void *
         {"ed_to", v.ed_to, 4 * B * D},         {"cn_near", v.cn_near, 4 * B},        {"cn_next", v.cn_next, 4 * B},
         {"cn_d", v.cn_d, 4 * B},               {"cn_fail", v.cn_fail, 4 * B},        {"ce_b", v.ce_b, 4 * cc},
         {"ce_coll", v.ce_coll, 4 * cc},        {"ce_from", v.ce_from, 4 * cc * D},   {"ce_to", v.ce_to, 4 * cc * D},
-        {"path", v.path, 4 * (size_t)v.path_cap * D},
+        {"path", v.path, 4 * pc * D},          {"sp_fail", v.sp_fail, 4 * pc * 16},  {"sp_mid", v.sp_mid, 4 * pc * D},
+        {"sp_item", v.sp_item, 4 * sc},        {"sp_coll", v.sp_coll, 4 * sc},       {"sp_from", v.sp_from, 4 * sc * D},
+        {"sp_to", v.sp_to, 4 * sc * D},        {"sp_cost", v.sp_cost, 4 * pc},       {"sp_parent", v.sp_parent, 4 * pc},
     };
     for (const F &f : fields) {
         if (std::strcmp(f.n, name) != 0) continue;

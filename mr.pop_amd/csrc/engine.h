@@ -123,6 +123,7 @@ class Engine {
     // arm a's roadmap out: nodes [n][8], degrees [n], neighbour lists [n][K].
     int debug_roadmap(Slot *s, int rounds);
     int debug_search(Slot *s, int rounds, bool keep_round);
+    int debug_simplify(Slot *s, int stage);
     int debug_copy_roadmap(Slot *s, int arm, float *q, int *deg, int *nbr, int *n, int cap);
 
   private:

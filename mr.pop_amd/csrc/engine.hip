@@ -938,6 +938,27 @@ int Engine::debug_search(Slot *s, int rounds, bool keep_round) {
     return MRPOP_OK;
 }
 
+// One simplification stage (0 .. bspline_step) of the slot's recorded path,
+// without the host state machine.
+int Engine::debug_simplify(Slot *s, int stage) {
+    if (stage < 0 || stage > s->st.bspline_step) return c_->fail(MRPOP_ERR_INVALID, "stage %d", stage);
+    if (int rc = ensure_lanes()) return rc;
+    if (int rc = drain()) return rc;
+    Lane &L = lanes_[0];
+    s->phase = Phase::Simplify;
+    s->sp_stage = stage;
+    L.rm.clear();
+    L.dr.clear();
+    L.sp.assign(1, s);
+    const int rc = launch_lane(L);
+    L.sp.clear();
+    if (rc) return rc;
+    MRPOP_HIP(c_, hipEventSynchronize(L.done));
+    L.busy = false;
+    s->lane = -1;
+    return MRPOP_OK;
+}
+
 int Engine::debug_copy_roadmap(Slot *s, int arm, float *q, int *deg, int *nbr, int *n, int cap) {
     if (arm < 0 || arm >= s->A) return c_->fail(MRPOP_ERR_INVALID, "arm %d outside the problem's arms", arm);
     int count = 0;

@@ -100,10 +100,11 @@ class _DiagSlot:
         self.h = ctypes.c_void_p()
         ctx.check(L.mrpop_diag_slot_open(ctx.handle, ctypes.byref(self.ref.c), ctypes.byref(st), ctypes.byref(op),
                                          ctypes.byref(self.h)))
-        info = np.zeros(10, np.int32)
+        L.mrpop_diag_slot_simplify.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        info = np.zeros(11, np.int32)
         ctx.check(L.mrpop_diag_slot_info(self.h, info.ctypes.data))
-        self.A, self.D, self.B, self.M, self.rm_cap, self.K, self.tr_cap, self.ce_cap, self.G, self.path_cap = \
-            (int(v) for v in info)
+        (self.A, self.D, self.B, self.M, self.rm_cap, self.K, self.tr_cap, self.ce_cap, self.G, self.path_cap,
+         self.sp_cap) = (int(v) for v in info)
 
     def close(self):
         self.lib.mrpop_diag_slot_close(self.h)
@@ -113,6 +114,9 @@ class _DiagSlot:
 
     def search(self, rounds, keep_round):
         self.ctx.check(self.lib.mrpop_diag_slot_search(self.h, rounds, int(keep_round)))
+
+    def simplify(self, stage):
+        self.ctx.check(self.lib.mrpop_diag_slot_simplify(self.h, stage))
 
     def read(self, name, dtype, shape):
         n = int(np.prod(shape))
@@ -323,5 +327,77 @@ def test_search_rounds_match_the_restatement(ctx, kind, seed):
             stats.append(_check_round(p, ds, before, ds.state(), st))
         print(f"[search] {kind}: {stats}", flush=True)
         assert sum(x["edges"] for x in stats) > 0 and sum(x["inserted"] for x in stats) > 0
+    finally:
+        ds.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,seed", [("table3", 2), ("shelf5", 3)])
+def test_simplification_stages_match_the_restatement(ctx, kind, seed):
+    """Path simplification (P9): every smoothing stage and the shortcut stage
+    of a freshly recorded path against oracle/simplify.py -- candidates,
+    chunk edges, the resulting path and its cost bit for bit; the chunk
+    verdicts the device used against the FP64 oracle."""
+    from oracle import simplify as SP
+    p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
+    st = _lib.settings(num_new_configs=48, granularity=8, range=2.0)
+    op = _lib.solve_options_default(block_per_robot=1, neighbour_candidates=8, neighbour_capacity=16,
+                                    roadmap_capacity=4096, tree_capacity=4096)
+    ds = _DiagSlot(ctx, p, st, op)
+    try:
+        for _ in range(12):
+            ds.roadmap(1)
+            deg = ds.read("rm_deg", np.int32, (ds.A, ds.rm_cap))
+            if (deg[:, :2] > 0).all():
+                break
+        keep = False
+        for _ in range(40):
+            ds.search(1, keep)
+            keep = True
+            if ds.read("counters", np.int32, (64,))[4] > 0:
+                break
+        ctr = ds.read("counters", np.int32, (64,))
+        assert ctr[4] > 0, "not solved in 40 rounds"
+        N, D = int(ctr[7]), ds.D
+        best = ctr[5:6].view(np.float32)[0]
+        exempt = 0
+        changed = 0
+        stages = int(st.bspline_step)
+        for stage in range(stages + 1):
+            path = ds.read("path", np.float32, (ds.path_cap, D))[:N].copy()
+            smooth = stage < stages
+            print(f"[simplify] {kind} stage {stage} N={N}", flush=True)
+            ds.simplify(stage)
+            fail0, mids, edges = SP.stage_edges(path, smooth, st.range, st.bspline_min_change, ds.sp_cap)
+            n_items = N if smooth else N * 16
+            ne = int(ds.read("counters", np.int32, (64,))[12])
+            assert ne == min(len(edges), ds.sp_cap) or ne == len(edges)
+            items = ds.read("sp_item", np.int32, (ds.sp_cap,))[:len(edges)]
+            frm = ds.read("sp_from", np.float32, (ds.sp_cap, D))[:len(edges)]
+            to = ds.read("sp_to", np.float32, (ds.sp_cap, D))[:len(edges)]
+            coll = ds.read("sp_coll", np.uint32, (ds.sp_cap,))[:len(edges)] != 0
+            assert items.tolist() == [e[0] for e in edges], "chunk order"
+            assert np.array_equal(frm, np.array([e[1] for e in edges]).reshape(-1, D)), "chunk starts"
+            assert np.array_equal(to, np.array([e[2] for e in edges]).reshape(-1, D)), "chunk ends"
+            exempt += _verdicts_agree(p, O.MODE_FULL, ds.G, frm, to, coll)
+            fail = fail0.copy()
+            for it, c in zip(items, coll):
+                if c:
+                    fail[it] = 1
+            got_fail = ds.read("sp_fail", np.uint32, (ds.path_cap * 16,))[:n_items]
+            assert np.array_equal(got_fail != 0, fail != 0), "candidate verdicts"
+            after = ds.read("path", np.float32, (ds.path_cap, D))[:N]
+            changed += int((after != path).any(axis=1).sum())
+            if smooth:
+                want = SP.apply_smooth(path, fail, mids)
+                assert np.array_equal(after, want), f"stage {stage}: smoothed path"
+            else:
+                want, cost, best2 = SP.shortcut(path, fail, best)
+                assert np.array_equal(after, want), "shortcut path"
+                c2 = ds.read("counters", np.int32, (64,))
+                assert c2[6:7].view(np.float32)[0] == cost and c2[5:6].view(np.float32)[0] == best2
+        print(f"[simplify] {kind}: {N} waypoints, {changed} waypoint updates, {exempt} near-contact chunks",
+              flush=True)
+        assert changed > 0  # the stages did move waypoints
     finally:
         ds.close()
