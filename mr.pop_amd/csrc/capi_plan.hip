@@ -448,6 +448,12 @@ extern "C" int mrpop_diag_slot_roadmap(void *handle, int rounds) {
     return h->engine->debug_roadmap(h->s, rounds);
 }
 
+extern "C" int mrpop_diag_slot_pass(void *handle, int pass) {
+    auto *h = (DiagSlot *)handle;
+    if (!h) return MRPOP_ERR_INVALID;
+    return h->engine->debug_start_pass(h->s, pass);
+}
+
 extern "C" int mrpop_diag_slot_simplify(void *handle, int stage) {
     auto *h = (DiagSlot *)handle;
     if (!h) return MRPOP_ERR_INVALID;
@@ -500,7 +506,14 @@ extern "C" long long mrpop_diag_slot_read(void *handle, const char *name, void *
         {"path", v.path, 4 * pc * D},          {"sp_fail", v.sp_fail, 4 * pc * 16},  {"sp_mid", v.sp_mid, 4 * pc * D},
         {"sp_item", v.sp_item, 4 * sc},        {"sp_coll", v.sp_coll, 4 * sc},       {"sp_from", v.sp_from, 4 * sc * D},
         {"sp_to", v.sp_to, 4 * sc * D},        {"sp_cost", v.sp_cost, 4 * pc},       {"sp_parent", v.sp_parent, 4 * pc},
+        {"ex_sample", v.ex_sample, 4 * B * D}, {"ex_crand", v.ex_crand, 4 * B},      {"phs_tf", v.phs_tf, 4 * D * D},
+        {"rw_key", v.rw_key, 8 * 2 * tc},
     };
+    if (std::strcmp(name, "phs_center") == 0) {  // held in the slot descriptor
+        const size_t n = std::min<size_t>(4 * D, cap > 0 ? (size_t)cap : 0);
+        if (out && n) std::memcpy(out, v.phs_center, n);
+        return (long long)(4 * D);
+    }
     for (const F &f : fields) {
         if (std::strcmp(f.n, name) != 0) continue;
         const size_t n = std::min<size_t>(f.bytes, cap > 0 ? (size_t)cap : 0);

@@ -124,6 +124,7 @@ class Engine {
     int debug_roadmap(Slot *s, int rounds);
     int debug_search(Slot *s, int rounds, bool keep_round);
     int debug_simplify(Slot *s, int stage);
+    int debug_start_pass(Slot *s, int pass);
     int debug_copy_roadmap(Slot *s, int arm, float *q, int *deg, int *nbr, int *n, int cap);
 
   private:

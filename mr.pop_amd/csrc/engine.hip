@@ -178,6 +178,7 @@ size_t Engine::slot_bytes(int A, const mrpop_settings &st, const SolveOptions &o
     L.add(sizeof(float) * (size_t)B * D);                                   // ex_q
     L.add(sizeof(int) * B * A);                                             // ex_rid
     L.add(sizeof(unsigned long long) * edge_hash_size(B));                  // ex_hash
+    L.add(sizeof(float) * (size_t)B * (D + 1));                             // ex_sample, ex_crand
     L.add(sizeof(float) * (size_t)B * D * 2 + sizeof(int) * B * 2);         // ed_from/to, ed_b, ed_coll
     L.add(sizeof(int) * B * 3 + sizeof(float) * B);                         // cn_*
     L.add(sizeof(float) * ce_cap * D * 2);                                  // ce_from/to
@@ -336,6 +337,8 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     v.ex_rid = (int *)take(sizeof(int) * B * A);
     v.ex_hash = (unsigned long long *)take(sizeof(unsigned long long) * edge_hash_size(B));
     v.ex_hmask = (int)edge_hash_size(B) - 1;
+    v.ex_sample = (float *)take(sizeof(float) * (size_t)B * (D + 1));
+    v.ex_crand = v.ex_sample + (size_t)B * D;
     float *ed = (float *)take(sizeof(float) * (size_t)B * D * 2 + sizeof(int) * B * 2);
     v.ed_from = ed;
     v.ed_to = ed + (size_t)B * D;
@@ -936,6 +939,18 @@ int Engine::debug_search(Slot *s, int rounds, bool keep_round) {
     L.busy = false;
     s->lane = -1;
     return MRPOP_OK;
+}
+
+// Start AO-x pass `pass` (trees reset, optimise mode from pass 1 on, PHS from
+// the device's best cost) without the host state machine.
+int Engine::debug_start_pass(Slot *s, int pass) {
+    if (pass < 0) return c_->fail(MRPOP_ERR_INVALID, "pass %d", pass);
+    if (int rc = drain()) return rc;
+    float best = 0.f;
+    MRPOP_HIP(c_, hipMemcpy(&best, s->dv.best_cost, sizeof(float), hipMemcpyDeviceToHost));
+    s->best_cost = best;
+    s->pass = pass;
+    return start_pass(s);
 }
 
 // One simplification stage (0 .. bspline_step) of the slot's recorded path,

@@ -541,6 +541,7 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
     }
     __syncthreads();
 
+    if (lane < D) s.ex_sample[(size_t)b * D + lane] = q[lane];
     // ---- informed rejection (pop_planner.cu:449-456)
     {
         const float *r0 = tree_q(s, 0, 0), *r1 = tree_q(s, 1, 0);
@@ -568,6 +569,7 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
         const uint4 cr = slot_random(s.seed, kStreamCostBound, rng_round, (unsigned)b, 0);
         c_rand = u01(cr.x) * c_range + g_hat;
     }
+    if (lane == 0) s.ex_crand[b] = c_rand;
 
     // ---- nearest node of tree t (pop_planner.cu:472-508)
     const int nt = s.tr_n[t];

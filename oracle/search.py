@@ -40,6 +40,8 @@ import numpy as np
 from .roadmap import MASK32, philox4x32_10, u01
 
 STREAM_TREE = 2
+STREAM_COST_BOUND = 3
+STREAM_PHS_RADIUS = 4
 FLT_MAX = np.float32(np.finfo(np.float32).max)
 NONE = 0x7FFFFFFF
 ANGLE_TOL = 1e-5  # radians
@@ -111,22 +113,66 @@ class Slot:
         self.__dict__.update(kw)
 
 
-def expand(s: Slot, b: int):
-    """Expansion b of the round up to the roadmap step.  Returns (tree,
-    near, dnear, cands) with cands[a] the admissible neighbours of arm a as
-    (angle, id) sorted by angle, or None when an arm has none."""
+def tree_choice(s: Slot, b: int) -> int:
     t = (0 if b < s.B // 2 else 1) if s.round == 0 else int(s.ex_tree[b])
     if s.round > 0 and s.balance == 2:
         nt, no = int(s.tr_n[t]), int(s.tr_n[1 - t])
         ratio = np.float32(abs(nt - no)) / np.float32(nt)
         if ratio < np.float32(s.tree_ratio):
             t = 1 - t
-    q = tree_sample(s.seed, s.rng_round, b, s.D, s.scale, s.low)
+    return t
+
+
+def phs_sample64(s: Slot, b: int) -> np.ndarray:
+    """The optimise-pass sample of expansion b in float64 (pop_planner.cu:
+    398-442): the logistic quantile of each uniform times sqrt(pi/8) as an
+    approximate normal, normalised, scaled by u^(1/D), mapped by the
+    hyperspheroid transform (phs_tf, phs_center), clamped to the joint box.
+    The device evaluates logf/powf in float32; compare with a tolerance."""
+    key = (s.seed & MASK32, (s.seed >> 32) & MASK32)
+    D = s.D
+    z = np.zeros(D)
+    for lane in range(D):
+        r = philox4x32_10((s.rng_round, b, lane >> 2, STREAM_TREE), key)
+        u = min(float(u01(r[lane & 3])), 1.0 - 2.0 ** -24)
+        z[lane] = np.log(u / (1.0 - u)) * np.sqrt(np.pi / 8.0)
+    z /= np.linalg.norm(z)
+    rad = float(u01(philox4x32_10((s.rng_round, b, 0, STREAM_PHS_RADIUS), key)[0]))
+    v = s.phs_tf.astype(np.float64) @ (z * rad ** (1.0 / D)) + s.phs_center.astype(np.float64)
+    sc = np.tile(s.scale.astype(np.float64), D // 7)
+    lo = np.tile(s.low.astype(np.float64), D // 7)
+    return np.clip((v - lo) / sc, 0.0, 1.0) * sc + lo
+
+
+def cost_bound64(s: Slot, b: int, t: int, q) -> float:
+    """c_rand (pop_planner.cu:459-469) in float64."""
+    key = (s.seed & MASK32, (s.seed >> 32) & MASK32)
+    q = np.asarray(q, np.float64)
+    g = np.linalg.norm(s.tr_q[t, 0].astype(np.float64) - q)
+    h = np.linalg.norm(s.tr_q[1 - t, 0].astype(np.float64) - q)
+    c_range = max(float(s.best) - (g + h), 0.0)
+    return float(u01(philox4x32_10((s.rng_round, b, 0, STREAM_COST_BOUND), key)[0])) * c_range + g
+
+
+def expand(s: Slot, b: int, q=None, c_rand=None):
+    """Expansion b of the round up to the roadmap step.  Returns (tree,
+    near, dnear, cands, bound) with cands[a] the admissible neighbours of
+    arm a as (angle, id, distance) sorted by angle, or None when an arm has
+    none.  First pass: q from the Philox stream.  Optimise passes: q and
+    c_rand are the device's (checked against phs_sample64 / cost_bound64),
+    the nearest node is taken among nodes cheaper than c_rand and roadmap
+    neighbours farther than the remaining cost bound are not admissible."""
+    t = tree_choice(s, b)
+    if q is None:
+        q = tree_sample(s.seed, s.rng_round, b, s.D, s.scale, s.low)
     nt = int(s.tr_n[t])
     d = sqdist(s.tr_q[t, :nt], q)
+    if c_rand is not None:
+        d = np.where(s.tr_cost[t, :nt] < np.float32(c_rand), d, np.float32(np.inf))
     near = int(np.argmin(d))
     dnear = sqrt32(d[near])
     vn = s.tr_q[t, near]
+    bound = np.float32(c_rand) - s.tr_cost[t, near] if c_rand is not None else None
     cands = []
     for a in range(s.A):
         rid = int(s.tr_rid[t, near, a])
@@ -137,39 +183,45 @@ def expand(s: Slot, b: int):
             if r == rid:
                 continue
             p = s.rm_q[a, r, :7]
-            if sqrt32(sqdist(vn[None, 7 * a:7 * a + 7], p)[0]) > np.float32(s.range):
+            dr = sqrt32(sqdist(vn[None, 7 * a:7 * a + 7], p)[0])
+            if dr > np.float32(s.range):
                 continue
-            c.append((angle64(vn[7 * a:7 * a + 7], q[7 * a:7 * a + 7], p), r))
+            c.append((angle64(vn[7 * a:7 * a + 7], q[7 * a:7 * a + 7], p), r, dr))
         if not c:
-            return t, near, dnear, None
+            return t, near, dnear, None, bound
         cands.append(sorted(c))
-    return t, near, dnear, cands
+    return t, near, dnear, cands, bound
 
 
 def pick_ok(cands_a, r: int) -> bool:
     """The device's pick r for one arm: admissible, and no neighbour's angle
     is smaller by more than ANGLE_TOL (the device compares acosf values)."""
-    ang = dict((rr, a) for a, rr in cands_a)
+    ang = dict((c[1], c[0]) for c in cands_a)
     return r in ang and ang[r] <= cands_a[0][0] + ANGLE_TOL
 
 
 def compose(s: Slot, t: int, near: int, dnear, nrid):
-    """New configuration, its cost (cost(near) + |sample - near|) and
-    whether tree t already holds a node within squared distance 1e-5."""
+    """New configuration, its cost (cost(near) + |sample - near|) and the
+    first node of tree t within squared distance 1e-5 of it (None if none)."""
     nq = np.concatenate([s.rm_q[a, nrid[a], :7] for a in range(s.A)]).astype(np.float32)
     cost = np.float32(s.tr_cost[t, near] + dnear)
     nt = int(s.tr_n[t])
-    exists = bool((sqdist(s.tr_q[t, :nt], nq) <= np.float32(1e-5)).any())
-    return nq, cost, exists
+    close = np.nonzero(sqdist(s.tr_q[t, :nt], nq) <= np.float32(1e-5))[0]
+    return nq, cost, (int(close[0]) if len(close) else None)
 
 
-def connect_plan(s: Slot, t: int, q: np.ndarray, ce_per_b: int):
+def connect_plan(s: Slot, t: int, q: np.ndarray, ce_per_b: int, ub=None):
     """Nearest node of the other tree and the connection chunks
     (pop_planner.cu:624-703), or None when the connection has more chunks
-    than an expansion may use."""
+    than an expansion may use.  Optimise passes: only nodes closer than ub
+    (best cost - cost of the new node) count; None if there is none."""
     o = 1 - t
     no = int(s.tr_n[o])
     d2 = sqdist(s.tr_q[o, :no], q)
+    if ub is not None:
+        d2 = np.where(sqrt32(d2) < np.float32(ub), d2, np.float32(np.inf))
+        if not np.isfinite(d2).any():
+            return None
     on = int(np.argmin(d2))
     d = sqrt32(d2[on])
     n = int(np.float32(d) / np.float32(s.range)) + 1

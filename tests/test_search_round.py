@@ -118,6 +118,10 @@ class _DiagSlot:
     def simplify(self, stage):
         self.ctx.check(self.lib.mrpop_diag_slot_simplify(self.h, stage))
 
+    def start_pass(self, k):
+        self.lib.mrpop_diag_slot_pass.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        self.ctx.check(self.lib.mrpop_diag_slot_pass(self.h, k))
+
     def read(self, name, dtype, shape):
         n = int(np.prod(shape))
         out = np.zeros(n, dtype)
@@ -147,6 +151,7 @@ class _DiagSlot:
             ce_from=self.read("ce_from", np.float32, (self.ce_cap, D)),
             ce_to=self.read("ce_to", np.float32, (self.ce_cap, D)),
             path=self.read("path", np.float32, (self.path_cap, D)),
+            ex_sample=self.read("ex_sample", np.float32, (B, D)), ex_crand=self.read("ex_crand", np.float32, (B,)),
             rm_q=self.read("rm_q", np.float32, (A, rc, 8)), rm_nbr=self.read("rm_nbr", np.int32, (A, rc, K)),
             rm_deg=self.read("rm_deg", np.int32, (A, rc)))
 
@@ -163,32 +168,72 @@ def _verdicts_agree(problem, mode, G, frm, to, device):
     return int((~robust).sum())
 
 
-def _check_round(p, ds, before, after, settings):
-    """Restate one round from `before` and compare with `after`."""
+def _check_round(p, ds, before, after, settings, optimize=False, phs=None):
+    """Restate one round from `before` and compare with `after`.  First
+    pass: everything from the state before the round.  Optimise passes: the
+    sample and its cost bound are the device's, checked in float64 against
+    the hyperspheroid sampler and the bound formula; the informed rejection,
+    the cost-filtered nearest node, the bounded roadmap step, rewires and
+    inserts are restated from them."""
     A, D, B = ds.A, ds.D, ds.B
     s = SR.Slot(A=A, D=D, B=B, K=ds.K, seed=p.seed, range=settings.range, balance=settings.balance,
                 tree_ratio=settings.tree_ratio, round=before["round"], rng_round=before["rng_round"],
                 scale=geometry.table("kScale"), low=geometry.table("kLow"), tr_n=before["tr_n"],
                 tr_q=before["tr_q"], tr_cost=before["tr_cost"], tr_rid=before["tr_rid"], ex_tree=before["ex_tree"],
-                rm_q=before["rm_q"], rm_nbr=before["rm_nbr"], rm_deg=before["rm_deg"])
-    assert before["status"] == 0 and before["best"] == SR.FLT_MAX  # first pass, still searching
+                rm_q=before["rm_q"], rm_nbr=before["rm_nbr"], rm_deg=before["rm_deg"], best=before["best"])
+    assert before["status"] == 0
+    if not optimize:
+        assert before["best"] == SR.FLT_MAX  # first pass, still searching
+    else:
+        s.phs_tf, s.phs_center = phs
+    best = before["best"]
     cand = {}
+    rejected = 0
     for b in range(B):
-        t, near, dnear, cands = SR.expand(s, b)
+        t = SR.tree_choice(s, b)
         assert after["ex_tree"][b] == t, f"expansion {b}: tree"
+        q = c_rand = None
+        if optimize:
+            q = after["ex_sample"][b]
+            q64 = SR.phs_sample64(s, b)
+            assert np.abs(q.astype(np.float64) - q64).max() < 1e-4, f"expansion {b}: PHS sample"
+            g = (np.linalg.norm(s.tr_q[0, 0].astype(np.float64) - q) + np.linalg.norm(s.tr_q[1, 0].astype(np.float64) - q))
+            assert abs(g - float(best)) > 1e-5, f"expansion {b}: informed rejection too close to call"
+            if g >= best:
+                rejected += 1
+                continue
+            c_rand = after["ex_crand"][b]
+            assert abs(float(c_rand) - SR.cost_bound64(s, b, t, q)) < 1e-4, f"expansion {b}: cost bound"
+        t, near, dnear, cands, bound = SR.expand(s, b, q, c_rand)
         if cands is None:
             continue
-        # least angle; among angles within ANGLE_TOL of it the device's pick
+        # least angle among the admissible (within the cost bound left, which
+        # each arm's step lowers); among angles within ANGLE_TOL of it the
+        # device's pick
         nrid = []
+        vn = s.tr_q[t, near]
         for a in range(A):
-            r0 = cands[a][0][1]
-            tie = len(cands[a]) > 1 and cands[a][1][0] <= cands[a][0][0] + SR.ANGLE_TOL
+            adm = [c for c in cands[a] if bound is None or not (c[2] > bound)]
+            if not adm:
+                break
+            r0 = adm[0][1]
+            tie = len(adm) > 1 and adm[1][0] <= adm[0][0] + SR.ANGLE_TOL
             dev = int(after["ex_rid"][b][a])
-            nrid.append(dev if tie and SR.pick_ok(cands[a], dev) else r0)
-        nq, cost, exists = SR.compose(s, t, near, dnear, nrid)
-        if exists:
+            nrid.append(dev if tie and SR.pick_ok(adm, dev) else r0)
+            if bound is not None:
+                step = SR.sqrt32(SR.sqdist(s.rm_q[a, nrid[-1], :7][None], vn[7 * a:7 * a + 7])[0])
+                bound = np.float32(bound - step)
+        if len(nrid) < A:
             continue
-        cand[b] = dict(t=t, near=near, nrid=tuple(nrid), nq=nq, cost=cost)
+        nq, cost, exist = SR.compose(s, t, near, dnear, nrid)
+        state = 1
+        if exist is not None:
+            if not optimize or not cost < s.tr_cost[t, exist]:
+                continue
+            state = 2
+        if optimize and not cost < best:
+            continue
+        cand[b] = dict(t=t, near=near, nrid=tuple(nrid), nq=nq, cost=cost, state=state, exist=exist)
     # candidates: one edge per distinct (tree, near, roadmap nodes)
     groups = {}
     for b, c in cand.items():
@@ -220,15 +265,28 @@ def _check_round(p, ds, before, after, settings):
         for b in members:
             verdict[b] = after["ed_coll"][e] != 0
     exempt = _verdicts_agree(p, O.MODE_DRRT, ds.G, frm, to, dev_v)
-    # commit: free candidates inserted in expansion order per tree
+    # commit: rewires (lowest cost, then lowest expansion, per existing node),
+    # then free inserts in expansion order per tree
     tr_q, tr_parent = before["tr_q"].copy(), before["tr_parent"].copy()
     tr_cost, tr_rid, tr_n = before["tr_cost"].copy(), before["tr_rid"].copy(), before["tr_n"].copy()
+    rewires = {}
+    for b in sorted(cand):
+        c = cand[b]
+        if c["state"] == 2 and not verdict[b]:
+            k = (c["t"], c["exist"])
+            if k not in rewires or (c["cost"], b) < (cand[rewires[k]]["cost"], rewires[k]):
+                rewires[k] = b
+    for (t, x), b in rewires.items():
+        tr_parent[t, x], tr_cost[t, x] = cand[b]["near"], cand[b]["cost"]
     index = {}
     for b in sorted(cand):
         if verdict[b]:
             assert after["ex_state"][b] == 0 and after["ex_index"][b] == -1, f"expansion {b}: colliding edge committed"
             continue
         c = cand[b]
+        if c["state"] == 2:
+            assert after["ex_state"][b] == 2 and after["ex_index"][b] == -1, f"expansion {b}: rewire"
+            continue
         t, idx = c["t"], int(tr_n[c["t"]])
         assert idx < ds.tr_cap
         tr_n[t] += 1
@@ -245,7 +303,8 @@ def _check_round(p, ds, before, after, settings):
             assert after["cn_near"][b] == -1
             continue
         t = cand[b]["t"]
-        plan = SR.connect_plan(s2, t, tr_q[t, index[b]], ds.ce_cap // B)
+        ub = np.float32(best - tr_cost[t, index[b]]) if optimize else None
+        plan = SR.connect_plan(s2, t, tr_q[t, index[b]], ds.ce_cap // B, ub)
         if plan is None:
             assert after["cn_near"][b] == -1
             continue
@@ -296,6 +355,7 @@ def _check_round(p, ds, before, after, settings):
         assert np.array_equal(after["path"][:len(path)], path), "recorded path"
         assert after["path_cost"] == cost and after["best"] == cost
     return dict(candidates=len(cand), edges=n_edges, inserted=len(index), connections=len(plans),
+                rejected=rejected, rewired=len(rewires),
                 solved=winner is not None, exempt=exempt, n_commit=n_commit)
 
 
@@ -399,5 +459,55 @@ def test_simplification_stages_match_the_restatement(ctx, kind, seed):
         print(f"[simplify] {kind}: {N} waypoints, {changed} waypoint updates, {exempt} near-contact chunks",
               flush=True)
         assert changed > 0  # the stages did move waypoints
+    finally:
+        ds.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,seed", [("table3", 2), ("shelf5", 3)])
+def test_optimise_pass_rounds_match_the_restatement(ctx, kind, seed):
+    """An AO-x optimise pass (P3 PHS sampling, P4 cost-filtered nearest node,
+    P5 bounded roadmap step, P6 rewire): after the first pass solves, the
+    next pass starts with the best cost as bound and its rounds are restated
+    from the device's samples (checked in float64 against the hyperspheroid
+    sampler) -- rejections, cost bounds, candidates, rewires, inserts,
+    connections and trees bit for bit."""
+    p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
+    st = _lib.settings(num_new_configs=48, granularity=8, range=2.0)
+    op = _lib.solve_options_default(block_per_robot=1, neighbour_candidates=8, neighbour_capacity=16,
+                                    roadmap_capacity=4096, tree_capacity=4096)
+    ds = _DiagSlot(ctx, p, st, op)
+    try:
+        for _ in range(12):
+            ds.roadmap(1)
+            deg = ds.read("rm_deg", np.int32, (ds.A, ds.rm_cap))
+            if (deg[:, :2] > 0).all():
+                break
+        keep = False
+        for _ in range(40):
+            ds.search(1, keep)
+            keep = True
+            if ds.read("counters", np.int32, (64,))[4] > 0:
+                break
+        assert ds.read("counters", np.int32, (64,))[4] > 0, "first pass not solved in 40 rounds"
+        ds.start_pass(1)
+        phs = (ds.read("phs_tf", np.float32, (ds.D, ds.D)), ds.read("phs_center", np.float32, (ds.D,)))
+        stats = []
+        keep = False
+        for _ in range(6):
+            before = ds.state()
+            if before["status"] != 0:
+                break
+            if not keep:
+                before["round"] = 0  # a fresh launch starts its round count at 0
+            ds.search(1, keep)
+            keep = True
+            stats.append(_check_round(p, ds, before, ds.state(), st, optimize=True, phs=phs))
+        print(f"[optimise] {kind}: {stats}", flush=True)
+        # samples come from the informed set, so rejections are rare (clamping
+        # to the joint box only); rewires must occur somewhere in the set
+        assert sum(x["candidates"] for x in stats) > 0
+        if kind == "table3":
+            assert sum(x["rewired"] for x in stats) > 0
     finally:
         ds.close()
