@@ -1,4 +1,4 @@
-"""CPU restatement of one dRRT search round of the planner (first AO-x pass).
+"""CPU restatement of one dRRT search round of the planner.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, never by the product.
 
@@ -26,6 +26,13 @@ each expansion b of the B of a round:
     whose chunks are all free adds its end node (cost: one chunk's length,
     pop_planner.cu:624-731), and the lowest such expansion records the path
     goal -> start (path_processing.cu:10-70).
+
+Optimise passes (AO-x pass >= 1, pop_planner.cu:389-469) draw the sample
+from the prolate hyperspheroid around start and goal, reject samples whose
+distance sum reaches the best cost, bound node costs by c_rand (nearest
+node among cheaper nodes, roadmap steps within the remaining bound) and
+rewire an existing node when the new cost is lower; phs_sample64 and
+cost_bound64 restate the sampler and the bound in float64.
 
 Float32 arithmetic is reproduced exactly where the device's is exact:
 fused multiply-adds (fma32, below), correctly rounded division and square
