@@ -87,11 +87,11 @@ typedef struct mrpop_settings {
     float range;
     int balance;
     float tree_ratio;
-    int dynamic_domain;
+    int dynamic_domain;        /* accepted; never skips an expansion, as in the reference (radii stay FLT_MAX) */
     float dd_alpha;
     float dd_radius;
     float dd_min_radius;
-    int roadmap_reassign;
+    int roadmap_reassign;      /* 1: arms whose start and goal are connected draw no more roadmap samples */
     int rrtc_iter;
     int resample_iter;
     int cost_bound_resample;

@@ -291,6 +291,7 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     v.rm_n = small;
     v.rm_connected = small + 8;
     v.re_count = small + 16;
+    v.rs_nsamp = small + 24;
     v.rs_q = (float *)take(sizeof(float) * A * M * 7);
     v.rs_coll = (uint32_t *)take(sizeof(uint32_t) * A * M);
     v.rs_id = (int *)take(sizeof(int) * A * M);
@@ -375,6 +376,7 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     v.world = id;
     v.seed = seed;
     v.G = st.granularity;
+    v.rm_reassign = st.roadmap_reassign ? 1 : 0;
     v.B = B;
     v.M = M;
     v.range = st.range;
@@ -408,7 +410,8 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
 
     // ---- edge groups (pointers and counts are fixed for the slot's life)
     for (int a = 0; a < A; ++a) {
-        s->g_valid[a] = dev::EdgeGroup{v.rs_q + (size_t)a * M * 7, v.rs_q + (size_t)a * M * 7, 7, 1, M, a, nullptr,
+        s->g_valid[a] = dev::EdgeGroup{v.rs_q + (size_t)a * M * 7, v.rs_q + (size_t)a * M * 7, 7, 1, M, a,
+                                       v.rs_nsamp + a,
                                        v.rs_coll + (size_t)a * M, nullptr, id, 0};
         s->g_rm[a] = dev::EdgeGroup{v.re_from + (size_t)a * M * Kc * 7, v.re_to + (size_t)a * M * Kc * 7, 7,
                                     st.granularity, M * Kc, a, v.re_count + a, v.re_coll + (size_t)a * M * Kc, nullptr,

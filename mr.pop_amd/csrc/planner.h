@@ -60,6 +60,8 @@ struct SlotDev {
     int *rm_uf;               // [A][rm_cap]   union-find parent
     int *rm_n;                // [A]
     int *rm_connected;        // [A]
+    int *rs_nsamp;            // [A] samples drawn this round (0: the arm sits the round out)
+    int rm_reassign;          // roadmap_reassign: arms already connected draw no more samples
     // roadmap round staging
     float *rs_q;              // [A][M][7]
     uint32_t *rs_coll;        // [A][M]

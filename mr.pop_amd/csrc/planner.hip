@@ -92,6 +92,15 @@ __global__ void k_rm_sample(SlotDev *slots, const int *ids) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= s.A * s.M) return;
     const int a = t / s.M, i = t - a * s.M;
+    // roadmap_reassign (roadmap_interior.cuh:316-327): once an arm's start and
+    // goal are connected, its share of the round goes to the arms still
+    // unconnected -- here the arm draws no samples and its checks drop out
+    const bool skip = s.rm_reassign && s.rm_connected[a];
+    if (i == 0) s.rs_nsamp[a] = skip ? 0 : s.M;
+    if (skip) {
+        s.rs_coll[(size_t)a * s.M + i] = 1u;  // no new nodes
+        return;
+    }
     const uint4 r0 = slot_random(s.seed, kStreamRoadmap, (unsigned)(s.rm_round * kMaxArms + a), (unsigned)i, 0);
     const uint4 r1 = slot_random(s.seed, kStreamRoadmap, (unsigned)(s.rm_round * kMaxArms + a), (unsigned)i, 1);
     const unsigned bits[7] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};

@@ -15,12 +15,15 @@ struct pop_settings {
     int balance = 2;             // 0 none, 1 or 2: tree balancing strategy
     float tree_ratio = 1.0;
 
-    bool dynamic_domain = false; // accepted; not used on the hot path
+    // accepted; behaves as in the reference, where every node radius starts at
+    // FLT_MAX and only finite radii grow (pop_planner.cu:534-537, 591-604,
+    // 860-863), so the radius test never skips an expansion
+    bool dynamic_domain = false;
     float dd_alpha = 0.0001;
     float dd_radius = 4.0;
     float dd_min_radius = 1.0;
 
-    bool roadmap_reassign = false;  // accepted; not used on the hot path
+    bool roadmap_reassign = false;  // arms whose start and goal are connected draw no more roadmap samples
 
     int rrtc_iter = 1;           // AO-x passes
     int resample_iter = 5;

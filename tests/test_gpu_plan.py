@@ -194,6 +194,20 @@ def test_graph_replay_changes_no_result(ctx, alone):
         assert [c for c, _ in a.seed_paths] == [c for c, _ in b.seed_paths]
 
 
+def test_roadmap_reassign(ctx):
+    """roadmap_reassign: arms whose start and goal are connected stop
+    sampling, the others go on; paths stay valid and the connected arms'
+    roadmaps stop growing."""
+    probs = [scenes.make_problem(k, 7, _lib.gpu_checker(ctx)) for k in ("bin4", "shelf5")]
+    for p in probs:
+        base = _lib.solve(ctx, p, _settings(), _options())
+        re = _lib.solve(ctx, p, _settings(roadmap_reassign=1), _options())
+        check_solution(p, re)
+        assert sum(re.info["roadmap_size"]) <= sum(base.info["roadmap_size"]), p.name
+        print(f"[reassign] {p.name}: roadmap {base.info['roadmap_size']} -> {re.info['roadmap_size']}, "
+              f"ttfs {base.ttfs_s:.4f} -> {re.ttfs_s:.4f} s", flush=True)
+
+
 @pytest.mark.parametrize("kind,seed", [("plain2", 3), ("table3", 5)])
 def test_generated_two_and_three_arm(ctx, kind, seed):
     p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
