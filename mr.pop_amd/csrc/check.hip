@@ -18,24 +18,25 @@
 //   A sample collides iff some fine test does; an edge iff some sample does.
 //
 // MI355X mapping.
-//   check_kernel: one wave per block; lane = (sample, arm), 64/A samples per
-//   wave.  FK runs per lane in registers with the fixed joint transforms
-//   folded in; coarse centres go to LDS so that the arm-arm pairs split evenly
-//   over the A lanes of a sample (each lane takes A-1 half-pairs: rows 0-5 or
-//   6-10 of arm a against arm b).  Obstacle records are wave-uniform and come
-//   through the scalar cache.  The fine self and arm-arm tests run in the same
-//   kernel, gated exactly as above, but only where they can hit: each coarse
-//   sphere grown by a few tenths of a millimetre (kBlockInflate) contains the
-//   fine spheres of its block, so a fine pair is tested only if its bounding
-//   balls overlap.  The culling never changes a verdict; it removes ~98% of
-//   the fine pairs, including the joint-5 ranges that the coarse pair
-//   (sphere 5, spheres 7..10) switches on for every configuration.
-//   env_fine_kernel: samples with environment bits and no verdict yet are
-//   appended to a compacted queue (ballot + prefix count, one atomic per
-//   wave); a persistent grid takes one queued sample per wave, builds its
-//   frames in LDS and tests the gated fine spheres against every obstacle.
-//   The last wave to finish resets the queue counter, so back-to-back
-//   launches need no memset.
+//   check_kernel<A, MODE>: a block of A waves checks a tile of 64 samples;
+//   wave a holds arm a of every sample (arm-major), so the obstacle and
+//   arm-pair culls of a wave see one arm of 64 neighbouring samples.  FK runs
+//   per item in registers with the fixed joint transforms folded in; coarse
+//   centres and a bounding ball per arm go to LDS, where the arm-arm pairs
+//   split evenly over the A items of a sample (each takes A-1 half-pairs:
+//   rows 0-5 or 6-10 of arm a against arm b; rows whose sphere misses arm b's
+//   ball are skipped).  Obstacle records are wave-uniform and come through
+//   the scalar cache; a per-item cull box skips obstacles far from the arm.
+//   The fine self, environment and arm-arm tests run in the same kernel,
+//   gated exactly as above, but only where they can hit: each coarse sphere
+//   grown by a few tenths of a millimetre (kBlockInflate) contains the fine
+//   spheres of its block, so a fine pair or obstacle is tested only if its
+//   block ball overlaps.  The culling never changes a verdict.  A tile needs
+//   three barriers; a persistent grid of resident blocks loops over the
+//   tiles of many edge groups (check_loop).
+//   check_kernel_mixed<MODE>: blocks of five waves, tiles of 320/A samples of
+//   the group's arm count, for planner rounds over problems of several arm
+//   counts.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
