@@ -98,7 +98,7 @@ struct SlotDev {
     int *ex_index;            // [B] node index after insertion (-1 none)
     int *ex_dup;              // [B] expansion whose identical edge is checked for b (-1 none)
     float *ex_sample;         // [B][D] the round's sample of expansion b
-    float *ex_crand;          // [B] its cost bound (optimise passes; FLT_MAX otherwise)
+    float *ex_crand;          // [B] its cost bound (optimise passes; FLT_MAX otherwise; -1 if rejected)
     unsigned long long *ex_hash;  // [ex_hmask+1] edges of the round: (round tag << 32) | b
     int ex_hmask;
     unsigned long long *rw_key;  // [tr_cap*2] rewire arbitration (cost bits << 32 | b)

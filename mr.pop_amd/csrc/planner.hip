@@ -551,6 +551,7 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
     __syncthreads();
 
     if (lane < D) s.ex_sample[(size_t)b * D + lane] = q[lane];
+    if (lane == 0) s.ex_crand[b] = -1.f;  // stays -1 if the sample is rejected
     // ---- informed rejection (pop_planner.cu:449-456)
     {
         const float *r0 = tree_q(s, 0, 0), *r1 = tree_q(s, 1, 0);
