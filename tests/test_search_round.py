@@ -396,6 +396,71 @@ def test_search_rounds_match_the_restatement(ctx, kind, seed):
         ds.close()
 
 
+def _open_solved_slot(ctx, p, st, op, rounds=40):
+    """A diagnostic slot whose first pass has run until solved."""
+    ds = _DiagSlot(ctx, p, st, op)
+    for _ in range(12):
+        ds.roadmap(1)
+        deg = ds.read("rm_deg", np.int32, (ds.A, ds.rm_cap))
+        if (deg[:, :2] > 0).all():
+            break
+    keep = False
+    for _ in range(rounds):
+        ds.search(1, keep)
+        keep = True
+        if ds.read("counters", np.int32, (64,))[4] > 0:
+            break
+    return ds
+
+
+def _check_simplify(p, ds, st):
+    """Every smoothing stage and the shortcut stage of the slot's recorded
+    path against oracle/simplify.py.  Returns (waypoint updates, exempt)."""
+    from oracle import simplify as SP
+    ctr = ds.read("counters", np.int32, (64,))
+    N, D = int(ctr[7]), ds.D
+    best = ctr[5:6].view(np.float32)[0]
+    exempt = changed = 0
+    stages = int(st.bspline_step)
+    for stage in range(stages + 1):
+        path = ds.read("path", np.float32, (ds.path_cap, D))[:N].copy()
+        smooth = stage < stages
+        ds.simplify(stage)
+        fail0, mids, edges = SP.stage_edges(path, smooth, st.range, st.bspline_min_change, ds.sp_cap)
+        n_items = N if smooth else N * 16
+        assert int(ds.read("counters", np.int32, (64,))[12]) == len(edges)
+        items = ds.read("sp_item", np.int32, (ds.sp_cap,))[:len(edges)]
+        frm = ds.read("sp_from", np.float32, (ds.sp_cap, D))[:len(edges)]
+        to = ds.read("sp_to", np.float32, (ds.sp_cap, D))[:len(edges)]
+        coll = ds.read("sp_coll", np.uint32, (ds.sp_cap,))[:len(edges)] != 0
+        assert items.tolist() == [e[0] for e in edges], "chunk order"
+        if edges:
+            assert np.array_equal(frm, np.array([e[1] for e in edges]).reshape(-1, D)), "chunk starts"
+            assert np.array_equal(to, np.array([e[2] for e in edges]).reshape(-1, D)), "chunk ends"
+            exempt += _verdicts_agree(p, O.MODE_FULL, ds.G, frm, to, coll)
+        fail = fail0.copy()
+        for it, c in zip(items, coll):
+            if c:
+                fail[it] = 1
+        got_fail = ds.read("sp_fail", np.uint32, (ds.path_cap * 16,))[:n_items]
+        assert np.array_equal(got_fail != 0, fail != 0), "candidate verdicts"
+        after = ds.read("path", np.float32, (ds.path_cap, D))[:N]
+        changed += int((after != path).any(axis=1).sum())
+        if smooth:
+            assert np.array_equal(after, SP.apply_smooth(path, fail, mids)), f"stage {stage}: smoothed path"
+        else:
+            want, cost, best2 = SP.shortcut(path, fail, best)
+            assert np.array_equal(after, want), "shortcut path"
+            c2 = ds.read("counters", np.int32, (64,))
+            assert c2[6:7].view(np.float32)[0] == cost and c2[5:6].view(np.float32)[0] == best2
+    return changed, exempt
+
+
+_SMALL = dict(num_new_configs=48, granularity=8, range=2.0)
+_SMALL_OPT = dict(block_per_robot=1, neighbour_candidates=8, neighbour_capacity=16, roadmap_capacity=4096,
+                  tree_capacity=4096)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind,seed", [("table3", 2), ("shelf5", 3)])
 def test_simplification_stages_match_the_restatement(ctx, kind, seed):
@@ -403,67 +468,50 @@ def test_simplification_stages_match_the_restatement(ctx, kind, seed):
     of a freshly recorded path against oracle/simplify.py -- candidates,
     chunk edges, the resulting path and its cost bit for bit; the chunk
     verdicts the device used against the FP64 oracle."""
-    from oracle import simplify as SP
     p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
-    st = _lib.settings(num_new_configs=48, granularity=8, range=2.0)
-    op = _lib.solve_options_default(block_per_robot=1, neighbour_candidates=8, neighbour_capacity=16,
-                                    roadmap_capacity=4096, tree_capacity=4096)
-    ds = _DiagSlot(ctx, p, st, op)
+    st = _lib.settings(**_SMALL)
+    ds = _open_solved_slot(ctx, p, st, _lib.solve_options_default(**_SMALL_OPT))
+    try:
+        assert ds.read("counters", np.int32, (64,))[4] > 0, "not solved in 40 rounds"
+        changed, exempt = _check_simplify(p, ds, st)
+        print(f"[simplify] {kind}: {changed} waypoint updates, {exempt} near-contact chunks", flush=True)
+        assert changed > 0  # the stages did move waypoints
+    finally:
+        ds.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["plain2", "table3", "bin4", "shelf5"])
+@pytest.mark.parametrize("seed", [10, 11, 12])
+def test_rounds_and_simplification_sweep(ctx, kind, seed):
+    """More problems through the same restatements: first-pass rounds until
+    solved, the simplification stages of the recorded path, then optimise
+    rounds (up to 30 rounds in all)."""
+    p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
+    st = _lib.settings(num_new_configs=64, granularity=16, range=2.0)
+    ds = _DiagSlot(ctx, p, st, _lib.solve_options_default(**_SMALL_OPT))
     try:
         for _ in range(12):
             ds.roadmap(1)
             deg = ds.read("rm_deg", np.int32, (ds.A, ds.rm_cap))
             if (deg[:, :2] > 0).all():
                 break
-        keep = False
-        for _ in range(40):
+        keep, opt, phs = False, False, None
+        for _ in range(30):
+            before = ds.state()
+            if before["status"] != 0:
+                if opt:
+                    break
+                _check_simplify(p, ds, st)
+                ds.start_pass(1)
+                opt, keep = True, False
+                phs = (ds.read("phs_tf", np.float32, (ds.D, ds.D)), ds.read("phs_center", np.float32, (ds.D,)))
+                before = ds.state()
+            if not keep:
+                before["round"] = 0
             ds.search(1, keep)
             keep = True
-            if ds.read("counters", np.int32, (64,))[4] > 0:
-                break
-        ctr = ds.read("counters", np.int32, (64,))
-        assert ctr[4] > 0, "not solved in 40 rounds"
-        N, D = int(ctr[7]), ds.D
-        best = ctr[5:6].view(np.float32)[0]
-        exempt = 0
-        changed = 0
-        stages = int(st.bspline_step)
-        for stage in range(stages + 1):
-            path = ds.read("path", np.float32, (ds.path_cap, D))[:N].copy()
-            smooth = stage < stages
-            print(f"[simplify] {kind} stage {stage} N={N}", flush=True)
-            ds.simplify(stage)
-            fail0, mids, edges = SP.stage_edges(path, smooth, st.range, st.bspline_min_change, ds.sp_cap)
-            n_items = N if smooth else N * 16
-            ne = int(ds.read("counters", np.int32, (64,))[12])
-            assert ne == min(len(edges), ds.sp_cap) or ne == len(edges)
-            items = ds.read("sp_item", np.int32, (ds.sp_cap,))[:len(edges)]
-            frm = ds.read("sp_from", np.float32, (ds.sp_cap, D))[:len(edges)]
-            to = ds.read("sp_to", np.float32, (ds.sp_cap, D))[:len(edges)]
-            coll = ds.read("sp_coll", np.uint32, (ds.sp_cap,))[:len(edges)] != 0
-            assert items.tolist() == [e[0] for e in edges], "chunk order"
-            assert np.array_equal(frm, np.array([e[1] for e in edges]).reshape(-1, D)), "chunk starts"
-            assert np.array_equal(to, np.array([e[2] for e in edges]).reshape(-1, D)), "chunk ends"
-            exempt += _verdicts_agree(p, O.MODE_FULL, ds.G, frm, to, coll)
-            fail = fail0.copy()
-            for it, c in zip(items, coll):
-                if c:
-                    fail[it] = 1
-            got_fail = ds.read("sp_fail", np.uint32, (ds.path_cap * 16,))[:n_items]
-            assert np.array_equal(got_fail != 0, fail != 0), "candidate verdicts"
-            after = ds.read("path", np.float32, (ds.path_cap, D))[:N]
-            changed += int((after != path).any(axis=1).sum())
-            if smooth:
-                want = SP.apply_smooth(path, fail, mids)
-                assert np.array_equal(after, want), f"stage {stage}: smoothed path"
-            else:
-                want, cost, best2 = SP.shortcut(path, fail, best)
-                assert np.array_equal(after, want), "shortcut path"
-                c2 = ds.read("counters", np.int32, (64,))
-                assert c2[6:7].view(np.float32)[0] == cost and c2[5:6].view(np.float32)[0] == best2
-        print(f"[simplify] {kind}: {N} waypoints, {changed} waypoint updates, {exempt} near-contact chunks",
-              flush=True)
-        assert changed > 0  # the stages did move waypoints
+            _check_round(p, ds, before, ds.state(), st, optimize=opt, phs=phs)
     finally:
         ds.close()
 
@@ -478,22 +526,9 @@ def test_optimise_pass_rounds_match_the_restatement(ctx, kind, seed):
     sampler) -- rejections, cost bounds, candidates, rewires, inserts,
     connections and trees bit for bit."""
     p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
-    st = _lib.settings(num_new_configs=48, granularity=8, range=2.0)
-    op = _lib.solve_options_default(block_per_robot=1, neighbour_candidates=8, neighbour_capacity=16,
-                                    roadmap_capacity=4096, tree_capacity=4096)
-    ds = _DiagSlot(ctx, p, st, op)
+    st = _lib.settings(**_SMALL)
+    ds = _open_solved_slot(ctx, p, st, _lib.solve_options_default(**_SMALL_OPT))
     try:
-        for _ in range(12):
-            ds.roadmap(1)
-            deg = ds.read("rm_deg", np.int32, (ds.A, ds.rm_cap))
-            if (deg[:, :2] > 0).all():
-                break
-        keep = False
-        for _ in range(40):
-            ds.search(1, keep)
-            keep = True
-            if ds.read("counters", np.int32, (64,))[4] > 0:
-                break
         assert ds.read("counters", np.int32, (64,))[4] > 0, "first pass not solved in 40 rounds"
         ds.start_pass(1)
         phs = (ds.read("phs_tf", np.float32, (ds.D, ds.D)), ds.read("phs_center", np.float32, (ds.D,)))
