@@ -70,7 +70,8 @@ def _dump(ctx, p, st, op, rounds, arm, cap, K):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind,seed,arm", [("plain2", 4, 0), ("bin4", 1, 2)])
+@pytest.mark.parametrize("kind,seed,arm", [("plain2", 4, 0), ("bin4", 1, 2), ("table3", 3, 0), ("bin4", 9, 3),
+                                           ("shelf5", 2, 4), ("shelf5", 6, 1)])
 def test_roadmap_rounds_match_the_restatement(ctx, kind, seed, arm):
     p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
     G, rng, kc, K, bpr, rounds = 8, 2.0, 4, 6, 1, 3
