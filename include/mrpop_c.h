@@ -217,7 +217,11 @@ typedef struct mrpop_result_info {
     int rounds;                /* dRRT rounds run */
     int roadmap_rounds;
     int roadmap_size[MRPOP_MAX_ARMS];
-    int status;                /* MRPOP_OK, or MRPOP_ERR_TIMEOUT if the time budget ended it */
+    int status;                /* MRPOP_OK; MRPOP_ERR_TIMEOUT if the time budget ended it;
+                                  MRPOP_ERR_INVALID if its settings, options or configurations
+                                  were rejected (granularity outside 1..64, num_new_configs
+                                  outside 1..65535, range not positive, a non-finite value):
+                                  the problem is not planned, the others of a batch are */
 } mrpop_result_info;
 
 typedef struct mrpop_solution mrpop_solution;
@@ -228,7 +232,10 @@ int mrpop_solve(mrpop_ctx *ctx, const mrpop_problem *problem, const mrpop_settin
                 const mrpop_solve_options *options, mrpop_solution **out);
 /* n problems planned together (one round of every active problem per step;
  * problems are admitted while their device memory fits).  settings and
- * options are arrays of n (or NULL for defaults); out receives n solutions. */
+ * options are arrays of n (or NULL for defaults); out receives n solutions.
+ * Returns MRPOP_ERR_INVALID for a malformed problem (null pointers, arm
+ * count outside 2..5, no goal); a problem rejected on admission gets a
+ * solution with status MRPOP_ERR_INVALID instead. */
 int mrpop_solve_batch(mrpop_ctx *ctx, int n, const mrpop_problem *problems, const mrpop_settings *settings,
                       const mrpop_solve_options *options, mrpop_solution **out);
 

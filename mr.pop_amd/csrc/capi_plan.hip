@@ -122,18 +122,24 @@ struct mrpop_server {
         while (!waiting.empty()) {
             if (max_active > 0 && (int)active.size() >= max_active) break;
             Request *r = waiting.front();
-            const size_t need = engine->slot_bytes(r->n_arms, r->st, r->opt);
-            if (engine->device_bytes() + need > budget) {
-                if (active.empty())
-                    return ctx->fail(MRPOP_ERR_BUDGET, "one problem needs %zu bytes, budget %zu", need, budget);
-                break;
-            }
             mrpop_scene sc{(int)(r->spheres.size() / 4), (int)(r->capsules.size() / 8), (int)(r->zcaps.size() / 8),
                            (int)(r->cuboids.size() / 15), (int)(r->zcubs.size() / 15),
                            r->spheres.data(), r->capsules.data(), r->zcaps.data(), r->cuboids.data(), r->zcubs.data()};
+            const float *bases = r->bases.empty() ? nullptr : r->bases.data();
+            // rejected settings or values end this problem only, before they size anything
+            int rc = engine->validate(sc, r->n_arms, bases, r->start.data(), r->n_goals, r->goals.data(), r->st, r->opt);
+            if (!rc) {
+                const size_t need = engine->slot_bytes(r->n_arms, r->st, r->opt);
+                if (engine->device_bytes() + need > budget) {
+                    if (active.empty())
+                        return ctx->fail(MRPOP_ERR_BUDGET, "one problem needs %zu bytes, budget %zu", need, budget);
+                    break;
+                }
+            }
             Slot *s = nullptr;
-            const int rc = engine->add(sc, r->n_arms, r->bases.empty() ? nullptr : r->bases.data(), r->start.data(),
-                                       r->n_goals, r->goals.data(), r->seed, r->st, r->opt, &s);
+            if (!rc)
+                rc = engine->add(sc, r->n_arms, bases, r->start.data(), r->n_goals, r->goals.data(), r->seed, r->st,
+                                 r->opt, &s);
             if (rc) {
                 if (rc == MRPOP_ERR_BUDGET && !active.empty()) break;  // wait for memory to free up
                 waiting.pop_front();

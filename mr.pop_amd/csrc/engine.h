@@ -111,6 +111,11 @@ class Engine {
     int add(const mrpop_scene &scene, int n_arms, const float *bases, const float *start, int n_goals,
             const float *goals, unsigned long long seed, const mrpop_settings &st, const SolveOptions &opt,
             Slot **out);
+    // The argument checks of add, without allocating: MRPOP_OK or
+    // MRPOP_ERR_INVALID (message in the context).  Sizes from slot_bytes are
+    // meaningful only for arguments that pass.
+    int validate(const mrpop_scene &scene, int n_arms, const float *bases, const float *start, int n_goals,
+                 const float *goals, const mrpop_settings &st, const SolveOptions &opt) const;
     // Release a slot's device memory (waits for its lane if it is running).
     void release(Slot *s);
     // Launch work on idle lanes, wait for one lane and advance its slots.

@@ -193,10 +193,8 @@ size_t Engine::slot_bytes(int A, const mrpop_settings &st, const SolveOptions &o
     return L.total;
 }
 
-int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float *start, int n_goals,
-                const float *goals, unsigned long long seed, const mrpop_settings &st, const SolveOptions &opt,
-                Slot **out) {
-    *out = nullptr;
+int Engine::validate(const mrpop_scene &scene, int A, const float *bases, const float *start, int n_goals,
+                     const float *goals, const mrpop_settings &st, const SolveOptions &opt) const {
     if (A < 2 || A > geom::kMaxArms) return c_->fail(MRPOP_ERR_INVALID, "planning needs 2..5 arms (got %d)", A);
     if (n_goals < 1 || n_goals > kMaxGoals) return c_->fail(MRPOP_ERR_INVALID, "1..%d goals supported", kMaxGoals);
     if (st.granularity < 1 || st.granularity > 64) return c_->fail(MRPOP_ERR_INVALID, "granularity outside 1..64");
@@ -205,7 +203,8 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     if (!(st.range > 0)) return c_->fail(MRPOP_ERR_INVALID, "range must be positive");
     if (opt.block_per_robot < 1 || opt.roadmap_capacity < 1 + n_goals + 1 || opt.neighbour_capacity < 1 ||
         opt.neighbour_candidates < 1 || opt.neighbour_candidates > 64 || opt.rounds_per_step < 1 ||
-        opt.rounds_per_step > 1024 || opt.probe_samples < 0 || opt.probe_samples > 64)
+        opt.rounds_per_step > 1024 || opt.tree_capacity < 0 ||
+        (opt.tree_capacity == 0 && st.max_samples < 2) || opt.probe_samples < 0 || opt.probe_samples > 64)
         return c_->fail(MRPOP_ERR_INVALID,
                         "bad solve options (neighbour_candidates 1..64, rounds_per_step 1..1024, probe_samples 0..64)");
     const int counts[5] = {scene.n_spheres, scene.n_capsules, scene.n_z_capsules, scene.n_cuboids, scene.n_z_cuboids};
@@ -215,6 +214,23 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
         n_obs += counts[k];
     }
     if (n_obs > MRPOP_MAX_OBSTACLES) return c_->fail(MRPOP_ERR_INVALID, "too many obstacles");
+    // a NaN configuration would compare false against every distance and leave
+    // nearest-node searches without a result
+    auto finite = [](const float *v, long n) {
+        for (long i = 0; i < n; ++i)
+            if (!std::isfinite(v[i])) return false;
+        return true;
+    };
+    if (!finite(start, 7L * A) || !finite(goals, 7L * A * n_goals) || (bases && !finite(bases, 12L * A)))
+        return c_->fail(MRPOP_ERR_INVALID, "start, goals and base poses must be finite");
+    return MRPOP_OK;
+}
+
+int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float *start, int n_goals,
+                const float *goals, unsigned long long seed, const mrpop_settings &st, const SolveOptions &opt,
+                Slot **out) {
+    *out = nullptr;
+    if (int rc = validate(scene, A, bases, start, n_goals, goals, st, opt)) return rc;
     const float *b = bases ? bases : &geom::default_bases(A)[0][0];
 
     auto *s = new Slot();

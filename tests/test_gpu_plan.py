@@ -282,3 +282,33 @@ def test_invalid_problems(ctx):
     nog = dataclasses.replace(p, goals=p.goals[:0])
     with pytest.raises(_lib.MrpopError):
         _lib.solve(ctx, nog)
+
+
+def test_invalid_problems_are_rejected(ctx):
+    """Malformed problems fail the call; problems whose settings or values
+    are rejected on admission come back unplanned with status ERR_INVALID
+    while the rest of the batch is planned as usual."""
+    p = PROBLEMS[0]
+    with pytest.raises(_lib.MrpopError) as e:
+        _lib.solve(ctx, dataclasses.replace(p, n_arms=1, start=p.start[:7], goals=p.goals[:, :7], bases=p.bases[:1]))
+    assert e.value.code == _lib.ERR_INVALID
+    nan_start = p.start.copy()
+    nan_start[3] = np.nan
+    inf_goal = p.goals.copy()
+    inf_goal[0, 9] = np.inf
+    ok = (_settings(), _options())
+    # rejected before any size is computed from them: negative counts would
+    # otherwise read as a slot larger than the device
+    bad = [(p, _settings(granularity=0), ok[1]), (p, _settings(granularity=65), ok[1]),
+           (p, _settings(num_new_configs=0), ok[1]), (p, _settings(num_new_configs=-5), ok[1]),
+           (p, _settings(range=0.0), ok[1]), (p, _settings(range=float("nan")), ok[1]),
+           (p, ok[0], _options(tree_capacity=-1)), (p, ok[0], _options(neighbour_candidates=65)),
+           (dataclasses.replace(p, start=nan_start), *ok), (dataclasses.replace(p, goals=inf_goal), *ok)]
+    cases = [(p, *ok)] + bad + [(PROBLEMS[3], *ok)]
+    probs = [c[0] for c in cases]
+    sols = _lib.solve_batch(ctx, probs, [c[1] for c in cases], [c[2] for c in cases])
+    for q, sol in zip(probs[1:-1], sols[1:-1]):
+        assert sol.info["status"] == _lib.ERR_INVALID and not sol.solved and len(sol.path) == 0
+    for q, sol in (probs[0], sols[0]), (probs[-1], sols[-1]):
+        assert sol.info["status"] == _lib.OK and sol.solved
+        check_solution(q, sol)
