@@ -223,6 +223,10 @@ PlannerResult<Robot> solve(typename Robot::Configuration &start, std::vector<typ
     if (rc != MRPOP_OK) return fail("mrpop_solve", rc, sc.ctx);
     mrpop_result_info info{};
     mrpop_solution_info(sol, &info);
+    if (info.status != MRPOP_OK && info.status != MRPOP_ERR_TIMEOUT) {  // settings or values rejected
+        mrpop_solution_free(sol);
+        return fail("mrpop_solve", info.status, sc.ctx);
+    }
     result.solved = info.solved != 0;
     result.cost = info.cost;
     result.start_tree_size = info.start_tree_size;

@@ -77,3 +77,17 @@ def test_drop_in_solve_matches_the_c_abi(tmp_path, ctx):
     assert np.array_equal(ref.path, path)
     assert cost == pytest.approx(ref.cost, rel=1e-6)
     check_solution(p, ref, granularity=G, rng=rng)
+
+
+@pytest.mark.gpu
+def test_drop_in_reports_rejected_settings(tmp_path):
+    """Settings the planner rejects reach the C++ caller as an unsolved
+    result with the reason printed, as the drop-in's other errors are."""
+    probs = scenes.load_problem_file(str(REPO / "tests/golden/panda_four_problems.json"), geometry.default_bases(4))
+    f = tmp_path / "p.txt"
+    write_problem(f, probs[0], 300, 0, 2.0, 2, 500)  # granularity 0
+    print("[dropin] solve_problem granularity 0", flush=True)
+    out = subprocess.run([_exe(), str(f)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 3, out.stdout[-2000:] + out.stderr[-2000:]
+    assert "granularity outside 1..64" in out.stderr
+    assert parse_output(out.stdout, probs[0].dim)[0] == 0
