@@ -939,13 +939,16 @@ __global__ __launch_bounds__(256) void k_dr_finish(SlotDev *slots, const int *id
         return;
     }
     float cost = 0.f;
-    // tree 1 reversed first: its root (node 0 of tree 1) ... its chain start
+    // tree 1 reversed first: its root (the goal reached) ... its chain start.
+    // The reference writes node 0 of each tree as the root
+    // (path_processing.cu:31-33, 48-50); for the start tree and a single goal
+    // that is the root, with several goals it would be goal 0 whichever goal
+    // the chain reached, so the root reached is written instead.
     {
         int cur = start_node[1], k = chain_n[1] - 1;
         while (true) {
             const int par = s.tr_parent[(size_t)s.tr_cap + cur];
-            const int node = par == cur ? 0 : cur;  // the root written is node 0 of the tree
-            const float *src = tree_q(s, 1, node);
+            const float *src = tree_q(s, 1, cur);
             for (int d = 0; d < D; ++d) s.path[(size_t)k * D + d] = src[d];
             if (par == cur) break;
             cost += sqrtf(sqdist(tree_q(s, 1, cur), tree_q(s, 1, par), D));
@@ -957,8 +960,7 @@ __global__ __launch_bounds__(256) void k_dr_finish(SlotDev *slots, const int *id
         int cur = start_node[0], k = chain_n[1];
         while (true) {
             const int par = s.tr_parent[cur];
-            const int node = par == cur ? 0 : cur;
-            const float *src = tree_q(s, 0, node);
+            const float *src = tree_q(s, 0, cur);  // the start tree's only root is node 0
             for (int d = 0; d < D; ++d) s.path[(size_t)k * D + d] = src[d];
             if (par == cur) break;
             cost += sqrtf(sqdist(tree_q(s, 0, cur), tree_q(s, 0, par), D));

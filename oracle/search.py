@@ -246,8 +246,11 @@ def connect_plan(s: Slot, t: int, q: np.ndarray, ce_per_b: int, ub=None):
 
 def record_path(tr_q, tr_parent, t: int, wnode: int, onode: int, path_cap: int):
     """Goal -> start path through the connection (path_processing.cu:10-70):
-    tree 1's chain reversed (its root written as node 0), then tree 0's chain
-    to its root; cost = float32 sum of segment lengths, tree 1 first."""
+    tree 1's chain reversed, then tree 0's chain to its root; cost = float32
+    sum of segment lengths, tree 1 first.  The reference writes node 0 of each
+    tree as the root (path_processing.cu:31-33, 48-50): the same node for the
+    start tree and for a single goal; with several goals the goal reached is
+    written here, not goal 0."""
     start = {t: wnode, 1 - t: onode}
     chains = []
     for tr in (0, 1):
@@ -261,8 +264,5 @@ def record_path(tr_q, tr_parent, t: int, wnode: int, onode: int, path_cap: int):
         ch = chains[tr]
         for a, b in zip(ch[:-1], ch[1:]):
             cost = np.float32(cost + sqrt32(sqdist(tr_q[tr, a][None], tr_q[tr, b])[0]))
-    roots = lambda tr, ch: [0 if i == len(ch) - 1 else n for i, n in enumerate(ch)]
-    nodes1 = roots(1, chains[1])[::-1]
-    nodes0 = roots(0, chains[0])
-    path = np.concatenate([tr_q[1, nodes1], tr_q[0, nodes0]])
+    path = np.concatenate([tr_q[1, chains[1][::-1]], tr_q[0, chains[0]]])
     return path, cost

@@ -84,6 +84,14 @@ def segment_verdicts(problem, path, granularity, rng):
     return [any(not hit[(i, v)] for v in range(3)) for i in range(len(path) - 1)]
 
 
+def interior_waypoint(path):
+    """The middle waypoint of a path among those equal to neither end."""
+    inner = [w for w in path if not (np.array_equal(w, path[0]) or np.array_equal(w, path[-1]))]
+    if not inner:
+        pytest.skip("the path has no interior waypoint")
+    return inner[len(inner) // 2]
+
+
 def check_solution(problem, sol, granularity=SETTINGS["granularity"], rng=SETTINGS["range"]):
     assert sol.solved, f"{problem.name}: not solved ({sol.info})"
     path = sol.path
@@ -312,3 +320,26 @@ def test_invalid_problems_are_rejected(ctx):
     for q, sol in (probs[0], sols[0]), (probs[-1], sols[-1]):
         assert sol.info["status"] == _lib.OK and sol.solved
         check_solution(q, sol)
+
+
+def test_several_goals(ctx):
+    """Problems with several goals: every returned path starts at one of them
+    (the one its goal-tree chain reached), costs its length and is free.  In
+    the second set goal 1 is a waypoint halfway along a solved path, closer
+    than goal 0, so paths end at a goal other than the first."""
+    checker = _lib.gpu_checker(ctx)
+    probs = [scenes.make_problem(kind, seed, checker, n_goals=3)
+             for kind, seed in (("table3", 21), ("bin4", 22), ("shelf5", 23), ("plain2", 24))]
+    first = _lib.solve_batch(ctx, probs, [_settings()] * len(probs), [_options()] * len(probs))
+    near = []
+    for p, sol in zip(probs, first):
+        check_solution(p, sol)
+        mid = interior_waypoint(sol.path)
+        near.append(dataclasses.replace(p, goals=np.stack([p.goals[0], mid, p.goals[2]])))
+    sols = _lib.solve_batch(ctx, near, [_settings()] * len(near), [_options()] * len(near))
+    reached = []
+    for p, sol in zip(near, sols):
+        check_solution(p, sol)
+        reached.append(int(np.argmin([np.abs(sol.path[0] - g).max() for g in p.goals])))
+    print(f"[goals] goal reached per problem: {reached}", flush=True)
+    assert any(r != 0 for r in reached)

@@ -18,6 +18,7 @@ either way) and then drive the restated commit.
 CPU tests pin fma32, the exact float32 fused multiply-add the restatement
 relies on."""
 import ctypes
+import dataclasses
 from fractions import Fraction
 
 import numpy as np
@@ -26,6 +27,7 @@ import pytest
 from mrpop import _lib, geometry, scenes
 from oracle import oracle as O
 from oracle import search as SR
+from test_gpu_plan import interior_waypoint
 
 
 def _round_half_even_f32(x: Fraction) -> np.float32:
@@ -357,11 +359,16 @@ def _check_round(p, ds, before, after, settings, optimize=False, phs=None):
         path, cost = SR.record_path(tr_q, tr_parent, t, wnode, onode, ds.path_cap)
         assert after["path_n"] == len(path)
         assert np.array_equal(after["path"][:len(path)], path), "recorded path"
+        # the path starts at the goal its chain reached (a root of tree 1) and ends at the start
+        reached = [k for k, g in enumerate(p.goals) if np.array_equal(path[0], g)]
+        assert reached and np.array_equal(path[-1], p.start)
+        length = sum(float(np.linalg.norm(a.astype(np.float64) - q)) for a, q in zip(path[:-1], path[1:]))
+        assert abs(length - cost) <= 1e-5 * length, "cost is the path's length (float32 sums in chain order)"
         # the best cost only falls (an optimise pass may record a dearer path)
         assert after["path_cost"] == cost and after["best"] == min(np.float32(best), cost)
     return dict(candidates=len(cand), edges=n_edges, inserted=len(index), connections=len(plans),
-                rejected=rejected, rewired=len(rewires),
-                solved=winner is not None, exempt=exempt, n_commit=n_commit)
+                rejected=rejected, rewired=len(rewires), solved=winner is not None,
+                goal=reached[0] if winner is not None else None, exempt=exempt, n_commit=n_commit)
 
 
 @pytest.mark.gpu
@@ -487,7 +494,34 @@ def test_rounds_and_simplification_sweep(ctx, kind, seed):
     """More problems through the same restatements: first-pass rounds until
     solved, the simplification stages of the recorded path, then optimise
     rounds (up to 30 rounds in all)."""
+    _sweep(ctx, scenes.make_problem(kind, seed, _lib.gpu_checker(ctx)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,seed", [("table3", 5), ("bin4", 6), ("shelf5", 7)])
+def test_multi_goal_rounds_match_the_restatement(ctx, kind, seed):
+    """Three goals: the goal tree starts with three roots, and a recorded
+    path begins at the goal its chain reached."""
+    _sweep(ctx, scenes.make_problem(kind, seed, _lib.gpu_checker(ctx), n_goals=3))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,seed", [("table3", 2), ("shelf5", 3)])
+def test_rounds_reaching_a_later_goal_match_the_restatement(ctx, kind, seed):
+    """Goal 1 is a free waypoint halfway along a solved path (closer than
+    goal 0): the rounds that solve record paths from the goal reached."""
     p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
+    st = _lib.settings(num_new_configs=64, granularity=16, range=2.0, path_simplify=0, rrtc_iter=1)
+    sol = _lib.solve(ctx, p, st, _lib.solve_options_default(**_SMALL_OPT))
+    assert sol.solved
+    mid = interior_waypoint(sol.path)  # a tree node: the path is not simplified
+    goals = [s["goal"] for s in _sweep(ctx, dataclasses.replace(p, goals=np.stack([p.goals[0], mid])))
+             if s["goal"] is not None]
+    print(f"[search] {kind}: goals reached {goals}", flush=True)
+    assert 1 in goals
+
+
+def _sweep(ctx, p):
     st = _lib.settings(num_new_configs=64, granularity=16, range=2.0)
     ds = _DiagSlot(ctx, p, st, _lib.solve_options_default(**_SMALL_OPT))
     try:
@@ -497,6 +531,7 @@ def test_rounds_and_simplification_sweep(ctx, kind, seed):
             if (deg[:, :2] > 0).all():
                 break
         keep, opt, phs = False, False, None
+        stats = []
         for _ in range(30):
             before = ds.state()
             if before["status"] != 0:
@@ -511,9 +546,10 @@ def test_rounds_and_simplification_sweep(ctx, kind, seed):
                 before["round"] = 0
             ds.search(1, keep)
             keep = True
-            _check_round(p, ds, before, ds.state(), st, optimize=opt, phs=phs)
+            stats.append(_check_round(p, ds, before, ds.state(), st, optimize=opt, phs=phs))
     finally:
         ds.close()
+    return stats
 
 
 @pytest.mark.gpu
