@@ -70,16 +70,17 @@ def _dump(ctx, p, st, op, rounds, arm, cap, K):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind,seed,arm", [("plain2", 4, 0), ("bin4", 1, 2), ("table3", 3, 0), ("bin4", 9, 3),
-                                           ("shelf5", 2, 4), ("shelf5", 6, 1)])
-def test_roadmap_rounds_match_the_restatement(ctx, kind, seed, arm):
-    p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
+@pytest.mark.parametrize("kind,seed,arm,n_goals", [("plain2", 4, 0, 1), ("bin4", 1, 2, 1), ("table3", 3, 0, 1),
+                                                   ("bin4", 9, 3, 1), ("shelf5", 2, 4, 1), ("shelf5", 6, 1, 1),
+                                                   ("table3", 4, 1, 3), ("shelf5", 8, 2, 4)])
+def test_roadmap_rounds_match_the_restatement(ctx, kind, seed, arm, n_goals):
+    p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx), n_goals=n_goals)
     G, rng, kc, K, bpr, rounds = 8, 2.0, 4, 6, 1, 3
     M = 230 * bpr
     st = _lib.settings(granularity=G, range=rng)
     op = _lib.solve_options_default(block_per_robot=bpr, neighbour_candidates=kc, neighbour_capacity=K,
                                     roadmap_capacity=4096)
-    print(f"[roadmap] {kind} seed {seed} arm {arm}", flush=True)
+    print(f"[roadmap] {kind} seed {seed} arm {arm} goals {n_goals}", flush=True)
     q, deg, nbr = _dump(ctx, p, st, op, rounds, arm, 4096, K)
     scale, low = geometry.table("kScale"), geometry.table("kLow")
     # nodes 0 .. n_goals: start and goal slices of the arm
