@@ -320,6 +320,14 @@ def test_invalid_problems_are_rejected(ctx):
     for q, sol in (probs[0], sols[0]), (probs[-1], sols[-1]):
         assert sol.info["status"] == _lib.OK and sol.solved
         check_solution(q, sol)
+    # the server reports a rejected request as done, unplanned, and holds no memory for it
+    srv = _lib.Server(ctx)
+    t = srv.submit(p, _settings(num_new_configs=-5), _options())
+    assert srv.step() == (0, 0)
+    st, sol = srv.poll(t, snapshot=True)
+    assert st == 2 and sol.info["status"] == _lib.ERR_INVALID and not sol.solved
+    assert srv.bytes() == 0
+    srv.close()
 
 
 def test_several_goals(ctx):
