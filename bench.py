@@ -137,6 +137,7 @@ def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_r
         tuned["probe_samples"] = batch_probe
     if tune:
         calib = plan_problems(checker, 16, first_seed=1000)
+        print(f"[bench] autotune on {len(calib)} calibration problems", file=sys.stderr, flush=True)
         res = autotune.tune(ctx, calib, common, op_common)
         tuned = res.best
         out["autotune"] = res.as_dict()
@@ -359,7 +360,9 @@ def main():
                      "kernel": "check_kernel<4, FULL> (fused coarse + fine edge check)",
                      "ms_per_launch": ms_events, "flops_per_sample": w_sample},
     }
+    print(f"[bench] edge check: {edges_per_s:.4g} edges/s, {ms_events:.3f} ms per launch", file=sys.stderr, flush=True)
     if not args.no_cpu:
+        print("[bench] cpu baseline (bounded sample)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(problem, frm, to, G)
     if not args.no_plan:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
