@@ -82,6 +82,23 @@ def test_sqdist_accumulates_with_fma():
         assert d[i] == s
 
 
+def test_record_path_on_a_two_goal_tree():
+    """Known answer on hand-built trees (one dimension, so lengths are
+    differences): tree 1 has goal roots 0 and 1 and the chain 3 -> 2 -> 1;
+    tree 0 the chain 2 -> 1 -> 0.  The path runs from goal 1 (the root the
+    chain reached, not goal 0) through the connection to the start."""
+    tr_q = np.zeros((2, 4, 1), np.float32)
+    tr_q[1, :, 0] = [10.0, 9.0, 7.0, 4.0]    # goals 10 and 9; nodes 7 and 4
+    tr_q[0, :, 0] = [0.0, 1.0, 3.0, 0.0]     # start 0; nodes 1 and 3
+    tr_parent = np.array([[0, 0, 1, 0], [0, 1, 1, 2]])
+    for t, wnode, onode in ((1, 3, 2), (0, 2, 3)):  # the same connection seen from either tree
+        path, cost = SR.record_path(tr_q, tr_parent, t, wnode, onode, 16)
+        assert path[:, 0].tolist() == [9.0, 7.0, 4.0, 3.0, 1.0, 0.0]
+        # tree 1's segments, then tree 0's; the joining segment 4 -> 3 is not
+        # counted (in the planner its two ends are the same configuration)
+        assert cost == np.float32(2 + 3 + 2 + 1)
+
+
 # ---------------------------------------------------------------- GPU ----
 
 class _DiagSlot:
