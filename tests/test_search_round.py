@@ -16,7 +16,7 @@ device used are checked against the FP64 oracle (near-contact edges may go
 either way) and then drive the restated commit.
 
 CPU tests pin fma32, the exact float32 fused multiply-add the restatement
-relies on."""
+relies on, and the path record on a hand-built two-goal tree."""
 import ctypes
 import dataclasses
 from fractions import Fraction
