@@ -34,6 +34,7 @@ extern "C" {
 #define MRPOP_ERR_TIMEOUT 4    /* wall-clock budget ran out                 */
 #define MRPOP_ERR_NO_DEVICE 5  /* no usable GPU                             */
 #define MRPOP_ERR_STATE 6      /* call not valid in the object's state      */
+#define MRPOP_ERR_INTERNAL 7   /* a device-side step bound was reached (union-find walk); problem ended */
 
 /* ---- collision modes (which reference checker an edge check follows) ---- */
 /* Multi-arm, environment + self + arm-arm: fkcc_single_buffer (utils.cuh:447-534). */
@@ -221,7 +222,15 @@ typedef struct mrpop_result_info {
                                   MRPOP_ERR_INVALID if its settings, options or configurations
                                   were rejected (granularity outside 1..64, num_new_configs
                                   outside 1..65535, range not positive, a non-finite value):
-                                  the problem is not planned, the others of a batch are */
+                                  the problem is not planned, the others of a batch are;
+                                  MRPOP_ERR_BUDGET if it needs more device memory than the
+                                  whole HBM budget (refused alone, the queue goes on);
+                                  MRPOP_ERR_INTERNAL if union_find_bound below is set */
+    /* Bounded work cut short during the solve (counts; none affects a returned path's validity): */
+    int union_find_bound;      /* union-find walks/unions that reached their step bound (ends the problem) */
+    int edge_table_full;       /* expansions whose duplicate-edge lookup found the table full (checked alone) */
+    int connect_skipped;       /* connections longer than 64 chunks per expansion, not attempted */
+    int simplify_dropped;      /* simplification candidates beyond the chunk staging, kept as colliding */
 } mrpop_result_info;
 
 typedef struct mrpop_solution mrpop_solution;
@@ -235,7 +244,8 @@ int mrpop_solve(mrpop_ctx *ctx, const mrpop_problem *problem, const mrpop_settin
  * options are arrays of n (or NULL for defaults); out receives n solutions.
  * Returns MRPOP_ERR_INVALID for a malformed problem (null pointers, arm
  * count outside 2..5, no goal); a problem rejected on admission gets a
- * solution with status MRPOP_ERR_INVALID instead. */
+ * solution with status MRPOP_ERR_INVALID (or MRPOP_ERR_BUDGET when it alone
+ * exceeds the device memory budget) instead, and the others are planned. */
 int mrpop_solve_batch(mrpop_ctx *ctx, int n, const mrpop_problem *problems, const mrpop_settings *settings,
                       const mrpop_solve_options *options, mrpop_solution **out);
 

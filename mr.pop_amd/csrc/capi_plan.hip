@@ -130,11 +130,15 @@ struct mrpop_server {
             int rc = engine->validate(sc, r->n_arms, bases, r->start.data(), r->n_goals, r->goals.data(), r->st, r->opt);
             if (!rc) {
                 const size_t need = engine->slot_bytes(r->n_arms, r->st, r->opt);
-                if (engine->device_bytes() + need > budget) {
-                    if (active.empty())
-                        return ctx->fail(MRPOP_ERR_BUDGET, "one problem needs %zu bytes, budget %zu", need, budget);
-                    break;
+                if (need > budget) {
+                    // can never fit: refuse this request alone and go on with the queue
+                    rc = ctx->fail(MRPOP_ERR_BUDGET, "one problem needs %zu bytes, budget %zu", need, budget);
+                    waiting.pop_front();
+                    r->state = 2;
+                    r->done_result.status = rc;
+                    continue;
                 }
+                if (engine->device_bytes() + need > budget) break;  // wait for memory to free up
             }
             Slot *s = nullptr;
             if (!rc)
@@ -279,6 +283,10 @@ int mrpop_solution_info(const mrpop_solution *s, mrpop_result_info *info) {
     info->roadmap_rounds = r.roadmap_rounds;
     for (int a = 0; a < MRPOP_MAX_ARMS; ++a) info->roadmap_size[a] = r.roadmap_size[a];
     info->status = r.status;
+    info->union_find_bound = r.events[kEvUnionFind];
+    info->edge_table_full = r.events[kEvEdgeTable];
+    info->connect_skipped = r.events[kEvConnectSkip];
+    info->simplify_dropped = r.events[kEvSimplifyDrop];
     return MRPOP_OK;
 }
 
@@ -377,11 +385,12 @@ size_t mrpop_server_bytes(const mrpop_server *srv) { return srv ? srv->engine->d
 
 // Diagnostics (not part of the public ABI): build the roadmaps of a problem
 // for `rounds` roadmap rounds and copy arm `arm`'s roadmap out (nodes as
-// [n][8] floats, degrees, neighbour lists [n][neighbour_capacity]); *n gets
-// the node count, at most `cap` nodes are copied.
+// [n][8] floats, degrees, neighbour lists [n][neighbour_capacity], union-find
+// parents [n] and the arm's start-goal connectivity flag; any output may be
+// NULL); *n gets the node count, at most `cap` nodes are copied.
 extern "C" int mrpop_diag_roadmap(mrpop_ctx *c, const mrpop_problem *p, const mrpop_settings *settings,
                                   const mrpop_solve_options *options, int rounds, int arm, float *q, int *deg,
-                                  int *nbr, int *n, int cap) {
+                                  int *nbr, int *uf, int *connected, int *n, int cap) {
     if (!c || !p || !n || rounds < 0) return MRPOP_ERR_INVALID;
     Context *ctx = (Context *)c;
     if (hipSetDevice(ctx->device) != hipSuccess) return MRPOP_ERR_HIP;
@@ -399,7 +408,34 @@ extern "C" int mrpop_diag_roadmap(mrpop_ctx *c, const mrpop_problem *p, const mr
                             r.goals.data(), r.seed, st, to_options(options), &s))
         return rc;
     if (int rc = engine.debug_roadmap(s, rounds)) return rc;
-    return engine.debug_copy_roadmap(s, arm, q, deg, nbr, n, cap);
+    return engine.debug_copy_roadmap(s, arm, q, deg, nbr, uf, connected, n, cap);
+}
+
+// Diagnostics (not part of the public ABI): device bytes one problem of
+// n_arms arms holds with these settings and options (what the server's HBM
+// budget admits against); 0 if they are rejected.
+extern "C" size_t mrpop_diag_slot_bytes(mrpop_ctx *c, int n_arms, const mrpop_settings *settings,
+                                        const mrpop_solve_options *options) {
+    if (!c || n_arms < 2 || n_arms > MRPOP_MAX_ARMS) return 0;
+    mrpop_settings st;
+    if (settings) st = *settings;
+    else mrpop_settings_default(&st);
+    Engine engine((Context *)c);
+    const SolveOptions opt = to_options(options);
+    const mrpop_scene none{};
+    std::vector<float> cfg((size_t)7 * n_arms, 0.f);
+    if (engine.validate(none, n_arms, nullptr, cfg.data(), 1, cfg.data(), st, opt)) return 0;
+    return engine.slot_bytes(n_arms, st, opt);
+}
+
+// Diagnostics (not part of the public ABI): the hyperspheroid transform an
+// optimise pass samples with, for foci f1, f2 (D floats each) and best cost
+// cb: tf (D x D row-major) and centre (D).  Host code: needs no device.
+extern "C" int mrpop_diag_phs_transform(int D, const float *f1, const float *f2, double cb, float *tf,
+                                        float *center) {
+    if (D < 1 || D > 35 || !f1 || !f2 || !tf || !center) return MRPOP_ERR_INVALID;
+    phs_transform(D, f1, f2, cb, tf, center);
+    return MRPOP_OK;
 }
 
 // Diagnostics (not part of the public ABI): the planner's counter-based

@@ -50,7 +50,8 @@ struct SlotResult {
     std::vector<uint64_t> improve_ns;  // submit -> each improving solution
     int passes = 0, rounds = 0, roadmap_rounds = 0;
     int roadmap_size[geom::kMaxArms] = {};
-    int status = 0;                // MRPOP_OK, MRPOP_ERR_TIMEOUT, MRPOP_ERR_BUDGET
+    int status = 0;                // MRPOP_OK, MRPOP_ERR_TIMEOUT, MRPOP_ERR_BUDGET, MRPOP_ERR_INTERNAL
+    int events[kEvents] = {};      // device event counters (kEv*) at the end of the solve
 };
 
 enum class Phase { Roadmap, Search, Simplify, Done };
@@ -73,6 +74,7 @@ struct Slot {
     Phase phase = Phase::Roadmap;
     int pass = 0;
     bool rm_until_connected = true;
+    bool seeking = true;   // the pass looks for a first solution (no optimise mode, no hyperspheroid)
     bool pass_solved = false;
     int dr_round = 0, dr_budget = 0;
     int sp_stage = 0;
@@ -95,8 +97,14 @@ struct SlotReport {
     int path_meta[3];
     int sp_count;
     int ctr[4];  // device round counters (kCtr*)
+    int events[kEvents];  // device event counters (kEv*)
 };
 constexpr int plan_ctr_round = kCtrRound, plan_ctr_run = kCtrRun;
+
+// Prolate-hyperspheroid sampling transform of an optimise pass
+// (pop_planner.cu:289-346): tf (D x D, row-major) maps the unit ball onto the
+// set {x : |x - f1| + |x - f2| <= cb}, centred at center = (f1 + f2) / 2.
+void phs_transform(int D, const float *f1, const float *f2, double cb, float *tf, float *center);
 
 // Work runs on two lanes, each a HIP stream with its own staging: the
 // roadmap lane grows the per-arm roadmaps of slots in the Roadmap phase, the
@@ -130,7 +138,10 @@ class Engine {
     int debug_search(Slot *s, int rounds, bool keep_round);
     int debug_simplify(Slot *s, int stage);
     int debug_start_pass(Slot *s, int pass);
-    int debug_copy_roadmap(Slot *s, int arm, float *q, int *deg, int *nbr, int *n, int cap);
+    // uf (optional): union-find parents [n]; connected (optional): the arm's
+    // start-goal connectivity flag as the device last computed it.
+    int debug_copy_roadmap(Slot *s, int arm, float *q, int *deg, int *nbr, int *uf, int *connected, int *n,
+                           int cap);
 
   private:
     struct Staged {

@@ -207,6 +207,11 @@ int Engine::validate(const mrpop_scene &scene, int A, const float *bases, const 
         (opt.tree_capacity == 0 && st.max_samples < 2) || opt.probe_samples < 0 || opt.probe_samples > 64)
         return c_->fail(MRPOP_ERR_INVALID,
                         "bad solve options (neighbour_candidates 1..64, rounds_per_step 1..1024, probe_samples 0..64)");
+    // the goal tree starts with one root per goal and the start tree with one
+    // node; a capacity below n_goals + 1 would not hold the roots
+    const long tr_cap = opt.tree_capacity > 0 ? opt.tree_capacity : std::min(st.max_samples, 262144);
+    if (tr_cap < n_goals + 1)
+        return c_->fail(MRPOP_ERR_INVALID, "tree capacity %ld below n_goals + 1 = %d", tr_cap, n_goals + 1);
     const int counts[5] = {scene.n_spheres, scene.n_capsules, scene.n_z_capsules, scene.n_cuboids, scene.n_z_cuboids};
     long n_obs = 0;
     for (int k = 0; k < 5; ++k) {
@@ -342,6 +347,7 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     v.path_meta = ctr + 8;
     v.sp_count = ctr + 12;
     v.ctr = ctr + 16;
+    v.events = ctr + 24;
     int *ex = (int *)take(sizeof(int) * B * 6 + sizeof(float) * B);
     v.ex_tree = ex;
     v.ex_state = ex + B;
@@ -481,12 +487,11 @@ void Engine::release(Slot *s) {
     delete s;
 }
 
-// PHS transform of the slot (pop_planner.cu:289-346): Householder rotation
-// taking e_1 to the unit start->goal direction, scaled by the hyperspheroid
-// radii (c_best / 2, sqrt(c_best^2 - c_min^2) / 2, ...).
-void Engine::compute_phs(Slot *s) {
-    const int D = s->D;
-    const float *f1 = s->sg.data(), *f2 = s->sg.data() + D;
+// PHS transform (pop_planner.cu:289-346): Householder rotation taking e_1 to
+// the unit start->goal direction, scaled by the hyperspheroid radii
+// (c_best / 2, sqrt(c_best^2 - c_min^2) / 2, ...); the centre is the
+// midpoint of the foci.  Host code only (no device call).
+void phs_transform(int D, const float *f1, const float *f2, double cb, float *tf, float *center) {
     double cmin = 0;
     for (int d = 0; d < D; ++d) cmin += (double)(f2[d] - f1[d]) * (f2[d] - f1[d]);
     cmin = std::sqrt(cmin);
@@ -505,23 +510,34 @@ void Engine::compute_phs(Slot *s) {
         for (int r = 0; r < D; ++r)
             for (int c = 0; c < D; ++c) rot[(size_t)r * D + c] = (r == c ? 1.0 : 0.0) - beta * vv[r] * vv[c];
     }
-    const double cb = s->best_cost;
     const double conj = std::sqrt(std::max(0.0, cb * cb - cmin * cmin));
-    std::vector<float> tf((size_t)D * D);
     for (int r = 0; r < D; ++r)
         for (int c = 0; c < D; ++c) tf[(size_t)r * D + c] = (float)(rot[(size_t)r * D + c] * (c == 0 ? cb / 2 : conj / 2));
-    for (int d = 0; d < D; ++d) s->dv.phs_center[d] = (f1[d] + f2[d]) / 2.0f;
+    for (int d = 0; d < D; ++d) center[d] = (f1[d] + f2[d]) / 2.0f;
+}
+
+// The slot's transform, from its start and goal 0 and its best cost.
+void Engine::compute_phs(Slot *s) {
+    const int D = s->D;
+    std::vector<float> tf((size_t)D * D);
+    phs_transform(D, s->sg.data(), s->sg.data() + D, s->best_cost, tf.data(), s->dv.phs_center);
     (void)hipMemcpy(s->dv.phs_tf, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice);
 }
 
+// A pass optimises (cost bounds, hyperspheroid samples) only once a solution
+// exists.  Pass 0 of the reference repeats until it is solved
+// (pop_planner.cu:967-986); here pass 0 can also end unsolved when a tree
+// fills up, and the next pass then searches for a first solution again
+// instead of optimising against an infinite cost bound.
 int Engine::start_pass(Slot *s) {
-    s->dv.optimize = s->pass > 0 ? 1 : 0;
+    s->seeking = !(s->best_cost < kFltMax);
+    s->dv.optimize = s->seeking ? 0 : 1;
     s->dv.pass = s->pass;
     s->pass_solved = false;
-    s->rm_until_connected = s->pass == 0;
+    s->rm_until_connected = s->seeking;
     s->phase = Phase::Roadmap;
     s->dr_round = 0;
-    s->dr_budget = s->pass == 0 ? s->st.max_iters : s->st.optimize_iters;
+    s->dr_budget = s->seeking ? s->st.max_iters : s->st.optimize_iters;
     if (s->dv.optimize && s->st.phs) compute_phs(s);
     return run_aux(s, plan::launch_tree_reset);
 }
@@ -831,7 +847,7 @@ int Engine::process_lane(Lane &L) {
                 // tree capacity reached: the pass ends unsolved
                 if (int rc = finish_pass(s, r)) return rc;
             } else if (s->dr_round >= s->dr_budget) {
-                if (s->pass == 0) {
+                if (s->seeking) {
                     s->phase = Phase::Roadmap;  // one more roadmap round, then a fresh dRRT launch
                 } else if (int rc = finish_pass(s, r)) {
                     return rc;
@@ -843,6 +859,14 @@ int Engine::process_lane(Lane &L) {
                 s->best_cost = r.best_cost;
                 if (int rc = finish_pass(s, r)) return rc;
             }
+        }
+        for (int k = 0; k < kEvents; ++k) s->res.events[k] = r.events[k];
+        if (s->phase != Phase::Done && r.events[kEvUnionFind] > 0) {
+            // roadmap connectivity can no longer be trusted: end the problem
+            s->phase = Phase::Done;
+            s->res.status = MRPOP_ERR_INTERNAL;
+            s->res.wall_ns = ns_since(s->t_submit);
+            s->res.kernel_ns = ns_since(s->t_kernel_start);
         }
         if (s->phase != Phase::Done && s->opt.time_budget_s > 0 &&
             std::chrono::duration<double>(now - s->t_submit).count() > s->opt.time_budget_s) {
@@ -993,7 +1017,8 @@ int Engine::debug_simplify(Slot *s, int stage) {
     return MRPOP_OK;
 }
 
-int Engine::debug_copy_roadmap(Slot *s, int arm, float *q, int *deg, int *nbr, int *n, int cap) {
+int Engine::debug_copy_roadmap(Slot *s, int arm, float *q, int *deg, int *nbr, int *uf, int *connected, int *n,
+                               int cap) {
     if (arm < 0 || arm >= s->A) return c_->fail(MRPOP_ERR_INVALID, "arm %d outside the problem's arms", arm);
     int count = 0;
     MRPOP_HIP(c_, hipMemcpy(&count, s->dv.rm_n + arm, sizeof(int), hipMemcpyDeviceToHost));
@@ -1005,6 +1030,8 @@ int Engine::debug_copy_roadmap(Slot *s, int arm, float *q, int *deg, int *nbr, i
     if (nbr && m)
         MRPOP_HIP(c_, hipMemcpy(nbr, s->dv.rm_nbr + off * s->dv.rm_K, sizeof(int) * (size_t)m * s->dv.rm_K,
                                 hipMemcpyDeviceToHost));
+    if (uf && m) MRPOP_HIP(c_, hipMemcpy(uf, s->dv.rm_uf + off, sizeof(int) * m, hipMemcpyDeviceToHost));
+    if (connected) MRPOP_HIP(c_, hipMemcpy(connected, s->dv.rm_connected + arm, sizeof(int), hipMemcpyDeviceToHost));
     return MRPOP_OK;
 }
 

@@ -37,6 +37,16 @@ enum : int {
     kCtrRun = 3,    // dRRT rounds run since the solve began
 };
 
+// Device event counters of a slot: bounded work that was cut short.  None
+// changes a returned path's validity; each is reported with the result.
+enum : int {
+    kEvUnionFind = 0,     // a union-find walk or union hit its step bound (connectivity may be incomplete)
+    kEvEdgeTable = 1,     // the round's duplicate-edge table was full: the expansion checked its own edge
+    kEvConnectSkip = 2,   // a connection needed more chunks than the budget per expansion: not attempted
+    kEvSimplifyDrop = 3,  // a simplification candidate did not fit the chunk staging: kept as colliding
+    kEvents = 4,
+};
+
 struct SlotDev {
     // problem
     int A, D, n_goals, world;
@@ -50,6 +60,7 @@ struct SlotDev {
     int round_reset;          // 1: k_dr_begin restarts the round count of a dRRT launch
     int round_limit;          // rounds of the current dRRT launch (max_iters / optimize_iters)
     int *ctr;                 // [4] device round counters, see kCtr*
+    int *events;              // [kEvents] event counters, see kEv*
     float phs_center[35];
     float *sg;                // [1 + n_goals][D]: start, then goals
     // roadmaps (arm-major)

@@ -255,9 +255,10 @@ __global__ __launch_bounds__(64) void k_rm_pack(SlotDev *slots, const int *ids) 
     s.re_coll[e] = 0u;  // the checker only sets bits
 }
 
-// Parents always have smaller ids than their children, so the walk ends; the
-// step cap only guards against corrupted input.
-__device__ __forceinline__ int uf_find(int *uf, int x) {
+// Parents always have smaller ids than their children, so the walk ends
+// within the node count; the step cap only guards against corrupted input
+// and is counted as an event when reached.
+__device__ __forceinline__ int uf_find(int *uf, int x, int *events) {
     for (int steps = 0; steps < (1 << 24); ++steps) {
         const int p = uf[x];
         if (p == x) return x;
@@ -265,15 +266,18 @@ __device__ __forceinline__ int uf_find(int *uf, int x) {
         if (gp != p) atomicCAS(&uf[x], p, gp);  // path halving
         x = p;
     }
+    atomicAdd(&events[kEvUnionFind], 1);
     return x;
 }
 
 // Lock-free union: the larger root is hooked under the smaller one, so each
 // component's root is its smallest node and the partition is deterministic.
-__device__ __forceinline__ void uf_union(int *uf, int a, int b) {
+// A failed CAS means another union of the same root succeeded, so the
+// retries are bounded by the round's edges; the cap is counted if reached.
+__device__ __forceinline__ void uf_union(int *uf, int a, int b, int *events) {
     for (int tries = 0; tries < (1 << 20); ++tries) {
-        a = uf_find(uf, a);
-        b = uf_find(uf, b);
+        a = uf_find(uf, a, events);
+        b = uf_find(uf, b, events);
         if (a == b) return;
         if (a < b) {
             const int t = a;
@@ -282,6 +286,7 @@ __device__ __forceinline__ void uf_union(int *uf, int a, int b) {
         }
         if (atomicCAS(&uf[a], a, b) == a) return;
     }
+    atomicAdd(&events[kEvUnionFind], 1);
 }
 
 // Collision-free candidate edges join both adjacency lists (the reference's
@@ -321,7 +326,7 @@ __global__ __launch_bounds__(64) void k_rm_link_own(SlotDev *slots, const int *i
     if (ok) {
         const int pos = __popcll(bal & ((1ull << lane) - 1ull));
         if (pos < s.rm_K) nbr[(size_t)v * s.rm_K + pos] = u;
-        uf_union(s.rm_uf + (size_t)a * s.rm_cap, u, v);
+        uf_union(s.rm_uf + (size_t)a * s.rm_cap, u, v, s.events);
         int *slot_of = s.rl_slot + (size_t)a * s.rm_cap;
         if (atomicCAS(&slot_of[u], -1, -2) == -1) {  // first claim of u this round
             const int k = atomicAdd(&s.rl_ntouched[a], 1);
@@ -397,9 +402,9 @@ __global__ void k_rm_connected(SlotDev *slots, const int *ids) {
     const int a = threadIdx.x;
     if (a >= s.A) return;
     int *uf = s.rm_uf + (size_t)a * s.rm_cap;
-    const int r0 = uf_find(uf, 0);
+    const int r0 = uf_find(uf, 0, s.events);
     int c = 0;
-    for (int g = 0; g < s.n_goals; ++g) c |= uf_find(uf, 1 + g) == r0;
+    for (int g = 0; g < s.n_goals; ++g) c |= uf_find(uf, 1 + g, s.events) == r0;
     s.rm_connected[a] = c;
 }
 
@@ -486,7 +491,8 @@ __device__ int claim_edge(const SlotDev &s, unsigned tag, int b, int t, int near
                    rid[a];
         if (same) return w;
     }
-    return -1;  // not reached: the table has more entries than a round has expansions
+    atomicAdd(&s.events[kEvEdgeTable], 1);  // not reached: the table has more entries than a round has expansions
+    return -1;
 }
 
 // One dRRT expansion per wave (pop_planner.cu:350-622 up to the edge check):
@@ -811,7 +817,10 @@ __global__ __launch_bounds__(64) void k_dr_connect(SlotDev *slots, const int *id
     if (bi == 0x7fffffff) return;
     const float d = sqrtf(bd);
     const int next = (int)(d / s.range) + 1;
-    if (next > s.ce_cap / s.B) return;  // longer than the chunk budget per expansion: no attempt
+    if (next > s.ce_cap / s.B) {  // longer than the chunk budget per expansion: no attempt
+        if (lane == 0) atomicAdd(&s.events[kEvConnectSkip], 1);
+        return;
+    }
     int base = 0;
     if (lane == 0) base = atomicAdd(s.ce_count, next);
     base = __shfl(base, 0);
@@ -1064,6 +1073,7 @@ __global__ __launch_bounds__(1024) void k_sp_edges(SlotDev *slots, const int *id
         if (live && cnt > 0) {
             if (off + cnt > s.sp_cap) {
                 s.sp_fail[item] = 1;  // no room: keep the waypoint / no shortcut
+                atomicAdd(&s.events[kEvSimplifyDrop], 1);
             } else {
                 int e = off;
                 for (int side = 0; side < (smooth ? 2 : 1); ++side) {
@@ -1170,6 +1180,7 @@ struct ReportDev {
     int path_meta[3];
     int sp_count;
     int ctr[4];
+    int events[kEvents];
 };
 
 __global__ void k_report(SlotDev *slots, const int *ids, int n, ReportDev *out) {
@@ -1191,6 +1202,7 @@ __global__ void k_report(SlotDev *slots, const int *ids, int n, ReportDev *out) 
     for (int k = 0; k < 3; ++k) r.path_meta[k] = s.path_meta[k];
     r.sp_count = *s.sp_count;
     for (int k = 0; k < 4; ++k) r.ctr[k] = s.ctr[k];
+    for (int k = 0; k < kEvents; ++k) r.events[k] = s.events[k];
 }
 
 // ------------------------------------------------------------ launchers ----

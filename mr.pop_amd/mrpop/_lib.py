@@ -15,7 +15,7 @@ PKG = Path(__file__).resolve().parent.parent          # mr.pop_amd/
 LIB_PATH = PKG / "libmrpop.so"
 HEADER = PKG.parent / "include" / "mrpop_c.h"
 
-OK, ERR_HIP, ERR_INVALID, ERR_BUDGET, ERR_TIMEOUT, ERR_NO_DEVICE, ERR_STATE = range(7)
+OK, ERR_HIP, ERR_INVALID, ERR_BUDGET, ERR_TIMEOUT, ERR_NO_DEVICE, ERR_STATE, ERR_INTERNAL = range(8)
 MODE_FULL, MODE_DRRT, MODE_SINGLE = 0, 1, 2
 
 P = ctypes.c_void_p
@@ -52,7 +52,8 @@ class ResultInfo(ctypes.Structure):
     _fields_ = [("solved", I), ("cost", ctypes.c_float), ("path_length", I), ("start_tree_size", I),
                 ("goal_tree_size", I), ("iters", I), ("wall_ns", U64), ("kernel_ns", U64), ("copy_ns", U64),
                 ("ttfs_ns", U64), ("n_seed_paths", I), ("passes", I), ("rounds", I), ("roadmap_rounds", I),
-                ("roadmap_size", I * 5), ("status", I)]
+                ("roadmap_size", I * 5), ("status", I), ("union_find_bound", I), ("edge_table_full", I),
+                ("connect_skipped", I), ("simplify_dropped", I)]
 
 
 class MrpopError(RuntimeError):
@@ -299,6 +300,15 @@ class Solution:
     @property
     def wall_s(self):
         return self.info["wall_ns"] * 1e-9
+
+
+def slot_bytes(ctx: Context, n_arms: int, settings=None, options=None) -> int:
+    """Device bytes of one problem (what a server's HBM budget admits against); 0 if rejected."""
+    fn = lib().mrpop_diag_slot_bytes
+    fn.restype = ctypes.c_size_t
+    fn.argtypes = [P, I, P, P]
+    return int(fn(ctx.handle, n_arms, None if settings is None else ctypes.byref(settings),
+                  None if options is None else ctypes.byref(options)))
 
 
 def solve(ctx: Context, problem, settings=None, options=None) -> Solution:

@@ -100,3 +100,33 @@ def expected_lists(n_nodes: int, accepted_edges, K: int):
         nbrs[v].append(u)
     lists = [sorted(x) for x in nbrs]
     return [l[:K] for l in lists], [len(l) for l in lists]
+
+
+def start_component(n_nodes: int, accepted_edges) -> np.ndarray:
+    """Nodes in the start's component, by the reference's label propagation
+    (roadmap_interior.cuh:275-303): node 0 (the start) carries label 0; every
+    sweep gives label 0 to each neighbour of a label-0 node, until a sweep
+    changes nothing.  Returns a bool array, True for label 0."""
+    adj = [[] for _ in range(n_nodes)]
+    for u, v in accepted_edges:
+        adj[u].append(v)
+        adj[v].append(u)
+    label0 = np.zeros(n_nodes, bool)
+    if n_nodes:
+        label0[0] = True
+    change = True
+    while change:
+        change = False
+        for t in range(n_nodes):
+            if label0[t]:
+                for nn in adj[t]:
+                    if not label0[nn]:
+                        label0[nn] = True
+                        change = True
+    return label0
+
+
+def start_goal_connected(label0: np.ndarray, n_goals: int) -> bool:
+    """Some goal node 1..n_goals carries the start's label
+    (roadmap_interior.cuh:305-313)."""
+    return bool(label0[1:1 + n_goals].any())

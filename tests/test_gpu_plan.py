@@ -263,12 +263,24 @@ def test_server_continuous_batching(ctx, alone):
 
 
 def test_server_budget(ctx):
-    """A problem larger than the whole budget is refused, not queued forever."""
-    srv = _lib.Server(ctx, hbm_budget_bytes=1 << 20)
-    srv.submit(PROBLEMS[0], _settings(), _options())
-    with pytest.raises(_lib.MrpopError) as e:
-        srv.step()
-    assert e.value.code == _lib.ERR_BUDGET
+    """A problem larger than the whole budget is refused, not queued forever:
+    it comes back done with status ERR_BUDGET and the requests behind it are
+    admitted and planned.  A batch with one such problem plans the others."""
+    small = _options(tree_capacity=65536, roadmap_capacity=20000, neighbour_capacity=64)
+    big = _options()
+    assert _lib.slot_bytes(ctx, PROBLEMS[0].n_arms, _settings(), big) > \
+        _lib.slot_bytes(ctx, PROBLEMS[1].n_arms, _settings(), small) + (1 << 20)
+    srv = _lib.Server(ctx, hbm_budget_bytes=_lib.slot_bytes(ctx, PROBLEMS[1].n_arms, _settings(), small) + (1 << 20))
+    t_big = srv.submit(PROBLEMS[0], _settings(), big)
+    t_small = srv.submit(PROBLEMS[1], _settings(), small)
+    deadline = time.time() + 120
+    while srv.step() != (0, 0):
+        assert time.time() < deadline, "server did not finish"
+    st, sol = srv.poll(t_big, snapshot=True)
+    assert st == 2 and sol.info["status"] == _lib.ERR_BUDGET and not sol.solved
+    st, sol = srv.poll(t_small, snapshot=True)
+    assert st == 2 and sol.info["status"] in (_lib.OK, _lib.ERR_TIMEOUT) and sol.solved
+    check_solution(PROBLEMS[1], sol)
     srv.close()
 
 
@@ -311,6 +323,10 @@ def test_invalid_problems_are_rejected(ctx):
            (p, _settings(num_new_configs=0), ok[1]), (p, _settings(num_new_configs=-5), ok[1]),
            (p, _settings(range=0.0), ok[1]), (p, _settings(range=float("nan")), ok[1]),
            (p, ok[0], _options(tree_capacity=-1)), (p, ok[0], _options(neighbour_candidates=65)),
+           # the goal tree's roots would not fit: refused on the host, no kernel runs
+           (p, ok[0], _options(tree_capacity=1)), (p, _settings(max_samples=1), ok[1]),
+           (dataclasses.replace(p, goals=np.stack([p.goals[0]] * 3)), ok[0], _options(tree_capacity=3)),
+           (dataclasses.replace(p, goals=np.stack([p.goals[0]] * 3)), _settings(max_samples=2), ok[1]),
            (dataclasses.replace(p, start=nan_start), *ok), (dataclasses.replace(p, goals=inf_goal), *ok)]
     cases = [(p, *ok)] + bad + [(PROBLEMS[3], *ok)]
     probs = [c[0] for c in cases]

@@ -54,19 +54,84 @@ def test_radius_candidates_take_the_first_in_id_order():
     assert RM.radius_candidates(nodes, 5, 0.9, 8) == [0, 2, 4]  # strictly closer than the range
 
 
+def test_start_component_matches_connected_components():
+    """The reference's label propagation (oracle.roadmap.start_component)
+    against scipy's connected components on random graphs, plus known
+    answers: a chain reaches its end, a goal on another component does not,
+    an isolated start reaches nothing."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    rng = np.random.default_rng(5)
+    for trial in range(60):
+        n = int(rng.integers(1, 80))
+        m = int(rng.integers(0, 2 * n))
+        edges = [tuple(int(x) for x in rng.integers(0, n, 2)) for _ in range(m)]
+        edges = [(u, v) for u, v in edges if u != v]
+        label0 = RM.start_component(n, edges)
+        if edges:
+            e = np.array(edges)
+            g = coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(n, n))
+            _, lab = connected_components(g, directed=False)
+        else:
+            lab = np.arange(n)
+        assert np.array_equal(label0, lab == lab[0]), f"trial {trial}"
+    chain = [(0, 5), (5, 3), (3, 1)]
+    assert RM.start_component(6, chain).tolist() == [True, True, False, True, False, True]
+    assert RM.start_goal_connected(RM.start_component(6, chain), 1)
+    assert not RM.start_goal_connected(RM.start_component(6, [(0, 5), (1, 2)]), 1)
+    assert RM.start_goal_connected(RM.start_component(6, [(0, 5), (5, 2)]), 2)  # the second goal
+    assert RM.start_component(4, [(1, 2), (2, 3)]).tolist() == [True, False, False, False]
+
+
 def _dump(ctx, p, st, op, rounds, arm, cap, K):
     lib = _lib.lib()
-    lib.mrpop_diag_roadmap.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_int]
+    lib.mrpop_diag_roadmap.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 6 + [ctypes.c_int]
     from mrpop._lib import _ProblemRef
     ref = _ProblemRef(p)
     q = np.zeros((cap, 8), np.float32)
     deg = np.zeros(cap, np.int32)
     nbr = np.zeros((cap, K), np.int32)
+    uf = np.zeros(cap, np.int32)
+    conn = ctypes.c_int()
     n = ctypes.c_int()
     ctx.check(lib.mrpop_diag_roadmap(ctx.handle, ctypes.byref(ref.c), ctypes.byref(st), ctypes.byref(op), rounds, arm,
-                                     q.ctypes.data, deg.ctypes.data, nbr.ctypes.data, ctypes.byref(n), cap))
+                                     q.ctypes.data, deg.ctypes.data, nbr.ctypes.data, uf.ctypes.data,
+                                     ctypes.byref(conn), ctypes.byref(n), cap))
     m = n.value
-    return q[:m, :7].copy(), deg[:m].copy(), nbr[:m].copy()
+    return q[:m, :7].copy(), deg[:m].copy(), nbr[:m].copy(), uf[:m].copy(), bool(conn.value)
+
+
+def _uf_roots(parent):
+    """Root of every node of a union-find parent array (host walk)."""
+    root = np.empty(len(parent), np.int64)
+    for i in range(len(parent)):
+        x, steps = i, 0
+        while parent[x] != x:
+            x = parent[x]
+            steps += 1
+            assert steps <= len(parent), "union-find parent links form a cycle"
+        root[i] = x
+    return root
+
+
+def _check_components(uf, connected, n_nodes, accepted, n_goals):
+    """P2: the device's union-find partition and start-goal flag against the
+    oracle's label propagation (start component) and scipy's components
+    (every component, rooted at its smallest node)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    root = _uf_roots(uf)
+    label0 = RM.start_component(n_nodes, accepted)
+    assert np.array_equal(root == root[0], label0), "start component differs from the label propagation"
+    assert connected == RM.start_goal_connected(label0, n_goals), "rm_connected differs"
+    e = np.array(accepted if accepted else [(0, 0)])
+    g = coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(n_nodes, n_nodes))
+    _, lab = connected_components(g, directed=False)
+    smallest = {}
+    for i, l in enumerate(lab):
+        smallest.setdefault(int(l), i)
+    assert np.array_equal(root, np.array([smallest[int(l)] for l in lab])), "union-find partition differs"
+    return int(label0.sum()), len(smallest)
 
 
 @pytest.mark.gpu
@@ -81,7 +146,7 @@ def test_roadmap_rounds_match_the_restatement(ctx, kind, seed, arm, n_goals):
     op = _lib.solve_options_default(block_per_robot=bpr, neighbour_candidates=kc, neighbour_capacity=K,
                                     roadmap_capacity=4096)
     print(f"[roadmap] {kind} seed {seed} arm {arm} goals {n_goals}", flush=True)
-    q, deg, nbr = _dump(ctx, p, st, op, rounds, arm, 4096, K)
+    q, deg, nbr, uf, connected = _dump(ctx, p, st, op, rounds, arm, 4096, K)
     scale, low = geometry.table("kScale"), geometry.table("kLow")
     # nodes 0 .. n_goals: start and goal slices of the arm
     n_fixed = 1 + len(p.goals)
@@ -122,4 +187,41 @@ def test_roadmap_rounds_match_the_restatement(ctx, kind, seed, arm, n_goals):
     for i, l in enumerate(lists):
         assert list(nbr[i][:len(l)]) == l, f"node {i}: neighbour list differs"
     assert len(accepted) > 100 and max(degs) > K  # the test exercises truncation
-    print(f"[roadmap] {len(nodes)} nodes, {len(accepted)} edges, {exempt} near-contact samples", flush=True)
+    n_start, n_comp = _check_components(uf, connected, len(nodes), accepted, len(p.goals))
+    print(f"[roadmap] {len(nodes)} nodes, {len(accepted)} edges, {exempt} near-contact samples, "
+          f"{n_comp} components, {n_start} nodes with the start, connected {connected}", flush=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,seed", [("bin4", 3), ("shelf5", 5), ("plain2", 7)])
+def test_union_find_matches_the_adjacency(ctx, kind, seed):
+    """P2 at planner scale: roadmaps of 1840 samples per round and arm, grown
+    for four rounds with neighbour lists long enough to hold every edge; per
+    arm the union-find partition, the start component and rm_connected
+    against the label propagation and the components of the dumped
+    adjacency."""
+    p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
+    K, cap = 512, 20000
+    st = _lib.settings(granularity=32, range=2.0)
+    op = _lib.solve_options_default(block_per_robot=8, neighbour_candidates=64, neighbour_capacity=K,
+                                    roadmap_capacity=cap)
+    seen_connected = []
+    for arm in range(p.n_arms):
+        print(f"[components] {kind} seed {seed} arm {arm}", flush=True)
+        q, deg, nbr, uf, connected = _dump(ctx, p, st, op, 4, arm, cap, K)
+        n = len(q)
+        assert deg.max() <= K, "a neighbour list was truncated: the adjacency would be incomplete"
+        edges = set()
+        for u in range(n):
+            for v in nbr[u][:deg[u]]:
+                v = int(v)
+                assert 0 <= v < n and v != u
+                edges.add((min(u, v), max(u, v)))
+        for u, v in edges:  # symmetric lists
+            assert u in nbr[v][:deg[v]] and v in nbr[u][:deg[u]]
+        assert sum(deg) == 2 * len(edges)
+        n_start, n_comp = _check_components(uf, connected, n, sorted(edges), len(p.goals))
+        seen_connected.append(connected)
+        print(f"[components] arm {arm}: {n} nodes, {len(edges)} edges, {n_comp} components, "
+              f"{n_start} with the start, connected {connected}", flush=True)
+    assert any(seen_connected)

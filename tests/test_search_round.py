@@ -28,6 +28,7 @@ from mrpop import _lib, geometry, scenes
 from oracle import oracle as O
 from oracle import search as SR
 from test_gpu_plan import interior_waypoint
+import test_phs
 
 
 def _round_half_even_f32(x: Fraction) -> np.float32:
@@ -208,6 +209,7 @@ def _check_round(p, ds, before, after, settings, optimize=False, phs=None):
     best = before["best"]
     cand = {}
     rejected = 0
+    inside = 0
     for b in range(B):
         t = SR.tree_choice(s, b)
         assert after["ex_tree"][b] == t, f"expansion {b}: tree"
@@ -217,6 +219,13 @@ def _check_round(p, ds, before, after, settings, optimize=False, phs=None):
             q64 = SR.phs_sample64(s, b)
             assert np.abs(q.astype(np.float64) - q64).max() < 1e-4, f"expansion {b}: PHS sample"
             g = (np.linalg.norm(s.tr_q[0, 0].astype(np.float64) - q) + np.linalg.norm(s.tr_q[1, 0].astype(np.float64) - q))
+            # a sample no joint limit clamped lies in the hyperspheroid with foci
+            # start and goal 0 and transverse diameter best (pop_planner.cu:398-442)
+            low = np.tile(np.array(geometry.table("kLow"), np.float32), A)
+            high = (low + np.tile(np.array(geometry.table("kScale"), np.float32), A)).astype(np.float32)
+            if not ((q == low) | (q == high)).any():
+                assert g <= float(best) * (1 + 1e-5) + 1e-5, f"expansion {b}: unclamped PHS sample outside"
+                inside += 1
             dev_rejected = after["ex_crand"][b] == -1.0
             if abs(g - float(best)) > 1e-5:
                 assert dev_rejected == (g >= best), f"expansion {b}: informed rejection"
@@ -385,7 +394,8 @@ def _check_round(p, ds, before, after, settings, optimize=False, phs=None):
         assert after["path_cost"] == cost and after["best"] == min(np.float32(best), cost)
     return dict(candidates=len(cand), edges=n_edges, inserted=len(index), connections=len(plans),
                 rejected=rejected, rewired=len(rewires), solved=winner is not None,
-                goal=reached[0] if winner is not None else None, exempt=exempt, n_commit=n_commit)
+                goal=reached[0] if winner is not None else None, exempt=exempt, n_commit=n_commit,
+                inside=inside)
 
 
 @pytest.mark.gpu
@@ -585,6 +595,13 @@ def test_optimise_pass_rounds_match_the_restatement(ctx, kind, seed):
         assert ds.read("counters", np.int32, (64,))[4] > 0, "first pass not solved in 40 rounds"
         ds.start_pass(1)
         phs = (ds.read("phs_tf", np.float32, (ds.D, ds.D)), ds.read("phs_center", np.float32, (ds.D,)))
+        # the device samples with the host transform pinned in test_phs.py:
+        # foci start and goal 0, transverse diameter the best cost
+        best = ds.state()["best"]
+        tf_ref, c_ref = test_phs.phs(p.start, p.goals[0], float(best))
+        assert np.array_equal(phs[0], tf_ref.astype(np.float32)) and np.array_equal(phs[1], c_ref.astype(np.float32))
+        cmin = float(np.linalg.norm(p.goals[0].astype(np.float64) - p.start))
+        assert float(best) > cmin and np.linalg.norm(phs[0][:, 0]) == pytest.approx(float(best) / 2, rel=1e-5)
         stats = []
         keep = False
         for _ in range(6):
@@ -600,6 +617,7 @@ def test_optimise_pass_rounds_match_the_restatement(ctx, kind, seed):
         # samples come from the informed set, so rejections are rare (clamping
         # to the joint box only); rewires must occur somewhere in the set
         assert sum(x["candidates"] for x in stats) > 0
+        assert sum(x["inside"] for x in stats) > 0, "no unclamped hyperspheroid sample was checked"
         if kind == "table3":
             assert sum(x["rewired"] for x in stats) > 0
     finally:
