@@ -159,6 +159,25 @@ int mrpop_check_edges_device(mrpop_ctx *ctx, const mrpop_world *w, int mode, int
                              void *stream);
 
 /*
+ * mrpop_check_edges without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture set-up, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and synchronous behaviour as mrpop_check_edges,
+ * but the host inputs and outputs are staged through one fixed array in
+ * device memory per GPU (MRPOP_NOALLOC_SCRATCH_BYTES, held by each GPU the
+ * library runs on until the process ends) instead of context buffers.
+ * Staging needs 2*D*4 bytes per edge for the endpoints, 5 per edge for the
+ * verdicts and 8 more with out_mask (each part rounded up to 256 bytes);
+ * larger calls fail with MRPOP_ERR_BUDGET before anything runs.  Every call
+ * on one GPU shares the array: calls that overlap on one GPU (two threads,
+ * two contexts) race and must be serialised by the caller.
+ */
+#define MRPOP_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int mrpop_check_edges_noalloc(mrpop_ctx *ctx, const mrpop_world *w, int mode, int arm, int granularity,
+                              int n_edges, const float *from, const float *to, uint8_t *out_edge,
+                              uint64_t *out_mask);
+
+/*
  * Sphere centres by forward kinematics (replaces fk / fk_approx,
  * panda_four.cuh:44-131, 289-373, and fk_mr / fk_approx_mr, panda.cuh:598-829).
  * For n_cfg configurations of the world's n_arms arms (D = 7*n_arms floats

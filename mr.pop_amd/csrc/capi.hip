@@ -4,10 +4,17 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
 #include "internal.h"
+
+// Staging array of mrpop_check_edges_noalloc, one copy per device.  Host code
+// never uses its name as an address: each device's copy is looked up once
+// with hipGetSymbolAddress (scratch_address, called from mrpop_create) and
+// that pointer is what kernels, copies and fills receive.
+static __device__ __align__(256) unsigned char scratch[MRPOP_NOALLOC_SCRATCH_BYTES];
 
 namespace mrpop {
 
@@ -28,7 +35,31 @@ int ensure(Context *c, DevBuf &b, size_t bytes) {
 
 namespace {
 
-const char *kVersion = "mrpop-mi355x 0.1 (gfx950)";
+const char *kVersion = "mrpop-mi355x 0.2 (gfx950)";
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+// Device address of this device's scratch array (the device must be
+// current); looked up on the first call per device.
+int scratch_address(int device, void **out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return MRPOP_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MRPOP_ERR_HIP;
+        g_scratch[device] = p;
+    }
+    *out = g_scratch[device];
+    return MRPOP_OK;
+}
+
+// Edge verdict words to bytes (0 or 1), on the device.
+__global__ void verdict_bytes_kernel(const uint32_t *in, uint8_t *out, int n) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) out[e] = in[e] != 0u;
+}
 
 bool is_world_axes(const float *rec) {
     // cuboid record: axes at 3..11; axis-aligned iff they are exactly e_x, e_y, e_z
@@ -254,6 +285,8 @@ int mrpop_create(int device, mrpop_ctx **out) {
         c->num_cus = prop.multiProcessorCount;
         if ((rc = ensure(c, c->counters, 256))) break;
         if (hipMemset(c->counters.ptr, 0, c->counters.bytes) != hipSuccess) { rc = MRPOP_ERR_HIP; break; }
+        // this device's static staging array (mrpop_check_edges_noalloc)
+        if ((rc = scratch_address(device, &c->scratch))) break;
     } while (0);
     if (rc) {
         if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -370,6 +403,42 @@ int mrpop_check_edges(mrpop_ctx *c, const mrpop_world *w, int mode, int arm, int
         MRPOP_HIP(c, hipMemcpyAsync(out_mask, c->out_mask.ptr, (size_t)n_edges * 8, hipMemcpyDeviceToHost, c->stream));
     MRPOP_HIP(c, hipStreamSynchronize(c->stream));
     for (int e = 0; e < n_edges; ++e) out_edge[e] = tmp[e] != 0;
+    return MRPOP_OK;
+}
+
+int mrpop_check_edges_noalloc(mrpop_ctx *c, const mrpop_world *w, int mode, int arm, int granularity, int n_edges,
+                              const float *from, const float *to, uint8_t *out_edge, uint64_t *out_mask) {
+    if (!c || !w || (n_edges > 0 && (!from || !to || !out_edge))) return MRPOP_ERR_INVALID;
+    if (n_edges < 0) return c->fail(MRPOP_ERR_INVALID, "negative edge count");
+    if (n_edges == 0) return MRPOP_OK;
+    if (!c->scratch) return c->fail(MRPOP_ERR_STATE, "the context has no staging array");
+    const int A = mode == MRPOP_MODE_SINGLE ? 1 : w->n_arms;
+    // layout of the staging array: from, to, verdict words, masks, verdict bytes
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t cfg_bytes = (size_t)n_edges * 7 * A * sizeof(float);
+    const size_t o_to = up(cfg_bytes), o_word = o_to + up(cfg_bytes), o_mask = o_word + up((size_t)n_edges * 4);
+    const size_t o_byte = o_mask + (out_mask ? up((size_t)n_edges * 8) : 0), total = o_byte + up((size_t)n_edges);
+    if (total > MRPOP_NOALLOC_SCRATCH_BYTES)
+        return c->fail(MRPOP_ERR_BUDGET, "%d edges need %zu bytes of staging, more than the %zu-byte array", n_edges,
+                       total, (size_t)MRPOP_NOALLOC_SCRATCH_BYTES);
+    char *base = (char *)c->scratch;  // device address of this device's array
+    float *d_from = (float *)base, *d_to = (float *)(base + o_to);
+    uint32_t *d_word = (uint32_t *)(base + o_word);
+    uint64_t *d_mask = out_mask ? (uint64_t *)(base + o_mask) : nullptr;
+    uint8_t *d_byte = (uint8_t *)(base + o_byte);
+    MRPOP_HIP(c, hipSetDevice(c->device));
+    MRPOP_HIP(c, hipMemcpyAsync(d_from, from, cfg_bytes, hipMemcpyHostToDevice, c->stream));
+    MRPOP_HIP(c, hipMemcpyAsync(d_to, to, cfg_bytes, hipMemcpyHostToDevice, c->stream));
+    // the check clears the verdict words and masks with a kernel before it ORs into them
+    if (int rc = launch_check(c, w, mode, arm, granularity, n_edges, d_from, d_to, d_word, d_mask, c->stream))
+        return rc;
+    hipLaunchKernelGGL(verdict_bytes_kernel, dim3(std::min(1024, (n_edges + 255) / 256)), dim3(256), 0, c->stream,
+                       d_word, d_byte, n_edges);
+    MRPOP_HIP(c, hipGetLastError());
+    MRPOP_HIP(c, hipMemcpyAsync(out_edge, d_byte, (size_t)n_edges, hipMemcpyDeviceToHost, c->stream));
+    if (out_mask)
+        MRPOP_HIP(c, hipMemcpyAsync(out_mask, d_mask, (size_t)n_edges * 8, hipMemcpyDeviceToHost, c->stream));
+    MRPOP_HIP(c, hipStreamSynchronize(c->stream));
     return MRPOP_OK;
 }
 

@@ -34,6 +34,7 @@ struct Context {
     // staging for host-buffer entry points
     DevBuf from, to, out_edge, out_mask, cfg, out_a, out_b;
     std::vector<DevBuf *> owned;
+    void *scratch = nullptr;  // this device's static staging array (device address), see capi.hip
     void *diag = nullptr;  // when set, edge checks record per-sample phase results (diagnostics)
     int probe = 0;         // > 0: edge-only checks run a probe pass of this many samples first (diagnostics)
 

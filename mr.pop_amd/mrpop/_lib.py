@@ -75,6 +75,7 @@ SIGNATURES = {
     "mrpop_world_destroy": (I, [P]),
     "mrpop_check_edges": (I, [P, P, I, I, I, I, P, P, P, P]),
     "mrpop_check_edges_device": (I, [P, P, I, I, I, I, P, P, P, P, P]),
+    "mrpop_check_edges_noalloc": (I, [P, P, I, I, I, I, P, P, P, P]),
     "mrpop_sphere_centres": (I, [P, P, I, P, P, P]),
     "mrpop_solve_options_default": (I, [P]),
     "mrpop_solve": (I, [P, P, P, P, P]),
@@ -189,15 +190,17 @@ class World:
         except Exception:
             pass
 
-    def check_edges(self, mode, granularity, frm, to, arm=0, with_mask=False):
-        """Host-buffer edge check: bool[n] (and uint64[n] sample masks)."""
+    def check_edges(self, mode, granularity, frm, to, arm=0, with_mask=False, noalloc=False):
+        """Host-buffer edge check: bool[n] (and uint64[n] sample masks).
+        noalloc: stage through the library's fixed per-device array
+        (mrpop_check_edges_noalloc) instead of context buffers."""
         frm, to = _f32(frm), _f32(to)
         n = frm.shape[0]
         out = np.zeros(n, np.uint8)
         mask = np.zeros(n, np.uint64) if with_mask else None
-        self.ctx.check(lib().mrpop_check_edges(self.ctx.handle, self._h, mode, arm, granularity, n,
-                                               frm.ctypes.data, to.ctypes.data, out.ctypes.data,
-                                               None if mask is None else mask.ctypes.data))
+        fn = lib().mrpop_check_edges_noalloc if noalloc else lib().mrpop_check_edges
+        self.ctx.check(fn(self.ctx.handle, self._h, mode, arm, granularity, n, frm.ctypes.data, to.ctypes.data,
+                          out.ctypes.data, None if mask is None else mask.ctypes.data))
         return (out.astype(bool), mask) if with_mask else out.astype(bool)
 
     def check_edges_device(self, mode, granularity, n, d_from, d_to, d_out, d_mask=None, stream=None, arm=0):
