@@ -203,6 +203,10 @@ class Engine {
     int *d_aux_id_ = nullptr;
     size_t group_cap_ = 0;
     size_t bytes_in_use_ = 0;
+    // search-lane steps while first solutions are pending; optimise passes
+    // run on one of every kOptimiseShare of them
+    static constexpr uint64_t kOptimiseShare = 4;
+    uint64_t search_steps_ = 0;
     // released slot blocks by size, reused by later problems (serving)
     static constexpr size_t kPoolBytes = size_t(16) << 30;
     std::multimap<size_t, void *> pool_;

@@ -916,9 +916,16 @@ int Engine::step() {
         if (int rc = launch_lane(R)) return rc;
     }
     if (!S.busy) {
+        // Time to first solution first: while some problem is still searching
+        // for its first path, the optimise passes of solved problems join the
+        // search launch on one step in kOptimiseShare, so a hard problem's
+        // rounds are not slowed by every other problem's optimisation.
+        bool seekers = false;
+        for (Slot *s : slots_) seekers |= s->lane < 0 && s->phase == Phase::Search && s->seeking;
+        const bool optimisers = !seekers || (search_steps_++ % kOptimiseShare) == 0;
         for (Slot *s : slots_) {
             if (s->lane >= 0) continue;
-            if (s->phase == Phase::Search) S.dr.push_back(s);
+            if (s->phase == Phase::Search && (s->seeking || optimisers)) S.dr.push_back(s);
             else if (s->phase == Phase::Simplify) S.sp.push_back(s);
         }
         if (int rc = launch_lane(S)) return rc;

@@ -367,3 +367,45 @@ def test_several_goals(ctx):
         reached.append(int(np.argmin([np.abs(sol.path[0] - g).max() for g in p.goals])))
     print(f"[goals] goal reached per problem: {reached}", flush=True)
     assert any(r != 0 for r in reached)
+
+
+# the reference's evaluation harness (scripts/evaluate_mr.cu:159-166): pop_settings
+# defaults with num_new_configs 300, granularity 64, range 2.0, rrtc_iter 3, and
+# dRRT::solve(..., n_arms, 8), i.e. block_per_robot 8
+HARNESS = dict(num_new_configs=300, granularity=64, range=2.0, rrtc_iter=3)
+FIXTURE_BUDGET_S = 20.0
+
+
+def test_reference_fixtures_all_solved(ctx):
+    """The reference's own benchmark: all 50 bin-picking (4 arms) and 50
+    shelf (5 arms) problems of scripts/panda_{four,five}_problems.json at the
+    harness settings, planned together in one batch.  README.md:17-20 claims
+    a 100% solve rate for both sets; every problem must be solved and every
+    returned path must pass the FP64 re-validation.  The optimise passes run
+    with the harness's optimize_iters (400 000) under a wall-clock budget
+    per problem.  Solve rate, time to first solution and cost distribution
+    are printed for the record."""
+    four = scenes.load_problem_file("tests/golden/panda_four_problems.json", geometry.default_bases(4))
+    five = scenes.load_problem_file("tests/golden/panda_five_problems.json", geometry.default_bases(5))
+    probs = four + five
+    assert len(four) == 50 and len(five) == 50
+    st = _lib.settings(**HARNESS)
+    op = _lib.solve_options_default(time_budget_s=FIXTURE_BUDGET_S)
+    print(f"[fixtures] solving {len(probs)} problems in one batch", flush=True)
+    t0 = time.time()
+    sols = _lib.solve_batch(ctx, probs, [st] * len(probs), [op] * len(probs))
+    wall = time.time() - t0
+    unsolved = [p.name for p, s in zip(probs, sols) if not s.solved]
+    for name, group in (("panda_four", sols[:50]), ("panda_five", sols[50:])):
+        solved = [s for s in group if s.solved]
+        costs = np.array([s.cost for s in solved])
+        ttfs = np.array([s.ttfs_s for s in solved])
+        first = np.array([s.info["n_seed_paths"] and s.seed_paths[0][0] for s in solved])
+        print(f"[fixtures] {name}: solved {len(solved)}/50, cost median {np.median(costs):.3f} "
+              f"p90 {np.percentile(costs, 90):.3f} (first solution median {np.median(first):.3f}), "
+              f"TTFS median {np.median(ttfs) * 1e3:.0f} ms p99 {np.percentile(ttfs, 99) * 1e3:.0f} ms", flush=True)
+    print(f"[fixtures] batch wall {wall:.1f} s", flush=True)
+    assert not unsolved, f"unsolved: {unsolved}"
+    for p, s in zip(probs, sols):
+        assert s.info["union_find_bound"] == 0 and s.info["edge_table_full"] == 0
+        check_solution(p, s, granularity=HARNESS["granularity"], rng=HARNESS["range"])
