@@ -218,9 +218,10 @@ class World:
         return co, fi
 
     def config_collides(self, cfgs):
-        """Full multi-arm verdict of single configurations (G = 1 edges cfg -> cfg)."""
+        """Full multi-arm verdict of single configurations (G = 1 edges cfg -> cfg);
+        a one-arm world uses the single-arm check (environment + self)."""
         cfgs = _f32(cfgs).reshape(-1, 7 * self.n_arms)
-        return self.check_edges(MODE_FULL, 1, cfgs, cfgs)
+        return self.check_edges(MODE_SINGLE if self.n_arms == 1 else MODE_FULL, 1, cfgs, cfgs)
 
 
 def gpu_checker(ctx: Context):

@@ -333,6 +333,134 @@ def scene_shelf(rng: np.random.Generator, n_cylinders=15) -> Scene:
     return Scene(capsules=np.array(cap, np.float32), cuboids=np.array(cub, np.float32))
 
 
+SHELF_CENTRE = (0.3, 0.0)
+SHELF_LEVELS = (0.25, 0.5, 0.75, 1.0)
+
+
+def shelf_cells(margin: float = 0.03):
+    """The 9 cells of scene_shelf (3 gaps between boards x 3 columns of the
+    1 m width) as (low, high) corner pairs, shrunk by `margin` metres."""
+    cx, cy = SHELF_CENTRE
+    xs = np.linspace(cx - 0.49, cx + 0.49, 4)
+    cells = []
+    for z0, z1 in zip(SHELF_LEVELS[:-1], SHELF_LEVELS[1:]):
+        for x0, x1 in zip(xs[:-1], xs[1:]):
+            lo = np.array([x0, cy - 0.175, z0 + 0.01]) + margin
+            hi = np.array([x1, cy + 0.175, z1 - 0.01]) - margin
+            cells.append((lo, hi))
+    return cells
+
+
+def hand_positions(base: np.ndarray, q: np.ndarray) -> np.ndarray:
+    """World position of the hand (mean centre of the 4 coarse spheres on
+    joint 7) of one arm with base pose `base` (12 floats, row-major 3x4) for
+    configurations q [n, 7], by the kinematic chain T_j = T_(j-1) FT_j Rz(q_j)
+    of the sphere model (float64)."""
+    from . import geometry
+    ft = geometry.table("kJointTf").reshape(8, 3, 4).astype(np.float64)
+    sph = geometry.table("kCoarseSphere").reshape(11, 4).astype(np.float64)
+    q = np.asarray(q, np.float64).reshape(-1, 7)
+    n = len(q)
+    B = np.asarray(base, np.float64).reshape(3, 4)
+    col = [np.broadcast_to(B[:, k], (n, 3)) for k in range(4)]  # columns of the 3x4 frame
+    for j in range(1, 8):
+        F = ft[j]
+        c, s = np.cos(q[:, j - 1])[:, None], np.sin(q[:, j - 1])[:, None]
+        # M = F Rz(q): columns F0 c + F1 s, F1 c - F0 s, F2, F3; new frame = T M
+        def mul(v):  # T applied to a direction v (3 rows of F)
+            return col[0] * v[0] + col[1] * v[1] + col[2] * v[2]
+        t0, t1, t2 = mul(F[:, 0]), mul(F[:, 1]), mul(F[:, 2])
+        col = [t0 * c + t1 * s, t1 * c - t0 * s, t2, mul(F[:, 3]) + col[3]]
+    off = sph[7:11, :3]
+    return sum(col[0] * o[0] + col[1] * o[1] + col[2] * o[2] + col[3] for o in off) / 4.0
+
+
+_CELL_POOLS = {}
+
+
+def _cell_pools(base, cells, count=512, batch=65536, rounds=8):
+    """Per cell, up to `count` arm configurations (uniform in the joint
+    limits) whose hand lies inside the cell.  They depend on the base pose
+    only, so they are drawn once per pose from a fixed stream and cached."""
+    key = (np.asarray(base, np.float32).tobytes(), count)
+    if key in _CELL_POOLS:
+        return _CELL_POOLS[key]
+    rng = np.random.default_rng(0x5E1F)
+    found = [[] for _ in cells]
+    for _ in range(rounds):
+        u = np.maximum(rng.uniform(0.0, 1.0, (batch, DOF_PER_ARM)), 1e-7)
+        q = (u * SCALE + LOW).astype(np.float32)
+        p = hand_positions(base, q)
+        for k, (lo, hi) in enumerate(cells):
+            if len(found[k]) < count:
+                found[k].extend(q[((p > lo) & (p < hi)).all(axis=1)][:count - len(found[k])])
+        if all(len(f) >= count for f in found):
+            break
+    _CELL_POOLS[key] = [np.array(f, np.float32).reshape(-1, DOF_PER_ARM) for f in found]
+    return _CELL_POOLS[key]
+
+
+def shelf_cell_configs(rng: np.random.Generator, scene: Scene, bases: np.ndarray, count: int, checker: "Checker",
+                       tries: int = 64, batch: int = 2048) -> np.ndarray:
+    """`count` collision-free multi-arm configurations with every arm's hand
+    inside a shelf cell, the arms in distinct cells (configs[2] of the
+    benchmark: end-effectors reaching into random shelf cells).  Per arm and
+    cell a pool of in-cell arm configurations is drawn by rejection on the
+    hand position; composites of pool members over random cell assignments
+    are then checked whole."""
+    n_arms = len(bases)
+    cells = shelf_cells()
+    pools = []
+    for a in range(n_arms):  # keep the arm configurations free on their own (environment + self)
+        row = _cell_pools(bases[a], cells)
+        pools.append([pl[~checker(scene, 1, bases[a:a + 1], pl)] if len(pl) else pl for pl in row])
+    reach = [[k for k, pl in enumerate(pools[a]) if len(pl) >= 2] for a in range(n_arms)]
+    if any(not r for r in reach):
+        raise RuntimeError("an arm reaches no shelf cell")
+    out = []
+    for _ in range(tries):
+        cand = np.zeros((batch, DOF_PER_ARM * n_arms), np.float32)
+        for i in range(batch):
+            used = set()
+            for a in rng.permutation(n_arms):
+                free = [k for k in reach[a] if k not in used] or reach[a]
+                k = free[rng.integers(len(free))]
+                used.add(k)
+                pl = pools[a][k]
+                cand[i, DOF_PER_ARM * a:DOF_PER_ARM * (a + 1)] = pl[rng.integers(len(pl))]
+        ok = ~checker(scene, n_arms, bases, cand)
+        out.extend(cand[ok])
+        if len(out) >= count:
+            return np.stack(out[:count]).astype(np.float32)
+    raise RuntimeError(f"could not sample {count} in-cell configurations ({len(out)} found)")
+
+
+def scene_posts(rng: np.random.Generator, bases, n_posts=10, n_boxes=10) -> Scene:
+    """Work cell with the reference's z-aligned obstacle types
+    (utils.cuh:165-181, 208-228): vertical posts as z-capsules (r 2-6 cm,
+    h 10-60 cm) and yaw-rotated boxes as z-cuboids (half-extents 3-12 cm),
+    plus a table top and a few spheres."""
+    zcap, zcub = [], []
+    while len(zcap) < n_posts:
+        x, y = rng.uniform(-0.7, 0.7, 2)
+        if not _far_from_bases((x, y), bases, 0.3):
+            continue
+        z0, h, r = rng.uniform(0.0, 0.5), rng.uniform(0.1, 0.6), rng.uniform(0.02, 0.06)
+        zcap.append([x, y, z0, 0.0, 0.0, h, r, 1.0 / (h * h)])
+    while len(zcub) < n_boxes:
+        c = rng.uniform((-0.7, -0.7, 0.1), (0.7, 0.7, 1.0))
+        if not _far_from_bases(c, bases, 0.3):
+            continue
+        yaw = rng.uniform(-math.pi, math.pi)
+        cz, sz = math.cos(yaw), math.sin(yaw)
+        hx, hy, hz = rng.uniform(0.03, 0.12, 3)
+        zcub.append([c[0], c[1], c[2], cz, sz, 0.0, -sz, cz, 0.0, 0.0, 0.0, 1.0, hx, hy, hz])
+    sph = [sphere_record(rng.uniform((-0.6, -0.6, 0.3), (0.6, 0.6, 1.0)), rng.uniform(0.03, 0.08)) for _ in range(3)]
+    table = [cuboid_record((0, 0, -0.01), (0, 0, 0), (0.8, 0.8, 0.01))]
+    return Scene(spheres=np.array(sph, np.float32), z_capsules=np.array(zcap, np.float32),
+                 cuboids=np.array(table, np.float32), z_cuboids=np.array(zcub, np.float32))
+
+
 def scene_table3(rng: np.random.Generator, bases, n_random=12) -> Scene:
     """Round 3-arm work cell: a table top plus n_random mixed obstacles."""
     cub = [cuboid_record((0, 0, -0.01), (0, 0, 0), (0.8, 0.8, 0.01))]
@@ -365,8 +493,16 @@ def sample_free_configs(rng: np.random.Generator, scene: Scene, n_arms: int, bas
 
 def make_problem(kind: str, seed: int, checker: Checker, n_goals: int = 1) -> Problem:
     """Seeded synthetic problem.  kind: 'plain2' (2 arms, 16 random cuboids),
-    'table3' (3 arms), 'bin4' (4 arms around a bin), 'shelf5' (5 arms, shelf)."""
-    rng = np.random.default_rng([seed, {"plain2": 2, "table3": 3, "bin4": 4, "shelf5": 5}[kind]])
+    'table3' (3 arms), 'bin4' (4 arms around a bin), 'shelf5' (5 arms, shelf),
+    'shelf5cell' (the shelf with every hand inside a shelf cell at the start
+    and the goals), 'posts3' (3 arms among z-aligned posts and boxes)."""
+    rng = np.random.default_rng([seed, {"plain2": 2, "table3": 3, "bin4": 4, "shelf5": 5, "shelf5cell": 6,
+                                        "posts3": 7}[kind]])
+    if kind == "shelf5cell":
+        n_arms, bases = 5, default_bases(5)
+        scene = scene_shelf(rng)
+        cfgs = shelf_cell_configs(rng, scene, bases, 1 + n_goals, checker)
+        return Problem(f"{kind}_{seed}", n_arms, bases, scene, cfgs[0], cfgs[1:], seed)
     if kind == "plain2":
         n_arms, bases = 2, default_bases(2)
         scene = scene_random_cuboids(rng, bases)
@@ -379,6 +515,9 @@ def make_problem(kind: str, seed: int, checker: Checker, n_goals: int = 1) -> Pr
     elif kind == "shelf5":
         n_arms, bases = 5, default_bases(5)
         scene = scene_shelf(rng)
+    elif kind == "posts3":
+        n_arms, bases = 3, default_bases(3)
+        scene = scene_posts(rng, bases)
     else:
         raise ValueError(kind)
     cfgs = sample_free_configs(rng, scene, n_arms, bases, 1 + n_goals, checker)

@@ -49,6 +49,12 @@ constexpr int kModeFull = MRPOP_MODE_FULL;
 constexpr int kModeDrrt = MRPOP_MODE_DRRT;
 constexpr int kModeSingle = MRPOP_MODE_SINGLE;
 
+// Test switch (orc_set_plain_multi_arm_ranges): when set, the FP64 multi-arm
+// fine self ranges are read the single-arm way (gate, centre and radius of
+// sphere ranges[r][0]) instead of the reference's multi-arm reading.  Only
+// used to find configurations where the reference's reading matters.
+int g_plain_multi_arm_ranges = 0;
+
 // ---------------------------------------------------------------- FK ----
 // Rigid frame, row-major 3x4 [R | t].
 template <class R> struct Frame {
@@ -245,7 +251,7 @@ bool composite(const mrpop_scene &scene, int mode, const SampleGeo &g, int bias,
         for (int r = 0; r < kFineSelfRanges; ++r) {
             int s1 = kFineSelfRange[r][0];
             int gate_joint, pos_idx;
-            if (mode == kModeSingle) {
+            if (mode == kModeSingle || g_plain_multi_arm_ranges) {
                 gate_joint = kFineJoint[s1];
                 pos_idx = s1;
             } else {
@@ -412,6 +418,10 @@ int dof(int mode, int n_arms) { return mode == kModeSingle ? 7 : 7 * n_arms; }
 }  // namespace
 
 extern "C" {
+
+// Test switch of the FP64 verdicts: 1 = multi-arm fine self ranges read the
+// single-arm way (see g_plain_multi_arm_ranges), 0 = the reference's reading.
+void orc_set_plain_multi_arm_ranges(int on) { g_plain_multi_arm_ranges = on ? 1 : 0; }
 
 // Sphere centres in FP64 for one configuration of n_arms arms.
 // out_coarse: n_arms*11*3, out_fine: n_arms*59*3 (either may be NULL).

@@ -111,6 +111,17 @@ def check_edges(scene, mode, bases, G, frm, to, arm=0, eps=EPS_CONTACT):
     return out
 
 
+def check_edges_plain_ranges(scene, mode, bases, G, frm, to, arm=0, eps=EPS_CONTACT):
+    """check_edges with the multi-arm fine self ranges read the single-arm
+    way (not the reference's reading, panda_four.cuh:390-401): only for
+    finding samples where that reading changes the verdict."""
+    lib().orc_set_plain_multi_arm_ranges(1)
+    try:
+        return check_edges(scene, mode, bases, G, frm, to, arm=arm, eps=eps)
+    finally:
+        lib().orc_set_plain_multi_arm_ranges(0)
+
+
 def check_edges_f32(scene, mode, bases, G, frm, to, arm=0, threads=1):
     """FP32 mirror of the reference checker with early exits (CPU baseline)."""
     bases = _f32(bases).reshape(-1, 12)
@@ -126,7 +137,8 @@ def check_edges_f32(scene, mode, bases, G, frm, to, arm=0, threads=1):
 def config_checker(scene, n_arms, bases, cfgs):
     """scenes.Checker backed by the FP32 mirror: True = colliding (G = 1 edges from a config to itself)."""
     cfgs = _f32(cfgs).reshape(len(cfgs), -1)
-    return check_edges_f32(scene, MODE_FULL, bases, 1, cfgs, cfgs, threads=os.cpu_count() or 1)
+    mode = MODE_SINGLE if n_arms == 1 else MODE_FULL  # one arm: environment + self
+    return check_edges_f32(scene, mode, bases, 1, cfgs, cfgs, threads=os.cpu_count() or 1)
 
 
 def tables():

@@ -78,7 +78,7 @@ def _random_edges(rng, starts, n, dim, lo, hi):
     return frm, to
 
 
-@pytest.mark.parametrize("kind", ["plain2", "table3", "bin4", "shelf5"])
+@pytest.mark.parametrize("kind", ["plain2", "table3", "bin4", "shelf5", "posts3"])
 @pytest.mark.parametrize("G", [1, 16, 32, 64])
 def test_live_oracle_parity(ctx, kind, G):
     from oracle import oracle as O

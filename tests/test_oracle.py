@@ -361,6 +361,34 @@ def test_golden_vectors_reproduce():
             assert np.array_equal(r[k], z[k]), (f.name, k)
 
 
+def test_quirk_golden_distinguishes_the_reading():
+    """edges_panda_four_quirk_drrt_g16.npz: every edge ends at a robust
+    configuration whose verdict the reference's multi-arm fine-range reading
+    (panda_four.cuh:390-401) flips against the single-arm reading, and that
+    reading is reproduced; so a device that read the ranges the single-arm
+    way fails the golden comparison on every edge's last sample."""
+    z = np.load(GOLDEN / "edges_panda_four_quirk_drrt_g16.npz")
+    scene = scenes.Scene(z["spheres"], z["capsules"], z["z_capsules"], z["cuboids"], z["z_cuboids"])
+    last = np.uint64(1) << np.uint64(int(z["G"]) - 1)
+    assert ((z["mask_nom"] ^ z["mask_plain"]) & last != 0).all()
+    assert ((z["mask_lo"] ^ z["mask_hi"]) == 0).all()
+    r = O.check_edges_plain_ranges(scene, int(z["mode"]), z["bases"], int(z["G"]), z["frm"], z["to"])
+    assert np.array_equal(r["mask_nom"], z["mask_plain"])
+
+
+def test_zaligned_golden_holds_the_z_obstacles():
+    """The z-aligned golden scene holds z-capsules and z-cuboids that the
+    edges actually hit (a device ignoring them would miss collisions)."""
+    for name in ("edges_posts3_full_g32.npz", "edges_posts3_single_g64.npz"):
+        z = np.load(GOLDEN / name)
+        assert len(z["z_capsules"]) and len(z["z_cuboids"])
+        full = scenes.Scene(z["spheres"], z["capsules"], z["z_capsules"], z["cuboids"], z["z_cuboids"])
+        bare = scenes.Scene(z["spheres"], z["capsules"], np.zeros((0, 8), np.float32), z["cuboids"],
+                            np.zeros((0, 15), np.float32))
+        r = O.check_edges(bare, int(z["mode"]), z["bases"], int(z["G"]), z["frm"], z["to"], arm=int(z["arm"]))
+        assert ((z["mask_nom"] & ~r["mask_nom"]) != 0).sum() >= 20, name
+
+
 def test_edge_interpolation_excludes_start_includes_end():
     """Sample k = 1..G of an edge: a free start with a colliding end must
     collide at sample G only when the midway samples are free."""
