@@ -1,0 +1,884 @@
+// check_kernels.h -- batched FK + two-phase collision check of interpolated edges (gfx950).
+//
+// Replaces the reference's fkcc_single_buffer / _mr / _drrt
+// (src/planning/utils.cuh:447-686) with the same per-sample verdict:
+//   coarse pass  11 spheres per arm: env test of spheres 1..10 sets the env bit
+//                of the sphere's joint (panda_four.cuh:232-276); the 21
+//                intra-arm pairs and, for several arms, all 11x11 pairs of each
+//                arm pair a < b set the self bit of arm a's joint
+//                (panda_four.cuh:134-229);
+//   fine pass    59 spheres per arm: spheres 1..58 on env-flagged joints against
+//                the environment (panda_four.cuh:456-489); intra-arm pairs
+//                gated on the self bit (multi-arm: range i gated by and centred
+//                on fine sphere i with the radius of sphere ranges[i][0],
+//                panda_four.cuh:390-416; one arm: gated by sphere ranges[i][0],
+//                panda.cuh:832-862); inter-arm: sphere i of arm a on a
+//                self-flagged joint against all 59 spheres of every arm b > a
+//                (panda_four.cuh:419-450).
+//   A sample collides iff some fine test does; an edge iff some sample does.
+//
+// MI355X mapping.
+//   check_kernel<A, MODE>: a block of A waves checks a tile of 64 samples;
+//   wave a holds arm a of every sample (arm-major), so the obstacle and
+//   arm-pair culls of a wave see one arm of 64 neighbouring samples.  FK runs
+//   per item in registers with the fixed joint transforms folded in; coarse
+//   centres and a bounding ball per arm go to LDS, where the arm-arm pairs
+//   split evenly over the A items of a sample (each takes A-1 half-pairs:
+//   rows 0-5 or 6-10 of arm a against arm b; rows whose sphere misses arm b's
+//   ball are skipped).  Obstacle records are wave-uniform and come through
+//   the scalar cache; a per-item cull box skips obstacles far from the arm.
+//   The fine self, environment and arm-arm tests run in the same kernel,
+//   gated exactly as above, but only where they can hit: each coarse sphere
+//   grown by a few tenths of a millimetre (kBlockInflate) contains the fine
+//   spheres of its block, so a fine pair or obstacle is tested only if its
+//   block ball overlaps.  The culling never changes a verdict.  A tile needs
+//   three barriers; a persistent grid of resident blocks loops over the
+//   tiles of many edge groups (check_loop).
+//   check_kernel_mixed<MODE>: blocks of five waves, tiles of 320/A samples of
+//   the group's arm count, for planner rounds over problems of several arm
+//   counts.
+//
+// This header holds the device code.  Each kernel instantiation is compiled
+// in one object built from check_inst.hip (so the instantiations build in
+// parallel); check.hip holds the launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "device_geom.h"
+#include "internal.h"
+
+namespace mrpop {
+namespace dev {
+
+using u64 = unsigned long long;
+
+}  // namespace dev
+}  // namespace mrpop
+#include "check_groups.h"
+namespace mrpop {
+namespace dev {
+using EdgeBatch = EdgeGroup;
+
+// ---------------------------------------------------------------------------
+// Lane-indexed constant tables (offsets, radii, joints of the fine spheres).
+struct SphereTable {
+    float off[kFineSpheres][4];
+    float r[kFineSpheres];
+    unsigned char joint[kFineSpheres];
+    float coarse_r[kCoarseSpheres];
+    float block_r[kCoarseSpheres];  // coarse radius + inflation: ball holding the block's fine spheres
+    unsigned char block_first[kCoarseSpheres];
+    unsigned char block_count[kCoarseSpheres];
+};
+constexpr SphereTable build_sphere_table() {
+    SphereTable t{};
+    for (int i = 0; i < kFineSpheres; ++i) {
+        for (int c = 0; c < 4; ++c) t.off[i][c] = kFineSphere[i][c];
+        t.r[i] = kFineSphere[i][3];
+        t.joint[i] = (unsigned char)kFineJoint[i];
+    }
+    for (int i = 0; i < kCoarseSpheres; ++i) {
+        t.coarse_r[i] = kCoarseSphere[i][3];
+        t.block_r[i] = kCoarseSphere[i][3] + kBlockInflate[i];
+        t.block_first[i] = (unsigned char)kBlockFirst[i];
+        t.block_count[i] = (unsigned char)kBlockCount[i];
+    }
+    return t;
+}
+static __constant__ SphereTable kTab = build_sphere_table();
+
+constexpr bool coarse_joint_is_min_i_7() {
+    for (int i = 0; i < kCoarseSpheres; ++i)
+        if (kCoarseJoint[i] != (i < 7 ? i : 7)) return false;
+    return true;
+}
+static_assert(coarse_joint_is_min_i_7(), "coarse sphere i belongs to joint min(i, 7)");
+constexpr bool blocks_follow_joints() {
+    for (int k = 0; k < kCoarseSpheres; ++k)
+        for (int i = 0; i < kBlockCount[k]; ++i)
+            if (kFineJoint[kBlockFirst[k] + i] != kCoarseJoint[k]) return false;
+    return kBlockFirst[5] == 17 && kBlockFirst[10] + kBlockCount[10] == kFineSpheres;
+}
+static_assert(blocks_follow_joints(), "fine-sphere blocks lie on their coarse sphere's joint");
+
+// Fine intra-arm ranges (src/robots/panda_four.cuh:388-417, panda.cuh:832-862):
+// the sphere whose centre is tested, the sphere whose radius is used, the
+// joint whose self bit gates the range.
+template <int MODE> constexpr int range_pos(int r) {
+    return MODE == MRPOP_MODE_SINGLE ? kFineSelfRange[r][0] : r;  // multi-arm: fine sphere r itself
+}
+constexpr int range_rad(int r) { return kFineSelfRange[r][0]; }
+template <int MODE> constexpr int range_gate(int r) { return kFineJoint[range_pos<MODE>(r)]; }
+
+// ------------------------------------------------------------- FK helpers --
+__device__ __forceinline__ void load_base(const WorldDesc &w, int arm, Frame &T) {
+    const float *b = w.obs + w.o_base + 12 * arm;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) T.m[i] = b[i];
+}
+
+// (to - from) / G.  For G a power of two the quotient is exact, so it
+// equals the product with 1/G (one instruction instead of a correctly
+// rounded division); G is uniform over a wave tile.
+__device__ __forceinline__ float step_of(float t, float f, int G) {
+    if ((G & (G - 1)) == 0) return (t - f) * __builtin_amdgcn_ldexpf(1.0f, -__builtin_ctz(G));
+    return (t - f) / (float)G;
+}
+
+// Interpolated joint angles of one arm: q = from + k * ((to - from) / G)
+// (pop_planner.cu:553-560).
+__device__ __forceinline__ void interp7(const EdgeBatch &eb, int e, int k, int off, float q[7]) {
+    const float *f = eb.from + (size_t)e * eb.dim + off;
+    const float *t = eb.to + (size_t)e * eb.dim + off;
+    const float kf = (float)k;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) q[j] = __builtin_fmaf(kf, step_of(t[j], f[j], eb.G), f[j]);
+}
+
+// Joint transforms for a rolled (runtime-depth) FK walk.
+struct JointTable {
+    float tf[kJoints][12];
+};
+constexpr JointTable build_joint_table() {
+    JointTable t{};
+    for (int j = 0; j < kJoints; ++j)
+        for (int i = 0; i < 12; ++i) t.tf[j][i] = kJointTf[j][i];
+    return t;
+}
+static __constant__ JointTable kJT = build_joint_table();
+
+// Frame of joint `jt` (runtime 0..7) of an arm at sample (e, k), walked with a
+// rolled loop.  Every operation matches joint_step on the same inputs (fma
+// with a 0 / +-1 factor is exact), so the frame equals the unrolled one.
+__device__ __forceinline__ Frame frame_rt(const WorldDesc &w, const EdgeBatch &eb, int base_arm, int dof_off, int e,
+                                       int k, int jt) {
+    Frame T;
+    load_base(w, base_arm, T);
+    const float *f = eb.from + (size_t)e * eb.dim + dof_off;
+    const float *t = eb.to + (size_t)e * eb.dim + dof_off;
+    const float kf = (float)k;
+#pragma unroll 1
+    for (int J = 1; J <= jt; ++J) {
+        const float qj = __builtin_fmaf(kf, step_of(t[J - 1], f[J - 1], eb.G), f[J - 1]);
+        float sq, cq;
+        joint_sincos(qj, sq, cq);
+        const float *F = kJT.tf[J];
+        float S[12];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            S[4 * r + 0] = __builtin_fmaf(F[4 * r + 1], sq, F[4 * r + 0] * cq);
+            S[4 * r + 1] = __builtin_fmaf(-F[4 * r + 0], sq, F[4 * r + 1] * cq);
+        }
+        Frame N;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                float v = T.m[4 * r + 0] * S[c];
+                v = __builtin_fmaf(T.m[4 * r + 1], S[4 + c], v);
+                v = __builtin_fmaf(T.m[4 * r + 2], S[8 + c], v);
+                N.m[4 * r + c] = v;
+            }
+            N.m[4 * r + 2] = __builtin_fmaf(F[10], T.m[4 * r + 2], __builtin_fmaf(F[6], T.m[4 * r + 1], F[2] * T.m[4 * r + 0]));
+            N.m[4 * r + 3] = __builtin_fmaf(F[11], T.m[4 * r + 2],
+                                            __builtin_fmaf(F[7], T.m[4 * r + 1], __builtin_fmaf(F[3], T.m[4 * r + 0], T.m[4 * r + 3])));
+        }
+        T = N;
+    }
+    return T;
+}
+
+__device__ __forceinline__ void centre_rt(const Frame &T, int s, float &x, float &y, float &z) {
+    centre(T, kTab.off[s][0], kTab.off[s][1], kTab.off[s][2], x, y, z);
+}
+
+// ------------------------------------------------------------ fused check --
+// Fine environment tests of block k's spheres (all but sphere 0) in frame T,
+// only against obstacles that its bounding ball (centre b, radius R) touches.
+__device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, int k, float bx, float by, float bz,
+                                          float R) {
+    bool hit = false;
+    const int j0 = max((int)kTab.block_first[k], 1), j1 = kTab.block_first[k] + kTab.block_count[k];
+    const float Rsq = R * R;
+    for (int i = 0; i < w.n_sph; ++i) {
+        const float *o = w.obs + w.o_sph + 4 * i;
+        if (sphere_sphere(o[0], o[1], o[2], o[3], bx, by, bz, R) < 0.f)
+            for (int j = j0; j < j1; ++j) {
+                float x, y, z;
+                centre_rt(T, j, x, y, z);
+                hit |= sphere_sphere(o[0], o[1], o[2], o[3], x, y, z, kTab.r[j]) < 0.f;
+            }
+    }
+    for (int i = 0; i < w.n_cap; ++i) {
+        const float *o = w.obs + w.o_cap + 8 * i;
+        if (sphere_capsule(o, bx, by, bz, R) < 0.f)
+            for (int j = j0; j < j1; ++j) {
+                float x, y, z;
+                centre_rt(T, j, x, y, z);
+                hit |= sphere_capsule(o, x, y, z, kTab.r[j]) < 0.f;
+            }
+    }
+    for (int i = 0; i < w.n_zcap; ++i) {
+        const float *o = w.obs + w.o_zcap + 8 * i;
+        if (sphere_zcapsule(o, bx, by, bz, R) < 0.f)
+            for (int j = j0; j < j1; ++j) {
+                float x, y, z;
+                centre_rt(T, j, x, y, z);
+                hit |= sphere_zcapsule(o, x, y, z, kTab.r[j]) < 0.f;
+            }
+    }
+    for (int i = 0; i < w.n_aabb; ++i) {
+        const float *o = w.obs + w.o_aabb + 6 * i;
+        if (sphere_aabb(o, bx, by, bz, Rsq) < 0.f)
+            for (int j = j0; j < j1; ++j) {
+                float x, y, z;
+                centre_rt(T, j, x, y, z);
+                hit |= sphere_aabb(o, x, y, z, kTab.r[j] * kTab.r[j]) < 0.f;
+            }
+    }
+    for (int i = 0; i < w.n_obb; ++i) {
+        const float *o = w.obs + w.o_obb + 15 * i;
+        if (sphere_obb(o, bx, by, bz, Rsq) < 0.f)
+            for (int j = j0; j < j1; ++j) {
+                float x, y, z;
+                centre_rt(T, j, x, y, z);
+                hit |= sphere_obb(o, x, y, z, kTab.r[j] * kTab.r[j]) < 0.f;
+            }
+    }
+    for (int i = 0; i < w.n_zcub; ++i) {
+        const float *o = w.obs + w.o_zcub + 15 * i;
+        if (sphere_zcuboid(o, bx, by, bz, Rsq) < 0.f)
+            for (int j = j0; j < j1; ++j) {
+                float x, y, z;
+                centre_rt(T, j, x, y, z);
+                hit |= sphere_zcuboid(o, x, y, z, kTab.r[j] * kTab.r[j]) < 0.f;
+            }
+    }
+    return hit;
+}
+
+// One FK walk of this lane's arm doing, exactly as the reference gates them:
+//   * the fine intra-arm ranges whose gate joint lies in [G0, G1], with
+//     bounding-ball culling of the target blocks;
+//   * if WITH_ENV, the fine environment tests of every env-flagged joint,
+//     with bounding-ball culling of the obstacles.
+// Run once for gates on joints 0..4 and once for joint 5 plus the
+// environment, so that at most 17 tested centres are live at a time.
+template <int MODE, int G0, int G1, bool WITH_ENV, int NT>
+__device__ __forceinline__ bool fine_walk(const WorldDesc &w, const Frame &base, const float q[7], unsigned selfbits,
+                                          unsigned envbits, const float (*cs)[3][NT], int lane) {
+    bool hit = false;
+    float px[kFineSelfRanges], py[kFineSelfRanges], pz[kFineSelfRanges];
+    Frame T = base;
+    static_for<0, kJoints>([&](auto jc) {
+        constexpr int J = decltype(jc)::value;
+        if constexpr (J > 0) {
+            Frame N;
+            joint_frame<J>(T, q[J - 1], N);
+            T = N;
+        }
+        // centres of the ranges' tested spheres lying on joint J
+        if constexpr (J >= G0 && J <= G1) {
+            if (selfbits & (1u << J)) {
+                static_for<0, kFineSelfRanges>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    constexpr int ps = range_pos<MODE>(r);
+                    if constexpr (kFineJoint[ps] == J) {
+                        constexpr float ox = kFineSphere[ps][0], oy = kFineSphere[ps][1], oz = kFineSphere[ps][2];
+                        centre(T, ox, oy, oz, px[r], py[r], pz[r]);
+                    }
+                });
+            }
+        }
+        // fine environment tests of joint J (sphere 0 is never tested: panda_four.cuh:468)
+        if constexpr (WITH_ENV && J > 0) {
+            if (envbits & (1u << J)) {
+                static_for<1, kCoarseSpheres>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    if constexpr (kCoarseJoint[k] == J) {
+                        constexpr float R = kCoarseSphere[k][3] + kBlockInflate[k];
+                        hit |= block_env(w, T, k, cs[k][0][lane], cs[k][1][lane], cs[k][2][lane], R);
+                    }
+                });
+            }
+        }
+        // target blocks on joint J (targets are fine spheres 17..58: blocks 5..10)
+        static_for<5, kCoarseSpheres>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            if constexpr (kCoarseJoint[k] == J) {
+                const float bx = cs[k][0][lane], by = cs[k][1][lane], bz = cs[k][2][lane];
+                constexpr int b0 = kBlockFirst[k], b1 = kBlockFirst[k] + kBlockCount[k] - 1;
+                constexpr float R = kCoarseSphere[k][3] + kBlockInflate[k];
+                static_for<0, kFineSelfRanges>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    constexpr int lo = kFineSelfRange[r][1] > b0 ? kFineSelfRange[r][1] : b0;
+                    constexpr int hi = kFineSelfRange[r][2] < b1 ? kFineSelfRange[r][2] : b1;
+                    constexpr int gj = range_gate<MODE>(r);
+                    if constexpr (lo <= hi && gj >= G0 && gj <= G1) {
+                        constexpr float rad = kFineSphere[range_rad(r)][3];
+                        if (selfbits & (1u << gj)) {
+                            const float dx = px[r] - bx, dy = py[r] - by, dz = pz[r] - bz;
+                            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                            if (d2 < (rad + R) * (rad + R)) {
+                                for (int t = lo; t <= hi; ++t) {
+                                    float x, y, z;
+                                    centre_rt(T, t, x, y, z);
+                                    hit |= sphere_sphere(px[r], py[r], pz[r], rad, x, y, z, kTab.r[t]) < 0.f;
+                                }
+                            }
+                        }
+                    }
+                });
+            }
+        });
+    });
+    return hit;
+}
+
+// Decode half-pair h (0 .. A(A-1)-1) into arm pair (a, b), a < b, and the
+// first row i0 (0 or 6) of arm a's coarse spheres it covers.
+template <int A> __device__ __forceinline__ void half_pair(int h, int &a, int &b, int &i0) {
+    const int p = h >> 1;
+    int aa = 0, rem = p;
+#pragma unroll
+    for (int t = 0; t < A - 1; ++t) {
+        const int row = A - 1 - t;
+        if (aa == t && rem >= row) {
+            rem -= row;
+            aa = t + 1;
+        }
+    }
+    a = aa;
+    b = aa + 1 + rem;
+    i0 = (h & 1) ? 6 : 0;
+}
+
+// Largest radius among the environment-tested coarse spheres 1..10.
+constexpr float env_radius_max() {
+    float r = 0.f;
+    for (int s = 1; s < kCoarseSpheres; ++s) r = kCoarseSphere[s][3] > r ? kCoarseSphere[s][3] : r;
+    return r;
+}
+constexpr float kEnvRadiusMax = env_radius_max();
+
+// Largest block radius (coarse radius + inflation) over the coarse spheres,
+// and the slack added to arm balls: 0.1 mm, orders of magnitude above the
+// float32 rounding of distances of a few metres.
+constexpr float max_block_radius() {
+    float r = 0.f;
+    for (int s = 0; s < kCoarseSpheres; ++s)
+        r = kCoarseSphere[s][3] + kBlockInflate[s] > r ? kCoarseSphere[s][3] + kBlockInflate[s] : r;
+    return r;
+}
+constexpr float kMaxBlockRadius = max_block_radius();
+constexpr float kBallMargin = 1e-4f;
+
+// Shared memory of one tile of NT items (sample, arm); KH = arm-arm
+// half-pairs per item.
+template <int KH, int NT> struct TileLds {
+    float cs[kCoarseSpheres][3][NT];
+    float ball[4][NT];                   // per item: its arm's bounding ball (centre, radius)
+    unsigned inter_flags[NT];
+    unsigned result[NT];
+    unsigned long long cand_lo[KH][NT];  // candidate blocks, rows 0..4: bit t*11 + k
+    unsigned cand_hi[KH][NT];            // row 5: bit k
+    unsigned edge_done[NT];              // per edge of the tile: a sample collides
+    unsigned fine[NT];                   // per item: its fine tests found a collision
+};
+
+// Samples per edge in a launch of the given pass.  Pass 0 checks every
+// sample k = 1..G; a probe pass P > 0 checks min(P, G) of them spread along
+// the edge, the last being k = G; pass -1 checks every sample of the edges
+// a probe pass left free.  The edge verdict, the OR over its samples, is
+// the same either way.
+__device__ __forceinline__ int pass_samples(const EdgeGroup &g, int pass) {
+    return pass > 0 ? min(pass, g.G) : g.G;
+}
+
+// Tiles of a group: a tile is W waves, 64 W / A samples of all A arms of
+// the group (one arm in SINGLE mode).
+__device__ __forceinline__ unsigned group_waves(const EdgeGroup &g, int single, int pass, int W) {
+    const int S = single ? 64 * W : (64 * W) / (g.dim / 7);
+    const unsigned long long samples = (unsigned long long)(g.count ? *g.count : g.n_edges) * pass_samples(g, pass);
+    return (unsigned)((samples + S - 1) / S);
+}
+
+// One tile of NT / 64 waves: samples lw*S .. lw*S + S - 1 of group eb,
+// S = NT / A, thread arm * S + slot holding arm `arm` of sample `slot`.  The
+// waves are arm-major, so the environment and arm-pair culls of a wave see
+// one arm of 64 nearby samples.
+template <int A, int MODE, int KH, int NT>
+__device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc *worlds, const EdgeGroup &eb,
+                                           unsigned lw, const FineQueue &fq, int pass) {
+    constexpr int S = NT / A;  // samples per tile
+    constexpr bool kEnv = MODE != MRPOP_MODE_DRRT;
+    constexpr bool kInter = A > 1;
+    const int lane = threadIdx.x;  // the thread's own item
+    const bool item = lane < S * A;
+    // spare threads (NT % A) shadow arm 0 of sample 0 and discard
+    const int arm = item ? lane / S : 0;
+    const int slot = item ? lane - arm * S : 0;
+    const WorldDesc w = worlds[eb.world];
+    const int per = pass_samples(eb, pass);
+    const long total = (long)(eb.count ? *eb.count : eb.n_edges) * per;
+    const long n = (long)lw * S + slot;
+    bool active = item && n < total;
+    const long nn = active ? n : 0;
+    const int e = (int)(nn / per);
+    const int j = (int)(nn - (long)e * per);
+    const int k = per == eb.G ? j + 1 : ((j + 1) * eb.G) / per;  // sample k of k = 1..G
+    if (pass < 0) {
+        // the remaining samples of edges still free after the probe pass
+        if (active && eb.out_edge[e] != 0u) active = false;
+        if (!__syncthreads_or(active)) return;
+    }
+    const int base_arm = MODE == MRPOP_MODE_SINGLE ? eb.arm : arm;
+    const int dof_off = MODE == MRPOP_MODE_SINGLE ? 0 : 7 * arm;
+
+    // ---- A. coarse FK of this lane's arm
+    float cx[kCoarseSpheres], cy[kCoarseSpheres], cz[kCoarseSpheres];
+    {
+        float q[7];
+        interp7(eb, e, k, dof_off, q);
+        Frame T;
+        load_base(w, base_arm, T);
+        {
+            constexpr float ox = kCoarseSphere[0][0], oy = kCoarseSphere[0][1], oz = kCoarseSphere[0][2];
+            centre(T, ox, oy, oz, cx[0], cy[0], cz[0]);
+        }
+        static_for<1, kJoints>([&](auto jc) {
+            constexpr int J = decltype(jc)::value;
+            Frame N;
+            joint_frame<J>(T, q[J - 1], N);
+            T = N;
+            static_for<kCoarseJointStart[J], kCoarseJointStart[J] + kCoarseJointCount[J]>([&](auto sc) {
+                constexpr int s = decltype(sc)::value;
+                constexpr float ox = kCoarseSphere[s][0], oy = kCoarseSphere[s][1], oz = kCoarseSphere[s][2];
+                centre(T, ox, oy, oz, cx[s], cy[s], cz[s]);
+            });
+        });
+    }
+#pragma unroll
+    for (int s = 0; s < kCoarseSpheres; ++s) {
+        sm.cs[s][0][lane] = cx[s];
+        sm.cs[s][1][lane] = cy[s];
+        sm.cs[s][2][lane] = cz[s];
+    }
+    sm.inter_flags[lane] = 0u;
+    if constexpr (kInter) {
+        // A ball around this arm's coarse block balls: centre = mean of the
+        // centres, radius = largest centre distance + largest block radius +
+        // a margin far above float rounding.  A sphere whose block ball
+        // misses it can neither hit nor be a candidate of any of the arm's
+        // spheres in the arm-arm stage.
+        float mx = cx[0], my = cy[0], mz = cz[0];
+#pragma unroll
+        for (int s = 1; s < kCoarseSpheres; ++s) {
+            mx += cx[s];
+            my += cy[s];
+            mz += cz[s];
+        }
+        constexpr float inv = 1.0f / kCoarseSpheres;
+        mx *= inv;
+        my *= inv;
+        mz *= inv;
+        float r2 = 0.f;
+#pragma unroll
+        for (int s = 0; s < kCoarseSpheres; ++s) {
+            const float dx = cx[s] - mx, dy = cy[s] - my, dz = cz[s] - mz;
+            r2 = fmaxf(r2, __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)));
+        }
+        sm.ball[0][lane] = mx;
+        sm.ball[1][lane] = my;
+        sm.ball[2][lane] = mz;
+        sm.ball[3][lane] = sqrtf(r2) + kMaxBlockRadius + kBallMargin;
+    }
+
+    // coarse environment: spheres 1..10 -> env bit of their joint (panda_four.cuh:241-261)
+    unsigned hitmask = 0;
+    if constexpr (kEnv) {
+        // Box of this lane's tested centres (spheres 1..10): an obstacle whose
+        // cull box, grown by the largest of their radii, misses it cannot hit
+        // any of them.
+        float lx = cx[1], ly = cy[1], lz = cz[1], hx = cx[1], hy = cy[1], hz = cz[1];
+#pragma unroll
+        for (int s = 2; s < kCoarseSpheres; ++s) {
+            lx = fminf(lx, cx[s]);
+            ly = fminf(ly, cy[s]);
+            lz = fminf(lz, cz[s]);
+            hx = fmaxf(hx, cx[s]);
+            hy = fmaxf(hy, cy[s]);
+            hz = fmaxf(hz, cz[s]);
+        }
+        const float *box = w.obs + w.o_box;
+        auto near = [&](int ob) {
+            const float *bb = box + 8 * ob;
+            const float g = kEnvRadiusMax * bb[6];
+            return !(hx < bb[0] - g || hy < bb[1] - g || hz < bb[2] - g || lx > bb[3] + g || ly > bb[4] + g ||
+                     lz > bb[5] + g);
+        };
+        auto test_all = [&](auto &&fn) {
+            static_for<1, kCoarseSpheres>([&](auto sc) {
+                constexpr int s = decltype(sc)::value;
+                constexpr float r = kCoarseSphere[s][3];
+                if (fn(cx[s], cy[s], cz[s], r) < 0.f) hitmask |= 1u << s;
+            });
+        };
+        int ob = 0;
+        for (int i = 0; i < w.n_sph; ++i, ++ob) {
+            if (!near(ob)) continue;
+            const float *o = w.obs + w.o_sph + 4 * i;
+            const float ox = o[0], oy = o[1], oz = o[2], orr = o[3];
+            test_all([&](float x, float y, float z, float r) { return sphere_sphere(ox, oy, oz, orr, x, y, z, r); });
+        }
+        for (int i = 0; i < w.n_cap; ++i, ++ob) {
+            if (!near(ob)) continue;
+            const float *o = w.obs + w.o_cap + 8 * i;
+            test_all([&](float x, float y, float z, float r) { return sphere_capsule(o, x, y, z, r); });
+        }
+        for (int i = 0; i < w.n_zcap; ++i, ++ob) {
+            if (!near(ob)) continue;
+            const float *o = w.obs + w.o_zcap + 8 * i;
+            test_all([&](float x, float y, float z, float r) { return sphere_zcapsule(o, x, y, z, r); });
+        }
+        for (int i = 0; i < w.n_aabb; ++i, ++ob) {
+            if (!near(ob)) continue;
+            const float *o = w.obs + w.o_aabb + 6 * i;
+            test_all([&](float x, float y, float z, float r) { return sphere_aabb(o, x, y, z, r * r); });
+        }
+        for (int i = 0; i < w.n_obb; ++i, ++ob) {
+            if (!near(ob)) continue;
+            const float *o = w.obs + w.o_obb + 15 * i;
+            test_all([&](float x, float y, float z, float r) { return sphere_obb(o, x, y, z, r * r); });
+        }
+        for (int i = 0; i < w.n_zcub; ++i, ++ob) {
+            if (!near(ob)) continue;
+            const float *o = w.obs + w.o_zcub + 15 * i;
+            test_all([&](float x, float y, float z, float r) { return sphere_zcuboid(o, x, y, z, r * r); });
+        }
+    }
+    unsigned envbits = 0;
+    static_for<1, kCoarseSpheres>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr unsigned bit = 1u << kCoarseJoint[s];
+        if (hitmask & (1u << s)) envbits |= bit;
+    });
+
+    // coarse intra-arm pairs -> self bit of the first sphere's joint (panda_four.cuh:144-176)
+    unsigned selfbits = 0;
+    static_for<0, kCoarseSelfRanges>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        constexpr int s1 = kCoarseSelfRange[r][0];
+        constexpr float r1 = kCoarseSphere[s1][3];
+        static_for<kCoarseSelfRange[r][1], kCoarseSelfRange[r][2] + 1>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            constexpr float rj = kCoarseSphere[j][3];
+            constexpr unsigned bit = 1u << kCoarseJoint[s1];
+            if (sphere_sphere(cx[s1], cy[s1], cz[s1], r1, cx[j], cy[j], cz[j], rj) < 0.f) selfbits |= bit;
+        });
+    });
+    // Barriers of a tile: (1) coarse centres, balls and intra-arm self bits
+    // published, (2) arm-arm flags complete, (3) fine results published.
+    // Every LDS entry is written by its own item before (1), or between (1)
+    // and (3) with all readers before (3); so a wave that has passed (3) may
+    // start the next tile while others finish this one.
+    sm.result[lane] = selfbits;  // intra-arm part of the self bits
+    sm.edge_done[lane] = 0u;
+    __syncthreads();
+
+    // coarse arm-arm pairs (panda_four.cuh:178-214), A-1 half-pairs per lane.
+    // The same distances give the candidate blocks of the fine arm-arm check:
+    // blocks p of arm a and k of arm b whose bounding balls overlap.
+    if constexpr (kInter) {
+#pragma unroll
+        for (int m = 0; m < A - 1; ++m) {
+            int a, b, i0;
+            half_pair<A>(arm + m * A, a, b, i0);
+            // rows whose block ball meets arm b's ball; rows no lane of the
+            // wave needs are skipped (their tests cannot hit)
+            unsigned need = 0;
+            {
+                const float ux = sm.ball[0][b * S + slot], uy = sm.ball[1][b * S + slot], uz = sm.ball[2][b * S + slot];
+                const float ur = sm.ball[3][b * S + slot];
+#pragma unroll
+                for (int t = 0; t < 6; ++t) {
+                    const int i = min(i0 + t, kCoarseSpheres - 1);
+                    const float dx = sm.cs[i][0][a * S + slot] - ux, dy = sm.cs[i][1][a * S + slot] - uy,
+                                dz = sm.cs[i][2][a * S + slot] - uz;
+                    const float lim = ur + kTab.block_r[i];
+                    if (__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) < lim * lim) need |= 1u << t;
+                }
+            }
+            if (!active) need = 0;
+            const unsigned long long rows_any = __ballot(need != 0u);
+            if (!rows_any) {
+                sm.cand_lo[m][lane] = 0ull;
+                sm.cand_hi[m][lane] = 0u;
+                continue;
+            }
+            float bx[kCoarseSpheres], by[kCoarseSpheres], bz[kCoarseSpheres];
+#pragma unroll
+            for (int j = 0; j < kCoarseSpheres; ++j) {
+                bx[j] = sm.cs[j][0][b * S + slot];
+                by[j] = sm.cs[j][1][b * S + slot];
+                bz[j] = sm.cs[j][2][b * S + slot];
+            }
+            unsigned mask = 0;
+            unsigned long long clo = 0;
+            unsigned chi = 0;
+#pragma unroll 1
+            for (int t = 0; t < 6; ++t) {
+                if (!__ballot((need >> t) & 1u)) continue;
+                const int i = min(i0 + t, kCoarseSpheres - 1);  // rows 6..10 are 5: repeat row 10
+                const float ax = sm.cs[i][0][a * S + slot], ay = sm.cs[i][1][a * S + slot], az = sm.cs[i][2][a * S + slot];
+                const float rad = kTab.coarse_r[i], brad = kTab.block_r[i];
+                bool hit = false;
+                unsigned cand = 0;
+                static_for<0, kCoarseSpheres>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    constexpr float rj = kCoarseSphere[j][3], Rj = kCoarseSphere[j][3] + kBlockInflate[j];
+                    const float dx = ax - bx[j], dy = ay - by[j], dz = az - bz[j];
+                    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                    const float rs = rad + rj;
+                    const float Rs = brad + Rj;
+                    hit |= d2 - rs * rs < 0.f;
+                    if (d2 < Rs * Rs) cand |= 1u << j;
+                });
+                if (hit) mask |= 1u << (i < 7 ? i : 7);
+                if (t < 5)
+                    clo |= (unsigned long long)cand << (11 * t);
+                else if (i0 == 0)
+                    chi = cand;
+            }
+            sm.cand_lo[m][lane] = active ? clo : 0ull;
+            sm.cand_hi[m][lane] = active ? chi : 0u;
+            if (mask && item) atomicOr(&sm.inter_flags[a * S + slot], mask);
+        }
+    }
+    __syncthreads();
+    if constexpr (kInter) selfbits |= sm.inter_flags[lane];
+
+    // Edge-only verdicts (no per-sample mask, no diagnostics): once one
+    // sample of an edge collides, the fine tests left for that edge's lanes
+    // cannot change its verdict and are skipped.  A tile holds the samples
+    // of at most S consecutive edges; le is this lane's edge within it.
+    const bool early_out = !eb.out_mask && !fq.diag;
+    const int le = e - (int)(((long)lw * S) / per);
+    volatile unsigned *done = sm.edge_done;
+
+    // ---- B. fine intra-arm pairs of this lane's arm
+    bool finehit = false;
+    if (active && (selfbits | envbits)) {
+        float q[7];
+        interp7(eb, e, k, dof_off, q);
+        Frame base;
+        load_base(w, base_arm, base);
+        if (selfbits & 0x1fu) finehit = fine_walk<MODE, 0, 4, false, NT>(w, base, q, selfbits, 0u, sm.cs, lane);
+        if ((selfbits & 0x20u) || envbits)
+            finehit |= fine_walk<MODE, 5, 5, kEnv, NT>(w, base, q, selfbits, envbits, sm.cs, lane);
+    }
+    if (active && finehit) done[le] = 1u;
+
+    // ---- C. fine arm-arm pairs for candidate blocks of this lane's half-pairs
+    if constexpr (kInter) {
+#pragma unroll 1
+        for (int m = 0; m < A - 1; ++m) {
+            if (early_out && active && done[le]) break;
+            int a, b, i0;
+            half_pair<A>(arm + m * A, a, b, i0);
+            const unsigned gate = sm.result[a * S + slot] | sm.inter_flags[a * S + slot];  // arm a's self bits
+            unsigned long long clo = sm.cand_lo[m][lane];
+            unsigned chi = sm.cand_hi[m][lane];
+            // keep rows whose block's joint is self-flagged on arm a
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                const int p = i0 + t;
+                if (!((gate >> (p < 7 ? p : 7)) & 1u)) clo &= ~(0x7ffull << (11 * t));
+            }
+            if (!((gate >> 5) & 1u)) chi = 0;  // row 5 exists only for i0 == 0 (block 5)
+            while (clo | chi) {
+                int t, kb;
+                if (clo) {
+                    const int bit = __builtin_ctzll(clo);
+                    clo &= clo - 1;
+                    t = bit / 11;
+                    kb = bit - 11 * t;
+                } else {
+                    kb = __builtin_ctz(chi);
+                    chi &= chi - 1;
+                    t = 5;
+                }
+                const int p = i0 + t;
+                const Frame Ta = frame_rt(w, eb, a, 7 * a, e, k, p < 7 ? p : 7);
+                const Frame Tb = frame_rt(w, eb, b, 7 * b, e, k, kb < 7 ? kb : 7);
+                const int p0 = kTab.block_first[p], pn = kTab.block_count[p];
+                const int k0 = kTab.block_first[kb], kn = kTab.block_count[kb];
+                for (int i = p0; i < p0 + pn; ++i) {
+                    float ax, ay, az;
+                    centre_rt(Ta, i, ax, ay, az);
+                    const float ar = kTab.r[i];
+                    for (int j = k0; j < k0 + kn; ++j) {
+                        float x, y, z;
+                        centre_rt(Tb, j, x, y, z);
+                        finehit |= sphere_sphere(ax, ay, az, ar, x, y, z, kTab.r[j]) < 0.f;
+                    }
+                }
+                if (finehit) done[le] = 1u;
+                if (early_out && done[le]) break;
+            }
+        }
+    }
+
+    // ---- D. per-sample verdict: OR over the sample's lanes
+    if (fq.diag && active)
+        fq.diag[(size_t)n * 8 + arm] = envbits | (selfbits << 8) | (finehit ? 1u << 16 : 0u) | (1u << 17);
+    sm.fine[lane] = finehit ? 1u : 0u;
+    __syncthreads();
+    if (active && arm == 0) {
+        bool coll = false;
+#pragma unroll
+        for (int a = 0; a < A; ++a) coll |= sm.fine[a * S + slot] != 0u;
+        if (coll) {
+            atomicOr(&eb.out_edge[e], 1u);
+            if (eb.out_mask) atomicOr(&eb.out_mask[e], 1ull << (k - 1));
+        }
+        if (fq.diag) fq.diag[(size_t)n * 8 + 7] = (coll ? 1u : 0u) | (1u << 17);
+    }
+}
+
+// Persistent loop over the wave tiles of n_groups edge groups (or of the
+// single group g0 when groups is null).  Up to 64 groups the wave computes
+// the groups' first waves itself, one group per lane (wave_start is then
+// null); more groups come with a prefix from wave_prefix_kernel.  A == 0:
+// groups of different arm counts (multi-arm modes), dispatched per tile.
+// Persistent loop of a block of W waves over the tiles of the groups.  Every
+// wave computes the same group lookup.
+template <int MODE, int W, class Tile>
+__device__ __forceinline__ void check_loop(const EdgeGroup *groups, const EdgeGroup &g0, int n_groups,
+                                           const unsigned *wave_start, int pass, Tile &&tile) {
+    const int lane = threadIdx.x & 63;
+    unsigned lane_start = 0, total_waves;
+    if (!groups) {
+        total_waves = group_waves(g0, MODE == MRPOP_MODE_SINGLE, pass, W);
+    } else if (!wave_start) {
+        const unsigned wcount =
+            lane < n_groups ? group_waves(groups[lane], MODE == MRPOP_MODE_SINGLE, pass, W) : 0u;
+        unsigned incl = wcount;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+        }
+        lane_start = incl - wcount;
+        total_waves = __shfl(incl, 63);
+    } else {
+        total_waves = wave_start[n_groups];
+    }
+    for (unsigned wv = blockIdx.x; wv < total_waves; wv += gridDim.x) {
+        int gi = 0;
+        unsigned gstart = 0;
+        if (groups && !wave_start) {  // last group whose first wave is <= wv (empty groups share starts)
+            const unsigned long long m = __ballot(lane < n_groups && lane_start <= wv);
+            gi = 63 - __builtin_clzll(m);
+            gstart = __shfl(lane_start, gi);
+        } else if (groups) {
+            int lo = 0, hi = n_groups - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (wave_start[mid] <= wv) lo = mid;
+                else hi = mid - 1;
+            }
+            gi = lo;
+            gstart = wave_start[gi];
+        }
+        const EdgeGroup eb = groups ? groups[gi] : g0;
+        tile(eb, wv - gstart);
+    }
+}
+
+// A waves per block, one per arm; three waves per SIMD: the register budget
+// (168 VGPRs) costs a few spills but hides more latency.
+template <int A, int MODE>
+__global__ __launch_bounds__(64 * A, 3) void check_kernel(const WorldDesc *worlds, const EdgeGroup *groups,
+                                                       EdgeGroup g0, int n_groups, const unsigned *wave_start,
+                                                       FineQueue fq, int pass) {
+    constexpr int KH = A > 1 ? A - 1 : 1;
+    __shared__ TileLds<KH, 64 * A> sm;
+    check_loop<MODE, A>(groups, g0, n_groups, wave_start, pass, [&](const EdgeGroup &eb, unsigned lw) {
+        check_tile<A, MODE, KH, 64 * A>(sm, worlds, eb, lw, fq, pass);
+    });
+}
+
+// Multi-arm groups of 2..5 arms in one launch: each tile runs the code of
+// its group's arm count, so a planner round checks all its problems in one
+// launch per mode instead of one per arm count.
+template <int MODE>
+__global__ __launch_bounds__(64 * kMixedWaves, 3) void check_kernel_mixed(const WorldDesc *worlds,
+                                                                       const EdgeGroup *groups, int n_groups,
+                                                                       const unsigned *wave_start, int pass) {
+    constexpr int NT = 64 * kMixedWaves;
+    __shared__ TileLds<4, NT> sm;
+    const EdgeGroup none{};
+    const FineQueue fq{nullptr};
+    check_loop<MODE, kMixedWaves>(groups, none, n_groups, wave_start, pass, [&](const EdgeGroup &eb, unsigned lw) {
+        switch (eb.dim / 7) {
+            case 2: check_tile<2, MODE, 4, NT>(sm, worlds, eb, lw, fq, pass); break;
+            case 3: check_tile<3, MODE, 4, NT>(sm, worlds, eb, lw, fq, pass); break;
+            case 4: check_tile<4, MODE, 4, NT>(sm, worlds, eb, lw, fq, pass); break;
+            default: check_tile<5, MODE, 4, NT>(sm, worlds, eb, lw, fq, pass); break;
+        }
+    });
+}
+
+}  // namespace dev
+}  // namespace mrpop
+
+// ------------------------------------------------------- launch wrappers --
+namespace mrpop {
+namespace dev {
+
+template <int A, int MODE>
+void launch_check_kernel(int blocks, hipStream_t stream, const WorldDesc *worlds, const EdgeGroup *groups,
+                         const EdgeGroup &g0, int n_groups, const unsigned *wave_start, const FineQueue &fq,
+                         int pass) {
+    hipLaunchKernelGGL((check_kernel<A, MODE>), dim3(blocks), dim3(64 * A), 0, stream, worlds, groups, g0, n_groups,
+                       wave_start, fq, pass);
+}
+
+template <int MODE>
+void launch_check_kernel_mixed(int blocks, hipStream_t stream, const WorldDesc *worlds, const EdgeGroup *groups,
+                               int n_groups, const unsigned *wave_start, int pass) {
+    hipLaunchKernelGGL((check_kernel_mixed<MODE>), dim3(blocks), dim3(64 * kMixedWaves), 0, stream, worlds, groups,
+                       n_groups, wave_start, pass);
+}
+
+#define MRPOP_CHECK_KERNEL(EXT, A, MODE)                                                                         \
+    EXT template void launch_check_kernel<A, MODE>(int, hipStream_t, const WorldDesc *, const EdgeGroup *,      \
+                                                   const EdgeGroup &, int, const unsigned *, const FineQueue &, \
+                                                   int);
+#define MRPOP_CHECK_KERNEL_MIXED(EXT, MODE)                                                                \
+    EXT template void launch_check_kernel_mixed<MODE>(int, hipStream_t, const WorldDesc *, const EdgeGroup *, \
+                                                      int, const unsigned *, int);
+// every instantiation lives in one object of check_inst.hip
+#define MRPOP_CHECK_ALL(EXT)                       \
+    MRPOP_CHECK_KERNEL(EXT, 1, MRPOP_MODE_SINGLE)  \
+    MRPOP_CHECK_KERNEL(EXT, 2, MRPOP_MODE_FULL)    \
+    MRPOP_CHECK_KERNEL(EXT, 3, MRPOP_MODE_FULL)    \
+    MRPOP_CHECK_KERNEL(EXT, 4, MRPOP_MODE_FULL)    \
+    MRPOP_CHECK_KERNEL(EXT, 5, MRPOP_MODE_FULL)    \
+    MRPOP_CHECK_KERNEL(EXT, 2, MRPOP_MODE_DRRT)    \
+    MRPOP_CHECK_KERNEL(EXT, 3, MRPOP_MODE_DRRT)    \
+    MRPOP_CHECK_KERNEL(EXT, 4, MRPOP_MODE_DRRT)    \
+    MRPOP_CHECK_KERNEL(EXT, 5, MRPOP_MODE_DRRT)    \
+    MRPOP_CHECK_KERNEL_MIXED(EXT, MRPOP_MODE_FULL) \
+    MRPOP_CHECK_KERNEL_MIXED(EXT, MRPOP_MODE_DRRT)
+#ifndef MRPOP_CHECK_INST
+MRPOP_CHECK_ALL(extern)
+#endif
+
+}  // namespace dev
+}  // namespace mrpop
