@@ -98,8 +98,8 @@ __global__ void wave_prefix_kernel(const dev::EdgeGroup *groups, int n_groups, i
 
 template <int A, int MODE>
 void enqueue_check(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups, const dev::EdgeGroup &g0,
-                   int n_groups, unsigned *d_wave_start, const FineQueue &aux, int check_blocks, bool zero,
-                   int pass, hipStream_t stream) {
+                   int n_groups, unsigned *d_wave_start, const FineQueue &aux, int check_blocks, bool persistent,
+                   bool zero, int pass, hipStream_t stream) {
     if (zero)
         hipLaunchKernelGGL(zero_outputs_kernel,
                            dim3(d_groups ? 4 : std::max(1, std::min(1024, (g0.n_edges + 255) / 256)),
@@ -110,13 +110,15 @@ void enqueue_check(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_group
         hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups,
                            MODE == MRPOP_MODE_SINGLE ? 1 : 0, pass, A, d_wave_start);
     if (n_groups <= 64) d_wave_start = nullptr;
-    dev::launch_check_kernel<A, MODE>(check_blocks, stream, d_worlds, d_groups, g0, n_groups, d_wave_start, aux, pass);
+    dev::launch_check_kernel<A, MODE>(check_blocks, persistent, stream, d_worlds, d_groups, g0, n_groups, d_wave_start,
+                                      aux, pass);
 }
 
 // Groups of several arm counts (multi-arm modes) in one launch.
 template <int MODE>
 void enqueue_mixed(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups, int n_groups,
-                   unsigned *d_wave_start, int check_blocks, bool zero, int pass, hipStream_t stream) {
+                   unsigned *d_wave_start, int check_blocks, bool persistent, bool zero, int pass,
+                   hipStream_t stream) {
     const dev::EdgeGroup none{};
     if (zero)
         hipLaunchKernelGGL(zero_outputs_kernel, dim3(4, std::min(n_groups, 1024)), dim3(256), 0, stream, d_groups,
@@ -124,22 +126,29 @@ void enqueue_mixed(const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_group
     if (n_groups > 64)
         hipLaunchKernelGGL(wave_prefix_kernel, dim3(1), dim3(256), 0, stream, d_groups, n_groups, 0, pass,
                            dev::kMixedWaves, d_wave_start);
-    dev::launch_check_kernel_mixed<MODE>(check_blocks, stream, d_worlds, d_groups, n_groups,
+    dev::launch_check_kernel_mixed<MODE>(check_blocks, persistent, stream, d_worlds, d_groups, n_groups,
                                          n_groups > 64 ? d_wave_start : nullptr, pass);
 }
 
-// Blocks of W waves that stay resident at three waves per SIMD: the
+// Blocks of W waves that stay resident at three waves per SIMD: a
 // persistent grid needs no more (idle blocks only cost dispatch).
 size_t resident_blocks(const Context *c, int W) {
     return (size_t)c->num_cus * std::max(1, 12 / W);
 }
 
+// Grouped launches run one tile per block over their bound unless the
+// caller marks them sparse or the bound exceeds this many resident grids
+// (then a persistent grid of resident blocks loops over the tiles), so that
+// blocks without a tile stay few.
+constexpr size_t kOneTileGrids = 64;
+
 int dispatch(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
              const dev::EdgeGroup &g0, int n_groups, unsigned *d_wave_start, const FineQueue &aux, int check_blocks,
-             bool zero, int pass, hipStream_t stream) {
+             bool persistent, bool zero, int pass, hipStream_t stream) {
 #define MRPOP_CASE(M, AA)                                                                                         \
     case M * 8 + AA:                                                                                              \
-        enqueue_check<AA, M>(d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, zero, pass, stream); \
+        enqueue_check<AA, M>(d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, persistent, zero, pass, \
+                             stream);                                                                             \
         break;
     switch (mode * 8 + A) {
         MRPOP_CASE(MRPOP_MODE_SINGLE, 1)
@@ -153,13 +162,13 @@ int dispatch(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const 
         MRPOP_CASE(MRPOP_MODE_DRRT, 5)
         case MRPOP_MODE_FULL * 8 + 0:  // groups of several arm counts
             if (!d_groups) return c->fail(MRPOP_ERR_INVALID, "a mixed-arm check needs edge groups");
-            enqueue_mixed<MRPOP_MODE_FULL>(d_worlds, d_groups, n_groups, d_wave_start, check_blocks, zero, pass,
-                                           stream);
+            enqueue_mixed<MRPOP_MODE_FULL>(d_worlds, d_groups, n_groups, d_wave_start, check_blocks, persistent, zero,
+                                           pass, stream);
             break;
         case MRPOP_MODE_DRRT * 8 + 0:
             if (!d_groups) return c->fail(MRPOP_ERR_INVALID, "a mixed-arm check needs edge groups");
-            enqueue_mixed<MRPOP_MODE_DRRT>(d_worlds, d_groups, n_groups, d_wave_start, check_blocks, zero, pass,
-                                           stream);
+            enqueue_mixed<MRPOP_MODE_DRRT>(d_worlds, d_groups, n_groups, d_wave_start, check_blocks, persistent, zero,
+                                           pass, stream);
             break;
         default: return c->fail(MRPOP_ERR_INVALID, "unsupported mode %d with %d arms", mode, A);
     }
@@ -186,29 +195,30 @@ int launch_check(Context *c, const World *w, int mode, int arm, int granularity,
                       d_out_edge, (unsigned long long *)d_out_mask, 0, 0};
     const int S = dev::tile_samples(A, false);
     const size_t waves = (total + S - 1) / S;  // tiles
-    const int check_blocks = (int)std::min<size_t>(waves, resident_blocks(c, A));
+    const int check_blocks = (int)waves;  // one tile per block
     const int probe = d_out_mask || c->diag ? 0 : std::min(c->probe, granularity);
     if (probe > 0 && probe < granularity) {
         const size_t pw = ((size_t)n_edges * probe + S - 1) / S;
-        if (int rc = dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux,
-                              (int)std::min<size_t>(pw, resident_blocks(c, A)), true, probe, stream))
+        if (int rc = dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux, (int)pw, false, true, probe, stream))
             return rc;
-        return dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux, check_blocks, false, -1, stream);
+        return dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux, check_blocks, false, false, -1, stream);
     }
-    return dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux, check_blocks, true, 0, stream);
+    return dispatch(c, mode, A, w->d_desc, nullptr, g0, 1, nullptr, aux, check_blocks, false, true, 0, stream);
 }
 
 int launch_check_groups(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
-                        int n_groups, unsigned *d_wave_start, size_t max_waves, bool zero, int pass,
+                        int n_groups, unsigned *d_wave_start, size_t max_waves, bool sparse, bool zero, int pass,
                         hipStream_t stream) {
     if (n_groups <= 0) return MRPOP_OK;
     if (pass < -1) return c->fail(MRPOP_ERR_INVALID, "check pass %d", pass);
     const FineQueue aux{nullptr};
     dev::EdgeGroup g0{};
     const int W = mode == MRPOP_MODE_SINGLE ? 1 : (A == 0 ? dev::kMixedWaves : A);
-    const int check_blocks = (int)std::max<size_t>(1, std::min<size_t>(max_waves, resident_blocks(c, W)));
-    return dispatch(c, mode, A, d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, zero, pass,
-                    stream);
+    const size_t resident = resident_blocks(c, W);
+    const bool persistent = sparse || max_waves > kOneTileGrids * resident;
+    const int check_blocks = (int)std::max<size_t>(1, persistent ? resident : max_waves);
+    return dispatch(c, mode, A, d_worlds, d_groups, g0, n_groups, d_wave_start, aux, check_blocks, persistent, zero,
+                    pass, stream);
 }
 
 int launch_centres(Context *c, const World *w, int n_cfg, const float *d_cfg, float *d_coarse, float *d_fine,

@@ -114,9 +114,24 @@ template <int MODE> constexpr int range_pos(int r) {
 constexpr int range_rad(int r) { return kFineSelfRange[r][0]; }
 template <int MODE> constexpr int range_gate(int r) { return kFineJoint[range_pos<MODE>(r)]; }
 
+// Ranges gated on joint 5 test spheres on joint 5 against targets on joint
+// 7 only, so the edge check evaluates them during the coarse FK (where both
+// frames are at hand) and applies the gate afterwards.  Coarse sphere 5
+// overlaps the hand's coarse spheres in practically every configuration,
+// so the gate is almost always open and nothing is computed in vain.
+template <int MODE> constexpr bool gate5_ranges_target_joint7() {
+    for (int r = 0; r < kFineSelfRanges; ++r)
+        if (range_gate<MODE>(r) == 5)
+            for (int t = kFineSelfRange[r][1]; t <= kFineSelfRange[r][2]; ++t)
+                if (kFineJoint[t] != 7) return false;
+    return true;
+}
+static_assert(gate5_ranges_target_joint7<MRPOP_MODE_FULL>() && gate5_ranges_target_joint7<MRPOP_MODE_SINGLE>(),
+              "joint-5 gated ranges target the hand (joint 7)");
+
 // ------------------------------------------------------------- FK helpers --
 __device__ __forceinline__ void load_base(const WorldDesc &w, int arm, Frame &T) {
-    const float *b = w.obs + w.o_base + 12 * arm;
+    cfloat *b = world_data(w) + w.o_base + 12 * arm;
 #pragma unroll
     for (int i = 0; i < 12; ++i) T.m[i] = b[i];
 }
@@ -205,7 +220,7 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
     const int j0 = max((int)kTab.block_first[k], 1), j1 = kTab.block_first[k] + kTab.block_count[k];
     const float Rsq = R * R;
     for (int i = 0; i < w.n_sph; ++i) {
-        const float *o = w.obs + w.o_sph + 4 * i;
+        cfloat *o = world_data(w) + w.o_sph + 4 * i;
         if (sphere_sphere(o[0], o[1], o[2], o[3], bx, by, bz, R) < 0.f)
             for (int j = j0; j < j1; ++j) {
                 float x, y, z;
@@ -214,7 +229,7 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
             }
     }
     for (int i = 0; i < w.n_cap; ++i) {
-        const float *o = w.obs + w.o_cap + 8 * i;
+        cfloat *o = world_data(w) + w.o_cap + 8 * i;
         if (sphere_capsule(o, bx, by, bz, R) < 0.f)
             for (int j = j0; j < j1; ++j) {
                 float x, y, z;
@@ -223,7 +238,7 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
             }
     }
     for (int i = 0; i < w.n_zcap; ++i) {
-        const float *o = w.obs + w.o_zcap + 8 * i;
+        cfloat *o = world_data(w) + w.o_zcap + 8 * i;
         if (sphere_zcapsule(o, bx, by, bz, R) < 0.f)
             for (int j = j0; j < j1; ++j) {
                 float x, y, z;
@@ -232,7 +247,7 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
             }
     }
     for (int i = 0; i < w.n_aabb; ++i) {
-        const float *o = w.obs + w.o_aabb + 6 * i;
+        cfloat *o = world_data(w) + w.o_aabb + 6 * i;
         if (sphere_aabb(o, bx, by, bz, Rsq) < 0.f)
             for (int j = j0; j < j1; ++j) {
                 float x, y, z;
@@ -241,7 +256,7 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
             }
     }
     for (int i = 0; i < w.n_obb; ++i) {
-        const float *o = w.obs + w.o_obb + 15 * i;
+        cfloat *o = world_data(w) + w.o_obb + 15 * i;
         if (sphere_obb(o, bx, by, bz, Rsq) < 0.f)
             for (int j = j0; j < j1; ++j) {
                 float x, y, z;
@@ -250,7 +265,7 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
             }
     }
     for (int i = 0; i < w.n_zcub; ++i) {
-        const float *o = w.obs + w.o_zcub + 15 * i;
+        cfloat *o = world_data(w) + w.o_zcub + 15 * i;
         if (sphere_zcuboid(o, bx, by, bz, Rsq) < 0.f)
             for (int j = j0; j < j1; ++j) {
                 float x, y, z;
@@ -411,7 +426,9 @@ __device__ __forceinline__ unsigned group_waves(const EdgeGroup &g, int single, 
 // S = NT / A, thread arm * S + slot holding arm `arm` of sample `slot`.  The
 // waves are arm-major, so the environment and arm-pair culls of a wave see
 // one arm of 64 nearby samples.
-template <int A, int MODE, int KH, int NT>
+// SPEC5: evaluate the joint-5 gated fine ranges during the coarse FK (the
+// one-tile-per-block kernels); otherwise in the fine walk of phase B.
+template <int A, int MODE, int KH, int NT, bool SPEC5>
 __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc *worlds, const EdgeGroup &eb,
                                            unsigned lw, const FineQueue &fq, int pass) {
     constexpr int S = NT / A;  // samples per tile
@@ -439,8 +456,11 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
     const int base_arm = MODE == MRPOP_MODE_SINGLE ? eb.arm : arm;
     const int dof_off = MODE == MRPOP_MODE_SINGLE ? 0 : 7 * arm;
 
-    // ---- A. coarse FK of this lane's arm
+    // ---- A. coarse FK of this lane's arm, and the fine self ranges gated on
+    // joint 5 (tested spheres on joint 5, targets on joint 7), evaluated
+    // here while both frames pass by; their gate is applied in phase B
     float cx[kCoarseSpheres], cy[kCoarseSpheres], cz[kCoarseSpheres];
+    bool self5hit = false;
     {
         float q[7];
         interp7(eb, e, k, dof_off, q);
@@ -450,6 +470,7 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
             constexpr float ox = kCoarseSphere[0][0], oy = kCoarseSphere[0][1], oz = kCoarseSphere[0][2];
             centre(T, ox, oy, oz, cx[0], cy[0], cz[0]);
         }
+        float px[kFineSelfRanges], py[kFineSelfRanges], pz[kFineSelfRanges];
         static_for<1, kJoints>([&](auto jc) {
             constexpr int J = decltype(jc)::value;
             Frame N;
@@ -460,6 +481,41 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
                 constexpr float ox = kCoarseSphere[s][0], oy = kCoarseSphere[s][1], oz = kCoarseSphere[s][2];
                 centre(T, ox, oy, oz, cx[s], cy[s], cz[s]);
             });
+            if constexpr (SPEC5 && J == 5)
+                static_for<0, kFineSelfRanges>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    if constexpr (range_gate<MODE>(r) == 5) {
+                        constexpr int ps = range_pos<MODE>(r);
+                        centre(T, kFineSphere[ps][0], kFineSphere[ps][1], kFineSphere[ps][2], px[r], py[r], pz[r]);
+                    }
+                });
+            if constexpr (SPEC5 && J == 7)
+                static_for<0, kFineSelfRanges>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    if constexpr (range_gate<MODE>(r) == 5) {
+                        constexpr float rad = kFineSphere[range_rad(r)][3];
+                        static_for<7, kCoarseSpheres>([&](auto kc) {  // blocks on joint 7
+                            constexpr int kb = decltype(kc)::value;
+                            constexpr int b0 = kBlockFirst[kb], b1 = kBlockFirst[kb] + kBlockCount[kb] - 1;
+                            constexpr int lo = kFineSelfRange[r][1] > b0 ? kFineSelfRange[r][1] : b0;
+                            constexpr int hi = kFineSelfRange[r][2] < b1 ? kFineSelfRange[r][2] : b1;
+                            if constexpr (lo <= hi) {
+                                constexpr float R = kCoarseSphere[kb][3] + kBlockInflate[kb];
+                                const float dx = px[r] - cx[kb], dy = py[r] - cy[kb], dz = pz[r] - cz[kb];
+                                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                                if (d2 < (rad + R) * (rad + R)) {
+                                    static_for<lo, hi + 1>([&](auto tc) {
+                                        constexpr int t = decltype(tc)::value;
+                                        float x, y, z;
+                                        centre(T, kFineSphere[t][0], kFineSphere[t][1], kFineSphere[t][2], x, y, z);
+                                        self5hit |= sphere_sphere(px[r], py[r], pz[r], rad, x, y, z,
+                                                                  kFineSphere[t][3]) < 0.f;
+                                    });
+                                }
+                            }
+                        });
+                    }
+                });
         });
     }
 #pragma unroll
@@ -514,9 +570,9 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
             hy = fmaxf(hy, cy[s]);
             hz = fmaxf(hz, cz[s]);
         }
-        const float *box = w.obs + w.o_box;
+        cfloat *box = world_data(w) + w.o_box;
         auto near = [&](int ob) {
-            const float *bb = box + 8 * ob;
+            cfloat *bb = box + 8 * ob;
             const float g = kEnvRadiusMax * bb[6];
             return !(hx < bb[0] - g || hy < bb[1] - g || hz < bb[2] - g || lx > bb[3] + g || ly > bb[4] + g ||
                      lz > bb[5] + g);
@@ -531,33 +587,33 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
         int ob = 0;
         for (int i = 0; i < w.n_sph; ++i, ++ob) {
             if (!near(ob)) continue;
-            const float *o = w.obs + w.o_sph + 4 * i;
+            cfloat *o = world_data(w) + w.o_sph + 4 * i;
             const float ox = o[0], oy = o[1], oz = o[2], orr = o[3];
             test_all([&](float x, float y, float z, float r) { return sphere_sphere(ox, oy, oz, orr, x, y, z, r); });
         }
         for (int i = 0; i < w.n_cap; ++i, ++ob) {
             if (!near(ob)) continue;
-            const float *o = w.obs + w.o_cap + 8 * i;
+            cfloat *o = world_data(w) + w.o_cap + 8 * i;
             test_all([&](float x, float y, float z, float r) { return sphere_capsule(o, x, y, z, r); });
         }
         for (int i = 0; i < w.n_zcap; ++i, ++ob) {
             if (!near(ob)) continue;
-            const float *o = w.obs + w.o_zcap + 8 * i;
+            cfloat *o = world_data(w) + w.o_zcap + 8 * i;
             test_all([&](float x, float y, float z, float r) { return sphere_zcapsule(o, x, y, z, r); });
         }
         for (int i = 0; i < w.n_aabb; ++i, ++ob) {
             if (!near(ob)) continue;
-            const float *o = w.obs + w.o_aabb + 6 * i;
+            cfloat *o = world_data(w) + w.o_aabb + 6 * i;
             test_all([&](float x, float y, float z, float r) { return sphere_aabb(o, x, y, z, r * r); });
         }
         for (int i = 0; i < w.n_obb; ++i, ++ob) {
             if (!near(ob)) continue;
-            const float *o = w.obs + w.o_obb + 15 * i;
+            cfloat *o = world_data(w) + w.o_obb + 15 * i;
             test_all([&](float x, float y, float z, float r) { return sphere_obb(o, x, y, z, r * r); });
         }
         for (int i = 0; i < w.n_zcub; ++i, ++ob) {
             if (!near(ob)) continue;
-            const float *o = w.obs + w.o_zcub + 15 * i;
+            cfloat *o = world_data(w) + w.o_zcub + 15 * i;
             test_all([&](float x, float y, float z, float r) { return sphere_zcuboid(o, x, y, z, r * r); });
         }
     }
@@ -678,8 +734,13 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
         Frame base;
         load_base(w, base_arm, base);
         if (selfbits & 0x1fu) finehit = fine_walk<MODE, 0, 4, false, NT>(w, base, q, selfbits, 0u, sm.cs, lane);
-        if ((selfbits & 0x20u) || envbits)
+        if constexpr (SPEC5) {
+            if (selfbits & 0x20u) finehit |= self5hit;  // the joint-5 ranges, evaluated in phase A
+            // environment only (no self range has a gate in 6..5)
+            if (kEnv && envbits) finehit |= fine_walk<MODE, 6, 5, kEnv, NT>(w, base, q, 0u, envbits, sm.cs, lane);
+        } else if ((selfbits & 0x20u) || envbits) {
             finehit |= fine_walk<MODE, 5, 5, kEnv, NT>(w, base, q, selfbits, envbits, sm.cs, lane);
+        }
     }
     if (active && finehit) done[le] = 1u;
 
@@ -750,14 +811,21 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
     }
 }
 
-// Persistent loop over the wave tiles of n_groups edge groups (or of the
-// single group g0 when groups is null).  Up to 64 groups the wave computes
-// the groups' first waves itself, one group per lane (wave_start is then
-// null); more groups come with a prefix from wave_prefix_kernel.  A == 0:
-// groups of different arm counts (multi-arm modes), dispatched per tile.
-// Persistent loop of a block of W waves over the tiles of the groups.  Every
-// wave computes the same group lookup.
-template <int MODE, int W, class Tile>
+// The wave tiles of n_groups edge groups (or of the single group g0 when
+// groups is null).  Up to 64 groups the wave computes the groups' first
+// tiles itself, one group per lane (wave_start is then null); more groups
+// come with a prefix from wave_prefix_kernel.  A == 0: groups of different
+// arm counts (multi-arm modes), dispatched per tile.  Every wave of the
+// block computes the same group lookup.
+// PERSISTENT = false: block b checks tile b (blocks beyond the last tile
+// return), so the grid must cover every tile.  That is the fast form:
+// gfx950 VOP3 instructions take no 32-bit literals, the fine tests' constants
+// are materialised in registers, and a loop around the tile lets the
+// compiler hoist them all out of it and spill them (168 VGPRs and 48 spilled
+// against 138 and none for the 4-arm kernel).  PERSISTENT = true: a grid of
+// any size loops over the tiles, for launches whose tile count is only
+// known on the device and bounded far above it.
+template <int MODE, int W, bool PERSISTENT, class Tile>
 __device__ __forceinline__ void check_loop(const EdgeGroup *groups, const EdgeGroup &g0, int n_groups,
                                            const unsigned *wave_start, int pass, Tile &&tile) {
     const int lane = threadIdx.x & 63;
@@ -778,7 +846,7 @@ __device__ __forceinline__ void check_loop(const EdgeGroup *groups, const EdgeGr
     } else {
         total_waves = wave_start[n_groups];
     }
-    for (unsigned wv = blockIdx.x; wv < total_waves; wv += gridDim.x) {
+    auto run = [&](unsigned wv) {
         int gi = 0;
         unsigned gstart = 0;
         if (groups && !wave_start) {  // last group whose first wave is <= wv (empty groups share starts)
@@ -797,26 +865,31 @@ __device__ __forceinline__ void check_loop(const EdgeGroup *groups, const EdgeGr
         }
         const EdgeGroup eb = groups ? groups[gi] : g0;
         tile(eb, wv - gstart);
+    };
+    if constexpr (PERSISTENT) {
+        for (unsigned wv = blockIdx.x; wv < total_waves; wv += gridDim.x) run(wv);
+    } else if (blockIdx.x < total_waves) {
+        run(blockIdx.x);
     }
 }
 
-// A waves per block, one per arm; three waves per SIMD: the register budget
-// (168 VGPRs) costs a few spills but hides more latency.
-template <int A, int MODE>
+// A waves per block, one per arm; three waves per SIMD (the LDS of a tile
+// allows three blocks per CU).
+template <int A, int MODE, bool PERSISTENT>
 __global__ __launch_bounds__(64 * A, 3) void check_kernel(const WorldDesc *worlds, const EdgeGroup *groups,
                                                        EdgeGroup g0, int n_groups, const unsigned *wave_start,
                                                        FineQueue fq, int pass) {
     constexpr int KH = A > 1 ? A - 1 : 1;
     __shared__ TileLds<KH, 64 * A> sm;
-    check_loop<MODE, A>(groups, g0, n_groups, wave_start, pass, [&](const EdgeGroup &eb, unsigned lw) {
-        check_tile<A, MODE, KH, 64 * A>(sm, worlds, eb, lw, fq, pass);
+    check_loop<MODE, A, PERSISTENT>(groups, g0, n_groups, wave_start, pass, [&](const EdgeGroup &eb, unsigned lw) {
+        check_tile<A, MODE, KH, 64 * A, !PERSISTENT>(sm, worlds, eb, lw, fq, pass);
     });
 }
 
 // Multi-arm groups of 2..5 arms in one launch: each tile runs the code of
 // its group's arm count, so a planner round checks all its problems in one
 // launch per mode instead of one per arm count.
-template <int MODE>
+template <int MODE, bool PERSISTENT>
 __global__ __launch_bounds__(64 * kMixedWaves, 3) void check_kernel_mixed(const WorldDesc *worlds,
                                                                        const EdgeGroup *groups, int n_groups,
                                                                        const unsigned *wave_start, int pass) {
@@ -824,12 +897,13 @@ __global__ __launch_bounds__(64 * kMixedWaves, 3) void check_kernel_mixed(const 
     __shared__ TileLds<4, NT> sm;
     const EdgeGroup none{};
     const FineQueue fq{nullptr};
-    check_loop<MODE, kMixedWaves>(groups, none, n_groups, wave_start, pass, [&](const EdgeGroup &eb, unsigned lw) {
+    check_loop<MODE, kMixedWaves, PERSISTENT>(groups, none, n_groups, wave_start, pass,
+                                             [&](const EdgeGroup &eb, unsigned lw) {
         switch (eb.dim / 7) {
-            case 2: check_tile<2, MODE, 4, NT>(sm, worlds, eb, lw, fq, pass); break;
-            case 3: check_tile<3, MODE, 4, NT>(sm, worlds, eb, lw, fq, pass); break;
-            case 4: check_tile<4, MODE, 4, NT>(sm, worlds, eb, lw, fq, pass); break;
-            default: check_tile<5, MODE, 4, NT>(sm, worlds, eb, lw, fq, pass); break;
+            case 2: check_tile<2, MODE, 4, NT, !PERSISTENT>(sm, worlds, eb, lw, fq, pass); break;
+            case 3: check_tile<3, MODE, 4, NT, !PERSISTENT>(sm, worlds, eb, lw, fq, pass); break;
+            case 4: check_tile<4, MODE, 4, NT, !PERSISTENT>(sm, worlds, eb, lw, fq, pass); break;
+            default: check_tile<5, MODE, 4, NT, !PERSISTENT>(sm, worlds, eb, lw, fq, pass); break;
         }
     });
 }
@@ -841,28 +915,37 @@ __global__ __launch_bounds__(64 * kMixedWaves, 3) void check_kernel_mixed(const 
 namespace mrpop {
 namespace dev {
 
+// persistent = false: blocks must be the tile count (one tile per block).
 template <int A, int MODE>
-void launch_check_kernel(int blocks, hipStream_t stream, const WorldDesc *worlds, const EdgeGroup *groups,
-                         const EdgeGroup &g0, int n_groups, const unsigned *wave_start, const FineQueue &fq,
-                         int pass) {
-    hipLaunchKernelGGL((check_kernel<A, MODE>), dim3(blocks), dim3(64 * A), 0, stream, worlds, groups, g0, n_groups,
-                       wave_start, fq, pass);
+void launch_check_kernel(int blocks, bool persistent, hipStream_t stream, const WorldDesc *worlds,
+                         const EdgeGroup *groups, const EdgeGroup &g0, int n_groups, const unsigned *wave_start,
+                         const FineQueue &fq, int pass) {
+    if (persistent)
+        hipLaunchKernelGGL((check_kernel<A, MODE, true>), dim3(blocks), dim3(64 * A), 0, stream, worlds, groups, g0,
+                           n_groups, wave_start, fq, pass);
+    else
+        hipLaunchKernelGGL((check_kernel<A, MODE, false>), dim3(blocks), dim3(64 * A), 0, stream, worlds, groups, g0,
+                           n_groups, wave_start, fq, pass);
 }
 
 template <int MODE>
-void launch_check_kernel_mixed(int blocks, hipStream_t stream, const WorldDesc *worlds, const EdgeGroup *groups,
-                               int n_groups, const unsigned *wave_start, int pass) {
-    hipLaunchKernelGGL((check_kernel_mixed<MODE>), dim3(blocks), dim3(64 * kMixedWaves), 0, stream, worlds, groups,
-                       n_groups, wave_start, pass);
+void launch_check_kernel_mixed(int blocks, bool persistent, hipStream_t stream, const WorldDesc *worlds,
+                               const EdgeGroup *groups, int n_groups, const unsigned *wave_start, int pass) {
+    if (persistent)
+        hipLaunchKernelGGL((check_kernel_mixed<MODE, true>), dim3(blocks), dim3(64 * kMixedWaves), 0, stream, worlds,
+                           groups, n_groups, wave_start, pass);
+    else
+        hipLaunchKernelGGL((check_kernel_mixed<MODE, false>), dim3(blocks), dim3(64 * kMixedWaves), 0, stream, worlds,
+                           groups, n_groups, wave_start, pass);
 }
 
 #define MRPOP_CHECK_KERNEL(EXT, A, MODE)                                                                         \
-    EXT template void launch_check_kernel<A, MODE>(int, hipStream_t, const WorldDesc *, const EdgeGroup *,      \
-                                                   const EdgeGroup &, int, const unsigned *, const FineQueue &, \
+    EXT template void launch_check_kernel<A, MODE>(int, bool, hipStream_t, const WorldDesc *, const EdgeGroup *, \
+                                                   const EdgeGroup &, int, const unsigned *, const FineQueue &,  \
                                                    int);
 #define MRPOP_CHECK_KERNEL_MIXED(EXT, MODE)                                                                \
-    EXT template void launch_check_kernel_mixed<MODE>(int, hipStream_t, const WorldDesc *, const EdgeGroup *, \
-                                                      int, const unsigned *, int);
+    EXT template void launch_check_kernel_mixed<MODE>(int, bool, hipStream_t, const WorldDesc *,             \
+                                                      const EdgeGroup *, int, const unsigned *, int);
 // every instantiation lives in one object of check_inst.hip
 #define MRPOP_CHECK_ALL(EXT)                       \
     MRPOP_CHECK_KERNEL(EXT, 1, MRPOP_MODE_SINGLE)  \

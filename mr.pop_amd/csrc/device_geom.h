@@ -116,6 +116,14 @@ struct WorldDesc {
     int o_box;
 };
 
+// The world buffer is read-only while a check runs.  Read through the
+// constant address space, its wave-uniform reads (obstacle records, cull
+// boxes) become scalar loads through the scalar cache; through a generic
+// pointer, in a kernel that also writes memory, they are vector loads whose
+// latency every obstacle iteration waits for.
+using cfloat = const __attribute__((address_space(4))) float;
+__device__ __forceinline__ cfloat *world_data(const WorldDesc &w) { return (cfloat *)w.obs; }
+
 // utils.cuh:100-107
 __device__ __forceinline__ float sphere_sphere(float ax, float ay, float az, float ar, float bx, float by, float bz,
                                                float br) {
@@ -125,7 +133,7 @@ __device__ __forceinline__ float sphere_sphere(float ax, float ay, float az, flo
 }
 
 // utils.cuh:148-158
-__device__ __forceinline__ float sphere_capsule(const float *c, float x, float y, float z, float r) {
+template <class P> __device__ __forceinline__ float sphere_capsule(P c, float x, float y, float z, float r) {
     float dot = (x - c[0]) * c[3] + (y - c[1]) * c[4] + (z - c[2]) * c[5];
     float t = fmaxf(fminf(dot * c[7], 1.f), 0.f);
     float dx = x - __builtin_fmaf(c[3], t, c[0]);
@@ -136,7 +144,7 @@ __device__ __forceinline__ float sphere_capsule(const float *c, float x, float y
 }
 
 // utils.cuh:165-174
-__device__ __forceinline__ float sphere_zcapsule(const float *c, float x, float y, float z, float r) {
+template <class P> __device__ __forceinline__ float sphere_zcapsule(P c, float x, float y, float z, float r) {
     float dot = (z - c[2]) * c[5];
     float t = fmaxf(fminf(dot * c[7], 1.f), 0.f);
     float dx = x - c[0], dy = y - c[1], dz = z - __builtin_fmaf(c[5], t, c[2]);
@@ -146,7 +154,7 @@ __device__ __forceinline__ float sphere_zcapsule(const float *c, float x, float 
 
 // utils.cuh:185-201 for a cuboid whose axes are the world axes: the axis dot
 // products are then exactly the coordinate differences.
-__device__ __forceinline__ float sphere_aabb(const float *b, float x, float y, float z, float rsq) {
+template <class P> __device__ __forceinline__ float sphere_aabb(P b, float x, float y, float z, float rsq) {
     float a1 = fmaxf(fabsf(x - b[0]) - b[3], 0.f);
     float a2 = fmaxf(fabsf(y - b[1]) - b[4], 0.f);
     float a3 = fmaxf(fabsf(z - b[2]) - b[5], 0.f);
@@ -154,7 +162,7 @@ __device__ __forceinline__ float sphere_aabb(const float *b, float x, float y, f
 }
 
 // utils.cuh:185-201
-__device__ __forceinline__ float sphere_obb(const float *b, float x, float y, float z, float rsq) {
+template <class P> __device__ __forceinline__ float sphere_obb(P b, float x, float y, float z, float rsq) {
     float xs = x - b[0], ys = y - b[1], zs = z - b[2];
     float a1 = fmaxf(fabsf(__builtin_fmaf(b[3], xs, __builtin_fmaf(b[4], ys, b[5] * zs))) - b[12], 0.f);
     float a2 = fmaxf(fabsf(__builtin_fmaf(b[6], xs, __builtin_fmaf(b[7], ys, b[8] * zs))) - b[13], 0.f);
@@ -163,7 +171,7 @@ __device__ __forceinline__ float sphere_obb(const float *b, float x, float y, fl
 }
 
 // utils.cuh:208-221
-__device__ __forceinline__ float sphere_zcuboid(const float *b, float x, float y, float z, float rsq) {
+template <class P> __device__ __forceinline__ float sphere_zcuboid(P b, float x, float y, float z, float rsq) {
     float xs = x - b[0], ys = y - b[1], zs = z - b[2];
     float a1 = fmaxf(fabsf(__builtin_fmaf(b[3], xs, b[4] * ys)) - b[12], 0.f);
     float a2 = fmaxf(fabsf(__builtin_fmaf(b[6], xs, b[7] * ys)) - b[13], 0.f);
@@ -176,14 +184,14 @@ __device__ __forceinline__ bool env_hit(const WorldDesc &w, float x, float y, fl
     const float rsq = r * r;
     bool hit = false;
     for (int i = 0; i < w.n_sph; ++i) {
-        const float *o = w.obs + w.o_sph + 4 * i;
+        cfloat *o = world_data(w) + w.o_sph + 4 * i;
         hit |= sphere_sphere(o[0], o[1], o[2], o[3], x, y, z, r) < 0.f;
     }
-    for (int i = 0; i < w.n_cap; ++i) hit |= sphere_capsule(w.obs + w.o_cap + 8 * i, x, y, z, r) < 0.f;
-    for (int i = 0; i < w.n_zcap; ++i) hit |= sphere_zcapsule(w.obs + w.o_zcap + 8 * i, x, y, z, r) < 0.f;
-    for (int i = 0; i < w.n_aabb; ++i) hit |= sphere_aabb(w.obs + w.o_aabb + 6 * i, x, y, z, rsq) < 0.f;
-    for (int i = 0; i < w.n_obb; ++i) hit |= sphere_obb(w.obs + w.o_obb + 15 * i, x, y, z, rsq) < 0.f;
-    for (int i = 0; i < w.n_zcub; ++i) hit |= sphere_zcuboid(w.obs + w.o_zcub + 15 * i, x, y, z, rsq) < 0.f;
+    for (int i = 0; i < w.n_cap; ++i) hit |= sphere_capsule(world_data(w) + w.o_cap + 8 * i, x, y, z, r) < 0.f;
+    for (int i = 0; i < w.n_zcap; ++i) hit |= sphere_zcapsule(world_data(w) + w.o_zcap + 8 * i, x, y, z, r) < 0.f;
+    for (int i = 0; i < w.n_aabb; ++i) hit |= sphere_aabb(world_data(w) + w.o_aabb + 6 * i, x, y, z, rsq) < 0.f;
+    for (int i = 0; i < w.n_obb; ++i) hit |= sphere_obb(world_data(w) + w.o_obb + 15 * i, x, y, z, rsq) < 0.f;
+    for (int i = 0; i < w.n_zcub; ++i) hit |= sphere_zcuboid(world_data(w) + w.o_zcub + 15 * i, x, y, z, rsq) < 0.f;
     return hit;
 }
 

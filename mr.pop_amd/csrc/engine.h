@@ -154,6 +154,7 @@ class Engine {
             size_t probe_waves;  // grid bound of the probe pass
         };
         int probe = 0;  // probe pass samples per edge (0: one pass)
+        bool sparse = false;  // counts usually far below the capacity bound (persistent launch)
         std::vector<Bucket> buckets;
     };
     struct Lane {

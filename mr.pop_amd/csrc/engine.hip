@@ -629,11 +629,11 @@ int Engine::launch_staged(Lane &L, const Staged &st) {
     for (const Staged::Bucket &bk : st.buckets) {
         if (st.probe > 0) {
             if (int rc = launch_check_groups(c_, st.mode, bk.A, d_worlds_, L.d_groups + bk.first, bk.n, L.d_wave,
-                                             bk.probe_waves, false, st.probe, L.stream))
+                                             bk.probe_waves, st.sparse, false, st.probe, L.stream))
                 return rc;
         }
         if (int rc = launch_check_groups(c_, st.mode, bk.A, d_worlds_, L.d_groups + bk.first, bk.n, L.d_wave,
-                                         bk.max_waves, false, st.probe > 0 ? -1 : 0, L.stream))
+                                         bk.max_waves, st.sparse, false, st.probe > 0 ? -1 : 0, L.stream))
             return rc;
     }
     return MRPOP_OK;
@@ -698,6 +698,7 @@ int Engine::launch_lane(Lane &L) {
         Staged s1, s2;
         if (int rc = stage_groups(L, MRPOP_MODE_DRRT, g1, 1, L.dr, s1)) return rc;
         if (int rc = stage_groups(L, MRPOP_MODE_FULL, g2, 1, L.dr, s2)) return rc;
+        s2.sparse = true;  // connection chunks: sized for 64 per expansion
         if (reset) plan::launch_dr_begin(ds, d, n, st);
         auto enqueue_rounds = [&]() -> int {
             for (int r = 0; r < rounds; ++r) {
@@ -781,6 +782,7 @@ int Engine::launch_lane(Lane &L) {
             for (Slot *s : same) gs.push_back(&s->g_sp);
             Staged ss;
             if (int rc = stage_groups(L, MRPOP_MODE_FULL, gs, 1, same, ss)) return rc;
+            ss.sparse = true;  // simplification chunks: sized for the longest path
             if (int rc = launch_staged(L, ss)) return rc;
             plan::launch_sp_collect(ds, d, n, st);
             if (stage < s0->st.bspline_step) plan::launch_sp_smooth_apply(ds, d, n, st);

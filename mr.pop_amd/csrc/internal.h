@@ -84,11 +84,14 @@ int launch_check(Context *c, const World *w, int mode, int arm, int granularity,
 // spread along it (k = floor(j G / P), j = 1..P); -1 then checks every
 // sample of the edges still free.  Probe + completion give the verdicts of
 // pass 0 and are for groups without a per-sample mask.
+// sparse: the groups' counts usually stay far below max_waves (connection
+// and simplification chunks, sized for the worst case); their launch uses a
+// persistent grid of resident blocks instead of one block per possible tile.
 namespace dev {
 struct EdgeGroup;
 }
 int launch_check_groups(Context *c, int mode, int A, const dev::WorldDesc *d_worlds, const dev::EdgeGroup *d_groups,
-                        int n_groups, unsigned *d_wave_start, size_t max_waves, bool zero, int pass,
+                        int n_groups, unsigned *d_wave_start, size_t max_waves, bool sparse, bool zero, int pass,
                         hipStream_t stream);
 
 // Sphere centres kernel (check.hip).
