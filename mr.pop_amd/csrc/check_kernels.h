@@ -26,7 +26,12 @@
 //   split evenly over the A items of a sample (each takes A-1 half-pairs:
 //   rows 0-5 or 6-10 of arm a against arm b; rows whose sphere misses arm b's
 //   ball are skipped).  Obstacle records are wave-uniform and come through
-//   the scalar cache; a per-item cull box skips obstacles far from the arm.
+//   the scalar cache.  The environment tests run only for pairs (sphere,
+//   obstacle) where the obstacle's cull box meets the box of that sphere's
+//   centres over the wave: the wave computes the 10 boxes from LDS and one
+//   lane per obstacle tests them, so a wave of 64 samples typically runs
+//   about 8 of the 300 sphere-obstacle tests of a 30-obstacle world (the
+//   persistent kernels cull per item with one box around the arm).
 //   The fine self, environment and arm-arm tests run in the same kernel,
 //   gated exactly as above, but only where they can hit: each coarse sphere
 //   grown by a few tenths of a millimetre (kBlockInflate) contains the fine
@@ -212,14 +217,49 @@ __device__ __forceinline__ void centre_rt(const Frame &T, int s, float &x, float
 }
 
 // ------------------------------------------------------------ fused check --
+// Orders this wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A value every lane of the wave holds alike, as a scalar.
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Calls f(i) for the obstacles i (0 .. n-1) of one type whose bits, from bit
+// `first` on, are set in m; for every one of them if all.
+template <class F>
+__device__ __forceinline__ void for_touched(unsigned long long m, int first, int n, bool all, F &&f) {
+    if (all) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    if (n == 0) return;  // first may then be 64
+    unsigned long long b = m >> first;
+    if (n < 64) b &= (1ull << n) - 1ull;
+    while (b) {
+        f(__builtin_ctzll(b));
+        b &= b - 1ull;
+    }
+}
+
 // Fine environment tests of block k's spheres (all but sphere 0) in frame T,
 // only against obstacles that its bounding ball (centre b, radius R) touches.
+// touch: a wave-uniform superset of the obstacles (bit = index in the order
+// spheres, capsules, z-capsules, aabbs, obbs, z-cuboids) that the ball can
+// touch on a lane of the wave, or all = true to visit every obstacle.
 __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, int k, float bx, float by, float bz,
-                                          float R) {
+                                          float R, unsigned long long touch, bool all) {
     bool hit = false;
     const int j0 = max((int)kTab.block_first[k], 1), j1 = kTab.block_first[k] + kTab.block_count[k];
     const float Rsq = R * R;
-    for (int i = 0; i < w.n_sph; ++i) {
+    int first = 0;
+    for_touched(touch, first, w.n_sph, all, [&](int i) {
         cfloat *o = world_data(w) + w.o_sph + 4 * i;
         if (sphere_sphere(o[0], o[1], o[2], o[3], bx, by, bz, R) < 0.f)
             for (int j = j0; j < j1; ++j) {
@@ -227,8 +267,9 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
                 centre_rt(T, j, x, y, z);
                 hit |= sphere_sphere(o[0], o[1], o[2], o[3], x, y, z, kTab.r[j]) < 0.f;
             }
-    }
-    for (int i = 0; i < w.n_cap; ++i) {
+    });
+    first += w.n_sph;
+    for_touched(touch, first, w.n_cap, all, [&](int i) {
         cfloat *o = world_data(w) + w.o_cap + 8 * i;
         if (sphere_capsule(o, bx, by, bz, R) < 0.f)
             for (int j = j0; j < j1; ++j) {
@@ -236,8 +277,9 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
                 centre_rt(T, j, x, y, z);
                 hit |= sphere_capsule(o, x, y, z, kTab.r[j]) < 0.f;
             }
-    }
-    for (int i = 0; i < w.n_zcap; ++i) {
+    });
+    first += w.n_cap;
+    for_touched(touch, first, w.n_zcap, all, [&](int i) {
         cfloat *o = world_data(w) + w.o_zcap + 8 * i;
         if (sphere_zcapsule(o, bx, by, bz, R) < 0.f)
             for (int j = j0; j < j1; ++j) {
@@ -245,8 +287,9 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
                 centre_rt(T, j, x, y, z);
                 hit |= sphere_zcapsule(o, x, y, z, kTab.r[j]) < 0.f;
             }
-    }
-    for (int i = 0; i < w.n_aabb; ++i) {
+    });
+    first += w.n_zcap;
+    for_touched(touch, first, w.n_aabb, all, [&](int i) {
         cfloat *o = world_data(w) + w.o_aabb + 6 * i;
         if (sphere_aabb(o, bx, by, bz, Rsq) < 0.f)
             for (int j = j0; j < j1; ++j) {
@@ -254,8 +297,9 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
                 centre_rt(T, j, x, y, z);
                 hit |= sphere_aabb(o, x, y, z, kTab.r[j] * kTab.r[j]) < 0.f;
             }
-    }
-    for (int i = 0; i < w.n_obb; ++i) {
+    });
+    first += w.n_aabb;
+    for_touched(touch, first, w.n_obb, all, [&](int i) {
         cfloat *o = world_data(w) + w.o_obb + 15 * i;
         if (sphere_obb(o, bx, by, bz, Rsq) < 0.f)
             for (int j = j0; j < j1; ++j) {
@@ -263,8 +307,9 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
                 centre_rt(T, j, x, y, z);
                 hit |= sphere_obb(o, x, y, z, kTab.r[j] * kTab.r[j]) < 0.f;
             }
-    }
-    for (int i = 0; i < w.n_zcub; ++i) {
+    });
+    first += w.n_obb;
+    for_touched(touch, first, w.n_zcub, all, [&](int i) {
         cfloat *o = world_data(w) + w.o_zcub + 15 * i;
         if (sphere_zcuboid(o, bx, by, bz, Rsq) < 0.f)
             for (int j = j0; j < j1; ++j) {
@@ -272,7 +317,7 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
                 centre_rt(T, j, x, y, z);
                 hit |= sphere_zcuboid(o, x, y, z, kTab.r[j] * kTab.r[j]) < 0.f;
             }
-    }
+    });
     return hit;
 }
 
@@ -285,7 +330,8 @@ __device__ __forceinline__ bool block_env(const WorldDesc &w, const Frame &T, in
 // environment, so that at most 17 tested centres are live at a time.
 template <int MODE, int G0, int G1, bool WITH_ENV, int NT>
 __device__ __forceinline__ bool fine_walk(const WorldDesc &w, const Frame &base, const float q[7], unsigned selfbits,
-                                          unsigned envbits, const float (*cs)[3][NT], int lane) {
+                                          unsigned envbits, const float (*cs)[3][NT], int lane,
+                                          const unsigned long long (&touch)[kCoarseSpheres], bool all) {
     bool hit = false;
     float px[kFineSelfRanges], py[kFineSelfRanges], pz[kFineSelfRanges];
     Frame T = base;
@@ -316,7 +362,8 @@ __device__ __forceinline__ bool fine_walk(const WorldDesc &w, const Frame &base,
                     constexpr int k = decltype(kc)::value;
                     if constexpr (kCoarseJoint[k] == J) {
                         constexpr float R = kCoarseSphere[k][3] + kBlockInflate[k];
-                        hit |= block_env(w, T, k, cs[k][0][lane], cs[k][1][lane], cs[k][2][lane], R);
+                        hit |= block_env(w, T, k, cs[k][0][lane], cs[k][1][lane], cs[k][2][lane], R,
+                                         uniform_u64(touch[k]), all);
                     }
                 });
             }
@@ -395,7 +442,9 @@ constexpr float kBallMargin = 1e-4f;
 // Shared memory of one tile of NT items (sample, arm); KH = arm-arm
 // half-pairs per item.
 template <int KH, int NT> struct TileLds {
-    float cs[kCoarseSpheres][3][NT];
+    alignas(16) float cs[kCoarseSpheres][3][NT];
+    float hull[NT / 64][kCoarseSpheres - 1][6];             // per wave: box of the centres of spheres 1..10
+    unsigned long long touch[NT / 64][kCoarseSpheres];      // per wave and block: candidate obstacles 0..63
     float ball[4][NT];                   // per item: its arm's bounding ball (centre, radius)
     unsigned inter_flags[NT];
     unsigned result[NT];
@@ -426,9 +475,12 @@ __device__ __forceinline__ unsigned group_waves(const EdgeGroup &g, int single, 
 // S = NT / A, thread arm * S + slot holding arm `arm` of sample `slot`.  The
 // waves are arm-major, so the environment and arm-pair culls of a wave see
 // one arm of 64 nearby samples.
-// SPEC5: evaluate the joint-5 gated fine ranges during the coarse FK (the
-// one-tile-per-block kernels); otherwise in the fine walk of phase B.
-template <int A, int MODE, int KH, int NT, bool SPEC5>
+// LEAN (the one-tile-per-block kernels, whose registers are not held across
+// a tile loop): evaluate the joint-5 gated fine ranges during the coarse FK
+// rather than in the fine walk of phase B, and record per block the
+// obstacles its ball touches, so that the fine environment tests visit only
+// those.
+template <int A, int MODE, int KH, int NT, bool LEAN>
 __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc *worlds, const EdgeGroup &eb,
                                            unsigned lw, const FineQueue &fq, int pass) {
     constexpr int S = NT / A;  // samples per tile
@@ -481,7 +533,7 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
                 constexpr float ox = kCoarseSphere[s][0], oy = kCoarseSphere[s][1], oz = kCoarseSphere[s][2];
                 centre(T, ox, oy, oz, cx[s], cy[s], cz[s]);
             });
-            if constexpr (SPEC5 && J == 5)
+            if constexpr (LEAN && J == 5)
                 static_for<0, kFineSelfRanges>([&](auto rc) {
                     constexpr int r = decltype(rc)::value;
                     if constexpr (range_gate<MODE>(r) == 5) {
@@ -489,7 +541,7 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
                         centre(T, kFineSphere[ps][0], kFineSphere[ps][1], kFineSphere[ps][2], px[r], py[r], pz[r]);
                     }
                 });
-            if constexpr (SPEC5 && J == 7)
+            if constexpr (LEAN && J == 7)
                 static_for<0, kFineSelfRanges>([&](auto rc) {
                     constexpr int r = decltype(rc)::value;
                     if constexpr (range_gate<MODE>(r) == 5) {
@@ -555,8 +607,9 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
     }
 
     // coarse environment: spheres 1..10 -> env bit of their joint (panda_four.cuh:241-261)
+    // (the lean kernels run it after barrier (1), below)
     unsigned hitmask = 0;
-    if constexpr (kEnv) {
+    if constexpr (kEnv && !LEAN) {
         // Box of this lane's tested centres (spheres 1..10): an obstacle whose
         // cull box, grown by the largest of their radii, misses it cannot hit
         // any of them.
@@ -577,52 +630,50 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
             return !(hx < bb[0] - g || hy < bb[1] - g || hz < bb[2] - g || lx > bb[3] + g || ly > bb[4] + g ||
                      lz > bb[5] + g);
         };
-        auto test_all = [&](auto &&fn) {
+        // obstacle of radius a (0 for cuboids) against spheres 1..10: the
+        // coarse hit is q - (a + r)^2 < 0 for the obstacle's distance term q
+        auto test_all = [&](float a, auto &&qf) {
             static_for<1, kCoarseSpheres>([&](auto sc) {
                 constexpr int s = decltype(sc)::value;
                 constexpr float r = kCoarseSphere[s][3];
-                if (fn(cx[s], cy[s], cz[s], r) < 0.f) hitmask |= 1u << s;
+                const float q = qf(cx[s], cy[s], cz[s]);
+                const float rs = a + r;
+                if (q - rs * rs < 0.f) hitmask |= 1u << s;
             });
         };
         int ob = 0;
         for (int i = 0; i < w.n_sph; ++i, ++ob) {
             if (!near(ob)) continue;
             cfloat *o = world_data(w) + w.o_sph + 4 * i;
-            const float ox = o[0], oy = o[1], oz = o[2], orr = o[3];
-            test_all([&](float x, float y, float z, float r) { return sphere_sphere(ox, oy, oz, orr, x, y, z, r); });
+            const float ox = o[0], oy = o[1], oz = o[2];
+            test_all(o[3], [&](float x, float y, float z) { return sphere_sphere_q(ox, oy, oz, x, y, z); });
         }
         for (int i = 0; i < w.n_cap; ++i, ++ob) {
             if (!near(ob)) continue;
             cfloat *o = world_data(w) + w.o_cap + 8 * i;
-            test_all([&](float x, float y, float z, float r) { return sphere_capsule(o, x, y, z, r); });
+            test_all(o[6], [&](float x, float y, float z) { return capsule_q(o, x, y, z); });
         }
         for (int i = 0; i < w.n_zcap; ++i, ++ob) {
             if (!near(ob)) continue;
             cfloat *o = world_data(w) + w.o_zcap + 8 * i;
-            test_all([&](float x, float y, float z, float r) { return sphere_zcapsule(o, x, y, z, r); });
+            test_all(o[6], [&](float x, float y, float z) { return zcapsule_q(o, x, y, z); });
         }
         for (int i = 0; i < w.n_aabb; ++i, ++ob) {
             if (!near(ob)) continue;
             cfloat *o = world_data(w) + w.o_aabb + 6 * i;
-            test_all([&](float x, float y, float z, float r) { return sphere_aabb(o, x, y, z, r * r); });
+            test_all(0.f, [&](float x, float y, float z) { return aabb_q(o, x, y, z); });
         }
         for (int i = 0; i < w.n_obb; ++i, ++ob) {
             if (!near(ob)) continue;
             cfloat *o = world_data(w) + w.o_obb + 15 * i;
-            test_all([&](float x, float y, float z, float r) { return sphere_obb(o, x, y, z, r * r); });
+            test_all(0.f, [&](float x, float y, float z) { return obb_q(o, x, y, z); });
         }
         for (int i = 0; i < w.n_zcub; ++i, ++ob) {
             if (!near(ob)) continue;
             cfloat *o = world_data(w) + w.o_zcub + 15 * i;
-            test_all([&](float x, float y, float z, float r) { return sphere_zcuboid(o, x, y, z, r * r); });
+            test_all(0.f, [&](float x, float y, float z) { return zcuboid_q(o, x, y, z); });
         }
     }
-    unsigned envbits = 0;
-    static_for<1, kCoarseSpheres>([&](auto sc) {
-        constexpr int s = decltype(sc)::value;
-        constexpr unsigned bit = 1u << kCoarseJoint[s];
-        if (hitmask & (1u << s)) envbits |= bit;
-    });
 
     // coarse intra-arm pairs -> self bit of the first sphere's joint (panda_four.cuh:144-176)
     unsigned selfbits = 0;
@@ -645,6 +696,101 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
     sm.result[lane] = selfbits;  // intra-arm part of the self bits
     sm.edge_done[lane] = 0u;
     __syncthreads();
+
+    // Lean kernels: the coarse environment tests, with this wave's centres in
+    // LDS.  (a) Lanes 0..29 take the hull (box) of one coordinate of one of
+    // spheres 1..10 over the wave's 64 lanes.  (b) One lane per obstacle:
+    // bit s of `cand` is set if the obstacle's cull box, grown by block s's
+    // radius, meets sphere s's hull; a sphere or block ball whose centre lies
+    // outside it cannot hit or touch the obstacle (the cull box property).
+    // (c) The exact tests of each lane's spheres against the candidates, the
+    // reference's q - (a + r)^2 < 0.  The candidate masks, per block, are also
+    // the obstacles the fine environment tests visit.
+    if constexpr (kEnv && LEAN) {
+        const int wv = lane >> 6, wl = lane & 63;
+        if (wl < 3 * (kCoarseSpheres - 1)) {
+            const int s = 1 + wl / 3, c = wl - 3 * (s - 1);
+            const float *v = &sm.cs[s][c][64 * wv];
+            float lo = v[0], hi = v[0];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const float4 x = *(const float4 *)(v + 4 * ((m + wl) & 15));  // rotated: spread over the banks
+                lo = fminf(lo, fminf(fminf(x.x, x.y), fminf(x.z, x.w)));
+                hi = fmaxf(hi, fmaxf(fmaxf(x.x, x.y), fmaxf(x.z, x.w)));
+            }
+            sm.hull[wv][s - 1][c] = lo;
+            sm.hull[wv][s - 1][3 + c] = hi;
+        }
+        wave_sync();
+        const int n_obs = w.n_sph + w.n_cap + w.n_zcap + w.n_aabb + w.n_obb + w.n_zcub;
+        const int e_sph = w.n_sph, e_cap = e_sph + w.n_cap, e_zcap = e_cap + w.n_zcap, e_aabb = e_zcap + w.n_aabb,
+                  e_obb = e_aabb + w.n_obb;
+        cfloat *box = world_data(w) + w.o_box;
+        for (int first = 0; first < n_obs; first += 64) {
+            unsigned cand = 0;
+            if (first + wl < n_obs) {
+                cfloat *bb = box + 8 * (first + wl);
+                const float b0 = bb[0], b1 = bb[1], b2 = bb[2], b3 = bb[3], b4 = bb[4], b5 = bb[5], f = bb[6];
+                static_for<1, kCoarseSpheres>([&](auto sc) {
+                    constexpr int s = decltype(sc)::value;
+                    constexpr float R = kCoarseSphere[s][3] + kBlockInflate[s];
+                    const float g = R * f;
+                    const float *h = sm.hull[wv][s - 1];
+                    if (!(h[3] < b0 - g || h[4] < b1 - g || h[5] < b2 - g || h[0] > b3 + g || h[1] > b4 + g ||
+                          h[2] > b5 + g))
+                        cand |= 1u << s;
+                });
+            }
+            if (first == 0)
+                static_for<1, kCoarseSpheres>([&](auto sc) {
+                    constexpr int s = decltype(sc)::value;
+                    sm.touch[wv][s] = __ballot((cand >> s) & 1u);
+                });
+            // obstacle of radius a (0 for cuboids) against the spheres in sb
+            auto tests = [&](unsigned sb, float a, auto &&qf) {
+                static_for<1, kCoarseSpheres>([&](auto sc) {
+                    constexpr int s = decltype(sc)::value;
+                    if (sb & (1u << s)) {
+                        constexpr float r = kCoarseSphere[s][3];
+                        const float q = qf(cx[s], cy[s], cz[s]);
+                        const float rs = a + r;
+                        if (q - rs * rs < 0.f) hitmask |= 1u << s;
+                    }
+                });
+            };
+            for (unsigned long long u = __ballot(cand != 0u); u; u &= u - 1ull) {
+                const int j = __builtin_ctzll(u);
+                const unsigned sb = __builtin_amdgcn_readlane(cand, j);
+                const int ob = first + j;
+                if (ob < e_sph) {
+                    cfloat *o = world_data(w) + w.o_sph + 4 * ob;
+                    const float ox = o[0], oy = o[1], oz = o[2];
+                    tests(sb, o[3], [&](float x, float y, float z) { return sphere_sphere_q(ox, oy, oz, x, y, z); });
+                } else if (ob < e_cap) {
+                    cfloat *o = world_data(w) + w.o_cap + 8 * (ob - e_sph);
+                    tests(sb, o[6], [&](float x, float y, float z) { return capsule_q(o, x, y, z); });
+                } else if (ob < e_zcap) {
+                    cfloat *o = world_data(w) + w.o_zcap + 8 * (ob - e_cap);
+                    tests(sb, o[6], [&](float x, float y, float z) { return zcapsule_q(o, x, y, z); });
+                } else if (ob < e_aabb) {
+                    cfloat *o = world_data(w) + w.o_aabb + 6 * (ob - e_zcap);
+                    tests(sb, 0.f, [&](float x, float y, float z) { return aabb_q(o, x, y, z); });
+                } else if (ob < e_obb) {
+                    cfloat *o = world_data(w) + w.o_obb + 15 * (ob - e_aabb);
+                    tests(sb, 0.f, [&](float x, float y, float z) { return obb_q(o, x, y, z); });
+                } else {
+                    cfloat *o = world_data(w) + w.o_zcub + 15 * (ob - e_obb);
+                    tests(sb, 0.f, [&](float x, float y, float z) { return zcuboid_q(o, x, y, z); });
+                }
+            }
+        }
+    }
+    unsigned envbits = 0;
+    static_for<1, kCoarseSpheres>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr unsigned bit = 1u << kCoarseJoint[s];
+        if (hitmask & (1u << s)) envbits |= bit;
+    });
 
     // coarse arm-arm pairs (panda_four.cuh:178-214), A-1 half-pairs per lane.
     // The same distances give the candidate blocks of the fine arm-arm check:
@@ -728,18 +874,24 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
 
     // ---- B. fine intra-arm pairs of this lane's arm
     bool finehit = false;
+    // candidate obstacles of the fine environment tests: the lean kernels'
+    // per-block masks (worlds of at most 64 obstacles), else every obstacle
+    const bool all_obs = !LEAN || w.n_sph + w.n_cap + w.n_zcap + w.n_aabb + w.n_obb + w.n_zcub > 64;
+    const unsigned long long(&touch)[kCoarseSpheres] = sm.touch[lane >> 6];
     if (active && (selfbits | envbits)) {
         float q[7];
         interp7(eb, e, k, dof_off, q);
         Frame base;
         load_base(w, base_arm, base);
-        if (selfbits & 0x1fu) finehit = fine_walk<MODE, 0, 4, false, NT>(w, base, q, selfbits, 0u, sm.cs, lane);
-        if constexpr (SPEC5) {
+        if (selfbits & 0x1fu)
+            finehit = fine_walk<MODE, 0, 4, false, NT>(w, base, q, selfbits, 0u, sm.cs, lane, touch, true);
+        if constexpr (LEAN) {
             if (selfbits & 0x20u) finehit |= self5hit;  // the joint-5 ranges, evaluated in phase A
             // environment only (no self range has a gate in 6..5)
-            if (kEnv && envbits) finehit |= fine_walk<MODE, 6, 5, kEnv, NT>(w, base, q, 0u, envbits, sm.cs, lane);
+            if (kEnv && envbits)
+                finehit |= fine_walk<MODE, 6, 5, kEnv, NT>(w, base, q, 0u, envbits, sm.cs, lane, touch, all_obs);
         } else if ((selfbits & 0x20u) || envbits) {
-            finehit |= fine_walk<MODE, 5, 5, kEnv, NT>(w, base, q, selfbits, envbits, sm.cs, lane);
+            finehit |= fine_walk<MODE, 5, 5, kEnv, NT>(w, base, q, selfbits, envbits, sm.cs, lane, touch, all_obs);
         }
     }
     if (active && finehit) done[le] = 1u;

@@ -124,59 +124,82 @@ struct WorldDesc {
 using cfloat = const __attribute__((address_space(4))) float;
 __device__ __forceinline__ cfloat *world_data(const WorldDesc &w) { return (cfloat *)w.obs; }
 
+// Each test below is q - (a + r)^2 < 0 for a radius-independent term q of
+// the obstacle and the sphere centre (squared distance to the obstacle's
+// core) and the obstacle's own radius a (0 for cuboids).  The *_q functions
+// give q alone, so that one evaluation serves several radii.
+
 // utils.cuh:100-107
+__device__ __forceinline__ float sphere_sphere_q(float ax, float ay, float az, float bx, float by, float bz) {
+    float dx = ax - bx, dy = ay - by, dz = az - bz;
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
 __device__ __forceinline__ float sphere_sphere(float ax, float ay, float az, float ar, float bx, float by, float bz,
                                                float br) {
-    float dx = ax - bx, dy = ay - by, dz = az - bz;
     float rs = ar + br;
-    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) - rs * rs;
+    return sphere_sphere_q(ax, ay, az, bx, by, bz) - rs * rs;
 }
 
 // utils.cuh:148-158
-template <class P> __device__ __forceinline__ float sphere_capsule(P c, float x, float y, float z, float r) {
+template <class P> __device__ __forceinline__ float capsule_q(P c, float x, float y, float z) {
     float dot = (x - c[0]) * c[3] + (y - c[1]) * c[4] + (z - c[2]) * c[5];
     float t = fmaxf(fminf(dot * c[7], 1.f), 0.f);
     float dx = x - __builtin_fmaf(c[3], t, c[0]);
     float dy = y - __builtin_fmaf(c[4], t, c[1]);
     float dz = z - __builtin_fmaf(c[5], t, c[2]);
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
+template <class P> __device__ __forceinline__ float sphere_capsule(P c, float x, float y, float z, float r) {
     float rs = r + c[6];
-    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) - rs * rs;
+    return capsule_q(c, x, y, z) - rs * rs;
 }
 
 // utils.cuh:165-174
-template <class P> __device__ __forceinline__ float sphere_zcapsule(P c, float x, float y, float z, float r) {
+template <class P> __device__ __forceinline__ float zcapsule_q(P c, float x, float y, float z) {
     float dot = (z - c[2]) * c[5];
     float t = fmaxf(fminf(dot * c[7], 1.f), 0.f);
     float dx = x - c[0], dy = y - c[1], dz = z - __builtin_fmaf(c[5], t, c[2]);
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
+template <class P> __device__ __forceinline__ float sphere_zcapsule(P c, float x, float y, float z, float r) {
     float rs = r + c[6];
-    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) - rs * rs;
+    return zcapsule_q(c, x, y, z) - rs * rs;
 }
 
 // utils.cuh:185-201 for a cuboid whose axes are the world axes: the axis dot
 // products are then exactly the coordinate differences.
-template <class P> __device__ __forceinline__ float sphere_aabb(P b, float x, float y, float z, float rsq) {
+template <class P> __device__ __forceinline__ float aabb_q(P b, float x, float y, float z) {
     float a1 = fmaxf(fabsf(x - b[0]) - b[3], 0.f);
     float a2 = fmaxf(fabsf(y - b[1]) - b[4], 0.f);
     float a3 = fmaxf(fabsf(z - b[2]) - b[5], 0.f);
-    return __builtin_fmaf(a1, a1, __builtin_fmaf(a2, a2, a3 * a3)) - rsq;
+    return __builtin_fmaf(a1, a1, __builtin_fmaf(a2, a2, a3 * a3));
+}
+template <class P> __device__ __forceinline__ float sphere_aabb(P b, float x, float y, float z, float rsq) {
+    return aabb_q(b, x, y, z) - rsq;
 }
 
 // utils.cuh:185-201
-template <class P> __device__ __forceinline__ float sphere_obb(P b, float x, float y, float z, float rsq) {
+template <class P> __device__ __forceinline__ float obb_q(P b, float x, float y, float z) {
     float xs = x - b[0], ys = y - b[1], zs = z - b[2];
     float a1 = fmaxf(fabsf(__builtin_fmaf(b[3], xs, __builtin_fmaf(b[4], ys, b[5] * zs))) - b[12], 0.f);
     float a2 = fmaxf(fabsf(__builtin_fmaf(b[6], xs, __builtin_fmaf(b[7], ys, b[8] * zs))) - b[13], 0.f);
     float a3 = fmaxf(fabsf(__builtin_fmaf(b[9], xs, __builtin_fmaf(b[10], ys, b[11] * zs))) - b[14], 0.f);
-    return __builtin_fmaf(a1, a1, __builtin_fmaf(a2, a2, a3 * a3)) - rsq;
+    return __builtin_fmaf(a1, a1, __builtin_fmaf(a2, a2, a3 * a3));
+}
+template <class P> __device__ __forceinline__ float sphere_obb(P b, float x, float y, float z, float rsq) {
+    return obb_q(b, x, y, z) - rsq;
 }
 
 // utils.cuh:208-221
-template <class P> __device__ __forceinline__ float sphere_zcuboid(P b, float x, float y, float z, float rsq) {
+template <class P> __device__ __forceinline__ float zcuboid_q(P b, float x, float y, float z) {
     float xs = x - b[0], ys = y - b[1], zs = z - b[2];
     float a1 = fmaxf(fabsf(__builtin_fmaf(b[3], xs, b[4] * ys)) - b[12], 0.f);
     float a2 = fmaxf(fabsf(__builtin_fmaf(b[6], xs, b[7] * ys)) - b[13], 0.f);
     float a3 = fmaxf(fabsf(zs) - b[14], 0.f);
-    return __builtin_fmaf(a1, a1, __builtin_fmaf(a2, a2, a3 * a3)) - rsq;
+    return __builtin_fmaf(a1, a1, __builtin_fmaf(a2, a2, a3 * a3));
+}
+template <class P> __device__ __forceinline__ float sphere_zcuboid(P b, float x, float y, float z, float rsq) {
+    return zcuboid_q(b, x, y, z) - rsq;
 }
 
 // True iff the sphere touches any obstacle (utils.cuh:231-260).

@@ -103,6 +103,70 @@ def test_live_oracle_parity(ctx, kind, G):
         _assert_masks(mask, edge, r["mask_lo"], r["mask_hi"], f"{kind} single arm {arm} G {G}")
 
 
+def _crowded_scene(rng, n):
+    """n small obstacles of all six record types (spheres, capsules,
+    z-capsules, world-axis cuboids, rotated cuboids, z-cuboids), in turn, in
+    a box that grows with n."""
+    from mrpop import scenes
+    sph, cap, zcap, cub, zcub = [], [], [], [], []
+    k = (n / 64.0) ** 0.5
+    for i in range(n):
+        c = rng.uniform((-1.2 * k, -1.2 * k, 0.0), (1.2 * k, 1.2 * k, 1.3 * k))
+        s = rng.uniform(0.01, 0.03)
+        kind = i % 6
+        if kind == 0:
+            sph.append(scenes.sphere_record(c, s))
+        elif kind == 1:
+            cap.append(scenes.capsule_record(c, tuple(rng.uniform(-np.pi, np.pi, 3)), s, rng.uniform(0.02, 0.15)))
+        elif kind == 2:
+            h = rng.uniform(0.02, 0.2)
+            zcap.append([c[0], c[1], c[2], 0.0, 0.0, h, s, 1.0 / (h * h)])
+        elif kind == 3:
+            cub.append(scenes.cuboid_record(c, (0.0, 0.0, 0.0), rng.uniform(0.01, 0.04, 3)))
+        elif kind == 4:
+            cub.append(scenes.cuboid_record(c, tuple(rng.uniform(-np.pi, np.pi, 3)), rng.uniform(0.01, 0.04, 3)))
+        else:
+            yaw = rng.uniform(-np.pi, np.pi)
+            cz, sz = np.cos(yaw), np.sin(yaw)
+            hx, hy, hz = rng.uniform(0.01, 0.04, 3)
+            zcub.append([c[0], c[1], c[2], cz, sz, 0.0, -sz, cz, 0.0, 0.0, 0.0, 1.0, hx, hy, hz])
+    f = lambda rows, k: np.array(rows, np.float32).reshape(-1, k)
+    return scenes.Scene(f(sph, 4), f(cap, 8), f(zcap, 8), f(cub, 15), f(zcub, 15))
+
+
+@pytest.mark.parametrize("n_obstacles", [63, 64, 65, 150])
+def test_crowded_scene_parity(ctx, n_obstacles):
+    """Worlds around the checker's 64-obstacle chunk: per-chunk candidate
+    masks up to 64 obstacles, several chunks and a full fine walk above."""
+    from oracle import oracle as O
+    from mrpop import scenes
+    from mrpop._lib import gpu_checker
+    p = scenes.make_problem("bin4", 11, gpu_checker(ctx), n_goals=7)
+    rng = np.random.default_rng(n_obstacles)
+    scene = _crowded_scene(rng, n_obstacles)
+    assert scene.n_obstacles == n_obstacles
+    starts = np.concatenate([p.start[None], p.goals])
+    frm, to = _random_edges(rng, starts, 128, p.dim, 0.02, 0.8)
+    print(f"[gpu] crowded scene n={n_obstacles}", flush=True)
+    w = ctx.world(scene, p.n_arms, p.bases)
+    G = 32
+    masks = {}
+    for mode in (O.MODE_FULL, O.MODE_DRRT):
+        edge, mask = w.check_edges(mode, G, frm, to, with_mask=True)
+        r = O.check_edges(scene, mode, p.bases, G, frm, to)
+        _assert_masks(mask, edge, r["mask_lo"], r["mask_hi"], f"crowded {n_obstacles} mode {mode}")
+        assert np.array_equal(w.check_edges(mode, G, frm, to), edge)
+        masks[mode] = mask.astype(np.uint64)
+    # samples colliding only with the environment, and free samples, both occur
+    env_only = np.unpackbits((masks[O.MODE_FULL] & ~masks[O.MODE_DRRT]).view(np.uint8)).mean()
+    free = 1.0 - np.unpackbits(masks[O.MODE_FULL].view(np.uint8)).mean() * 64 / G
+    assert env_only > 0.05 / 2 and free > 0.05, (env_only, free)
+    f1, t1 = frm[:, 7:14], to[:, 7:14]
+    edge, mask = w.check_edges(O.MODE_SINGLE, G, f1, t1, arm=1, with_mask=True)
+    r = O.check_edges(scene, O.MODE_SINGLE, p.bases, G, f1, t1, arm=1)
+    _assert_masks(mask, edge, r["mask_lo"], r["mask_hi"], f"crowded {n_obstacles} single")
+
+
 def test_fixture_configs_free_on_gpu(ctx):
     """Every start/goal of the reference's two problem files is collision-free
     (the files mark all 100 problems valid)."""
