@@ -55,6 +55,17 @@ def make_edges(problem, checker, n_edges, seed):
     return frm.astype(np.float32), to
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(problem, frm, to, G, budget_s=12.0):
     """Oracle FP32 restatement of the reference checker (early exits), timed
     on a bounded sample of the same edges: 1 thread, then all cores."""
@@ -96,6 +107,10 @@ def measured_traffic(n_edges, G):
 
 
 PLAN_KINDS = ("plain2", "table3", "bin4", "shelf5")   # 2 / 3 / 4 / 5 arms, ratio 1:1:1:1
+SCENE_KINDS = ("bin4", "shelf5cell")   # 4 arms around a bin; 5 arms reaching into shelf cells
+# per-problem time budget of the fixed-work scene lines: the 5-arm shelf-cell
+# problems that go unsolved search the composite roadmap until the budget ends
+SCENE_BUDGET_S = {"bin4": 30.0, "shelf5cell": 10.0}
 
 
 def plan_problems(checker, n, first_seed=0):
@@ -119,7 +134,7 @@ def _stats(sols, ttfs, wall):
 
 
 def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_rates, serve_s, tune=True,
-                   n_budget=0, batch_kc=16, batch_probe=0):
+                   n_budget=0, batch_kc=16, batch_probe=0, n_scene=0, n_scene_plain=8, scene_kinds=SCENE_KINDS):
     """Plain vs batched planning on the same seeded problems, a budgeted
     batch, and serving."""
     from mrpop import _lib, autotune
@@ -156,25 +171,24 @@ def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_r
             plain_ttfs.append(start + s.ttfs_s)
             own_ttfs.append(s.ttfs_s)
     plain = _stats(plain_sols, plain_ttfs, time.perf_counter() - t0) if plain_sols else None
-    if plain is None:
+    if plain is not None:
+        plain["ttfs_own_p50_s"] = float(np.percentile(own_ttfs, 50)) if own_ttfs else None
+        plain["ttfs_own_p99_s"] = float(np.percentile(own_ttfs, 99)) if own_ttfs else None
+        print(f"[bench] plain: {plain}", file=sys.stderr, flush=True)
+        out["plain"] = plain
+    # batched: every problem in one engine, all submitted at t = 0
+    if n_batch:
         t0 = time.perf_counter()
         batch_sols = _lib.solve_batch(ctx, probs[:n_batch], [st_batch] * n_batch, [op_batch] * n_batch)
-        out["batched"] = _stats(batch_sols, [s.ttfs_s for s in batch_sols if s.solved], time.perf_counter() - t0)
-        return out
-    plain["ttfs_own_p50_s"] = float(np.percentile(own_ttfs, 50)) if own_ttfs else None
-    plain["ttfs_own_p99_s"] = float(np.percentile(own_ttfs, 99)) if own_ttfs else None
-    print(f"[bench] plain: {plain}", file=sys.stderr, flush=True)
-    # batched: every problem in one engine, all submitted at t = 0
-    t0 = time.perf_counter()
-    batch_sols = _lib.solve_batch(ctx, probs[:n_batch], [st_batch] * n_batch, [op_batch] * n_batch)
-    wall = time.perf_counter() - t0
-    batched = _stats(batch_sols, [s.ttfs_s for s in batch_sols if s.solved], wall)
-    both = [(a, b) for a, b in zip(plain_sols, batch_sols) if a.solved and b.solved]
-    batched["median_cost_on_plain_problems"] = float(np.median([b.cost for _, b in both])) if both else None
-    plain["median_cost_on_plain_problems"] = float(np.median([a.cost for a, _ in both])) if both else None
-    print(f"[bench] batched: {batched}", file=sys.stderr, flush=True)
-    out.update(plain=plain, batched=batched,
-               batched_vs_plain_problems_per_s=batched["problems_per_s"] / plain["problems_per_s"])
+        wall = time.perf_counter() - t0
+        batched = _stats(batch_sols, [s.ttfs_s for s in batch_sols if s.solved], wall)
+        print(f"[bench] batched: {batched}", file=sys.stderr, flush=True)
+        out["batched"] = batched
+        if plain is not None:
+            both = [(a, b) for a, b in zip(plain_sols, batch_sols) if a.solved and b.solved]
+            batched["median_cost_on_plain_problems"] = float(np.median([b.cost for _, b in both])) if both else None
+            plain["median_cost_on_plain_problems"] = float(np.median([a.cost for a, _ in both])) if both else None
+            out["batched_vs_plain_problems_per_s"] = batched["problems_per_s"] / plain["problems_per_s"]
     if n_budget:
         # the batched configuration with a fixed 2 s budget per problem and
         # the long optimisation passes of the 4-/5-arm configurations
@@ -188,6 +202,96 @@ def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_r
         print(f"[bench] batched, 2 s budget: {out['batched_2s_budget']}", file=sys.stderr, flush=True)
     op_kw = {k: v for k, v in tuned.items() if k != "num_new_configs"}
     out["serving"] = [serving_bench(ctx, checker, rate, serve_s, st_batch, op_kw) for rate in serve_rates]
+    if n_scene:
+        out["scenes"] = scene_bench(ctx, checker, n_scene, max(1, n_scene_plain), op_kw, kinds=scene_kinds)
+    out["two_arm_plain"] = two_arm_plain_bench(ctx, checker)
+    return out
+
+
+def _ttfs_pcts(ttfs, prefix):
+    return {f"{prefix}_p50_s": float(np.percentile(ttfs, 50)) if len(ttfs) else None,
+            f"{prefix}_p99_s": float(np.percentile(ttfs, 99)) if len(ttfs) else None}
+
+
+def scene_bench(ctx, checker, n_problems, n_plain, option_kw, budget_s=2.0, kinds=SCENE_KINDS):
+    """Per scene: n_problems seeded start/goal pairs at granularity 32,
+    num_new_configs 1024, rrtc_iter 6, range 2.0.
+
+    * fixed work (optimize_iters 2000, per-problem budget SCENE_BUDGET_S):
+      the plain form (one problem at a time, all submitted at t = 0) on the
+      first n_plain problems and the batched form (one engine, all
+      submitted at t = 0) on all of them;
+    * optimize_iters 400000 under a per-problem budget of budget_s, batched.
+
+    TTFS percentiles are over the solved problems ("own": from the problem's
+    own start; "queued": from t = 0)."""
+    from mrpop import _lib, scenes
+    out = []
+    for kind in kinds:
+        probs = [scenes.make_problem(kind, seed, checker) for seed in range(n_problems)]
+        print(f"[bench] scene {kind}: {n_problems} problems generated", file=sys.stderr, flush=True)
+        common = dict(granularity=32, range=2.0, rrtc_iter=6, num_new_configs=1024)
+        budget = SCENE_BUDGET_S.get(kind, 30.0)
+        st = _lib.settings(optimize_iters=2000, **common)
+        op = _lib.solve_options_default(time_budget_s=budget, **option_kw)
+        res = {"scene": kind, "arms": probs[0].n_arms, "problems": n_problems,
+               "settings": dict(common, optimize_iters=2000, time_budget_s=budget, **option_kw)}
+        sols, own, queued = [], [], []
+        t0 = time.perf_counter()
+        for p in probs[:n_plain]:
+            start = time.perf_counter() - t0
+            s = _lib.solve(ctx, p, st, op)
+            sols.append(s)
+            if s.solved:
+                own.append(s.ttfs_s)
+                queued.append(start + s.ttfs_s)
+            print(f"[bench] scene {kind} plain {len(sols)}/{n_plain}: solved {s.solved} "
+                  f"{time.perf_counter() - t0 - start:.2f} s", file=sys.stderr, flush=True)
+        plain = _stats(sols, own, time.perf_counter() - t0)
+        plain.update(_ttfs_pcts(own, "ttfs_own"), **_ttfs_pcts(queued, "ttfs_queued"))
+        t0 = time.perf_counter()
+        bsols = _lib.solve_batch(ctx, probs, [st] * n_problems, [op] * n_problems)
+        wall = time.perf_counter() - t0
+        bt = [s.ttfs_s for s in bsols if s.solved]
+        batched = _stats(bsols, bt, wall)
+        batched.update(_ttfs_pcts(bt, "ttfs_own"), **_ttfs_pcts(bt, "ttfs_queued"))
+        both = [(a, b) for a, b in zip(sols, bsols) if a.solved and b.solved]
+        plain["median_cost_on_common"] = float(np.median([a.cost for a, _ in both])) if both else None
+        batched["median_cost_on_plain_problems"] = float(np.median([b.cost for _, b in both])) if both else None
+        res.update(plain=plain, batched=batched,
+                   batched_vs_plain_problems_per_s=batched["problems_per_s"] / plain["problems_per_s"])
+        # the long optimisation of the scene's configuration under a fixed budget
+        st_b = _lib.settings(optimize_iters=400000, **common)
+        op_b = _lib.solve_options_default(time_budget_s=budget_s, **option_kw)
+        t0 = time.perf_counter()
+        s2 = _lib.solve_batch(ctx, probs, [st_b] * n_problems, [op_b] * n_problems)
+        b2 = _stats(s2, [s.ttfs_s for s in s2 if s.solved], time.perf_counter() - t0)
+        b2["settings"] = dict(common, optimize_iters=400000, time_budget_s=budget_s)
+        res["batched_budget"] = b2
+        print(f"[bench] scene {kind}: {res}", file=sys.stderr, flush=True)
+        out.append(res)
+    return out
+
+
+def two_arm_plain_bench(ctx, checker, budget_s=5.0):
+    """The 2-arm cuboid scene (seed 0) in the plain form at granularity 32,
+    num_new_configs 64, rrtc_iter 1 and range 0.5 rad, under a budget, and
+    the same problem at range 2.0.  At range 0.5 a roadmap node of 7 joints
+    has a fraction of a neighbour on average, so the arms' roadmaps do not
+    connect start and goal within any practical budget."""
+    from mrpop import _lib, scenes
+    p = scenes.make_problem("plain2", 0, checker)
+    out = {"scene": "plain2 (2 arms, 16 cuboids, seed 0)"}
+    for rng_ in (0.5, 2.0):
+        st = _lib.settings(granularity=32, range=rng_, num_new_configs=64, rrtc_iter=1)
+        op = _lib.solve_options_default(time_budget_s=budget_s)
+        t0 = time.perf_counter()
+        s = _lib.solve(ctx, p, st, op)
+        out[f"range_{rng_}"] = {"solved": s.solved, "cost": s.cost if s.solved else None,
+                                "ttfs_s": s.ttfs_s if s.solved else None, "wall_s": time.perf_counter() - t0,
+                                "status": s.info["status"], "roadmap_rounds": s.info["roadmap_rounds"],
+                                "roadmap_size": s.info["roadmap_size"][:2], "budget_s": budget_s}
+    print(f"[bench] 2-arm plain: {out}", file=sys.stderr, flush=True)
     return out
 
 
@@ -261,6 +365,10 @@ def main():
     ap.add_argument("--batch-candidates", type=int, default=16, help="roadmap candidates per node, untuned runs")
     ap.add_argument("--batch-probe", type=int, default=0, help="probe samples per planner edge, untuned runs")
     ap.add_argument("--plan-only", action="store_true", help="only the planner measurements (profiling runs)")
+    ap.add_argument("--scene-problems", type=int, default=50,
+                    help="seeded problems per scene of the 4-arm bin / 5-arm shelf-cell lines (0: skip)")
+    ap.add_argument("--scene-plain", type=int, default=8, help="of which the plain form runs this many")
+    ap.add_argument("--scenes", type=str, default=",".join(SCENE_KINDS), help="scenes of the per-scene lines")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 256 MiB three times and exit: the reference kernel that scales the HBM counters")
     args = ap.parse_args()
@@ -289,7 +397,8 @@ def main():
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         print(json.dumps(planning_bench(ctx, checker, args.plain_problems, args.batch_problems, args.plain_nnc,
                                         args.batch_nnc, rates, args.serve_seconds, not args.no_tune,
-                                        args.budget_problems, args.batch_candidates, args.batch_probe)),
+                                        args.budget_problems, args.batch_candidates, args.batch_probe,
+                                        args.scene_problems, args.scene_plain, args.scenes.split(","))),
               flush=True)
         return
     problem = scenes.make_problem("bin4", 0, checker)
@@ -364,11 +473,28 @@ def main():
     if not args.no_cpu:
         print("[bench] cpu baseline (bounded sample)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(problem, frm, to, G)
+        out["cpu_baseline"]["cpu_model"] = cpu_model()
+        # the 2-arm cuboid scene's edges: the CPU port on all cores beside this GPU
+        p2 = scenes.make_problem("plain2", 0, checker)
+        f2, t2 = make_edges(p2, checker, args.edges, seed=2)
+        c2 = cpu_baseline(p2, f2, t2, G, budget_s=6.0)
+        w2 = ctx.world(p2.scene, p2.n_arms, p2.bases)
+        w2.check_edges(_lib.MODE_FULL, G, f2, t2)
+        torch.cuda.synchronize()
+        t_g = time.perf_counter()
+        for _ in range(3):
+            w2.check_edges(_lib.MODE_FULL, G, f2, t2)
+        gpu2 = 3 * args.edges / (time.perf_counter() - t_g)
+        out["cpu_baseline"]["two_arm_cuboids"] = {
+            "cpu_edges_per_s": c2["value"], "cpu_single_thread_edges_per_s": c2["single_thread_value"],
+            "cores": c2["cores"], "sample": c2["sample"], "gpu_edges_per_s_host_buffers": gpu2,
+            "workload": "2 Panda arms (14 DoF), 16 cuboids, granularity 32, 65536 edges"}
     if not args.no_plan:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         out["planning"] = planning_bench(ctx, checker, args.plain_problems, args.batch_problems, args.plain_nnc,
                                          args.batch_nnc, rates, args.serve_seconds, not args.no_tune,
-                                         args.budget_problems, args.batch_candidates, args.batch_probe)
+                                         args.budget_problems, args.batch_candidates, args.batch_probe,
+                                         args.scene_problems, args.scene_plain, args.scenes.split(","))
     print(json.dumps(out), flush=True)
 
 

@@ -499,10 +499,17 @@ def make_problem(kind: str, seed: int, checker: Checker, n_goals: int = 1) -> Pr
     rng = np.random.default_rng([seed, {"plain2": 2, "table3": 3, "bin4": 4, "shelf5": 5, "shelf5cell": 6,
                                         "posts3": 7}[kind]])
     if kind == "shelf5cell":
+        # a draw whose cylinders leave an arm no free cell (or no free
+        # composite) is replaced by the next scene of the same stream
         n_arms, bases = 5, default_bases(5)
-        scene = scene_shelf(rng)
-        cfgs = shelf_cell_configs(rng, scene, bases, 1 + n_goals, checker)
-        return Problem(f"{kind}_{seed}", n_arms, bases, scene, cfgs[0], cfgs[1:], seed)
+        for _ in range(16):
+            scene = scene_shelf(rng)
+            try:
+                cfgs = shelf_cell_configs(rng, scene, bases, 1 + n_goals, checker)
+            except RuntimeError:
+                continue
+            return Problem(f"{kind}_{seed}", n_arms, bases, scene, cfgs[0], cfgs[1:], seed)
+        raise RuntimeError(f"{kind} seed {seed}: no scene with free in-cell configurations in 16 draws")
     if kind == "plain2":
         n_arms, bases = 2, default_bases(2)
         scene = scene_random_cuboids(rng, bases)
