@@ -229,7 +229,7 @@ def gpu_checker(ctx: Context):
     cache = {}
 
     def checker(scene, n_arms, bases, cfgs):
-        key = (id(scene), n_arms)
+        key = (id(scene), n_arms, None if bases is None else _f32(bases).tobytes())
         if key not in cache:
             cache.clear()
             cache[key] = World(ctx, scene, n_arms, bases)

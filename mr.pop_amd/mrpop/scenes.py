@@ -400,35 +400,72 @@ def _cell_pools(base, cells, count=512, batch=65536, rounds=8):
     return _CELL_POOLS[key]
 
 
+def reachable(rng: np.random.Generator, scene: Scene, base: np.ndarray, q: np.ndarray, checker: "Checker",
+              reach: float = 2.0, probes: int = 32, points: int = 16, min_free: int = 3) -> np.ndarray:
+    """Which one-arm configurations q [n, 7] can be left in a straight line:
+    of `probes` segments from q to points uniform in the joint-space ball of
+    radius `reach` around it (clipped to the joint limits), at least
+    `min_free` are collision-free for the arm alone (environment + self,
+    checked at `points` evenly spaced points).  A roadmap links a node to
+    the nodes within its range by such segments, so a configuration deep in
+    a shelf cell that sees almost none of that ball (the hand walled in by
+    boards and cylinders) is a start or goal no roadmap connects."""
+    q = np.asarray(q, np.float32).reshape(-1, DOF_PER_ARM)
+    n = len(q)
+    if n == 0:
+        return np.zeros(0, bool)
+    d = rng.normal(size=(n, probes, DOF_PER_ARM))
+    d /= np.linalg.norm(d, axis=2, keepdims=True)
+    r = reach * rng.uniform(0.0, 1.0, (n, probes, 1)) ** (1.0 / DOF_PER_ARM)
+    x = np.clip(q[:, None, :] + d * r, LOW, LOW + SCALE)
+    t = (np.arange(1, points + 1) / points)[None, None, :, None]
+    pts = (q[:, None, None, :] + (x - q[:, None, :])[:, :, None, :] * t).astype(np.float32)
+    coll = checker(scene, 1, np.asarray(base)[None], pts.reshape(-1, DOF_PER_ARM)).reshape(n, probes, points)
+    return (~coll.any(axis=2)).sum(axis=1) >= min_free
+
+
 def shelf_cell_configs(rng: np.random.Generator, scene: Scene, bases: np.ndarray, count: int, checker: "Checker",
-                       tries: int = 64, batch: int = 2048) -> np.ndarray:
+                       tries: int = 64, batch: int = 2048, per_cell: int = 64) -> np.ndarray:
     """`count` collision-free multi-arm configurations with every arm's hand
     inside a shelf cell, the arms in distinct cells (configs[2] of the
     benchmark: end-effectors reaching into random shelf cells).  Per arm and
     cell a pool of in-cell arm configurations is drawn by rejection on the
-    hand position; composites of pool members over random cell assignments
-    are then checked whole."""
+    hand position; up to `per_cell` of them (a random subset) that are free
+    for the arm alone and reachable (see `reachable`) form the cell's pool;
+    composites of pool members over random cell assignments are then checked
+    whole."""
     n_arms = len(bases)
     cells = shelf_cells()
     pools = []
-    for a in range(n_arms):  # keep the arm configurations free on their own (environment + self)
-        row = _cell_pools(bases[a], cells)
-        pools.append([pl[~checker(scene, 1, bases[a:a + 1], pl)] if len(pl) else pl for pl in row])
+    for a in range(n_arms):
+        row = []
+        for pl in _cell_pools(bases[a], cells):
+            if len(pl):
+                pl = pl[rng.permutation(len(pl))[:per_cell]]
+                pl = pl[~checker(scene, 1, bases[a:a + 1], pl)]  # free on its own (environment + self)
+                pl = pl[reachable(rng, scene, bases[a], pl, checker)]
+            row.append(pl)
+        pools.append(row)
     reach = [[k for k, pl in enumerate(pools[a]) if len(pl) >= 2] for a in range(n_arms)]
     if any(not r for r in reach):
         raise RuntimeError("an arm reaches no shelf cell")
     out = []
     for _ in range(tries):
         cand = np.zeros((batch, DOF_PER_ARM * n_arms), np.float32)
+        placed = np.ones(batch, bool)
         for i in range(batch):
             used = set()
             for a in rng.permutation(n_arms):
-                free = [k for k in reach[a] if k not in used] or reach[a]
+                free = [k for k in reach[a] if k not in used]
+                if not free:  # every cell this arm reaches is taken: no composite
+                    placed[i] = False
+                    break
                 k = free[rng.integers(len(free))]
                 used.add(k)
                 pl = pools[a][k]
                 cand[i, DOF_PER_ARM * a:DOF_PER_ARM * (a + 1)] = pl[rng.integers(len(pl))]
-        ok = ~checker(scene, n_arms, bases, cand)
+        cand = cand[placed]
+        ok = ~checker(scene, n_arms, bases, cand) if len(cand) else np.zeros(0, bool)
         out.extend(cand[ok])
         if len(out) >= count:
             return np.stack(out[:count]).astype(np.float32)
