@@ -215,6 +215,20 @@ typedef struct mrpop_solve_options {
                                   same results, no measured gain; not for runs under a kernel tracer) */
     int arm_split;             /* batches of several arm counts: 1 = one check launch per arm count (tiles of
                                   64 samples), 0 = one launch of the mixed-arm kernel (default 0) */
+    int biased_iter;           /* roadmap steering (BIASED_ITER, roadmap_interior.cuh:24, 117-165): while the
+                                  first roadmap rounds of a solve run, every biased_iter-th sample of each
+                                  block of 230 samples per round is moved to within `range` of its nearest
+                                  roadmap node (default 10000 as in the reference; 0: never) */
+    int roadmap_cache;         /* 1: reuse roadmaps across queries of one scene (no counterpart in the reference,
+                                  which rebuilds them per solve, pop_planner.cu:812-830): each arm starts from
+                                  the context's roadmap for the same obstacles, base pose, granularity, range and
+                                  roadmap sizes, if there is one, with the query's start and goals inserted as
+                                  new nodes that take the nearest roadmap nodes within range as candidate
+                                  neighbours; roadmaps of queries whose start and goals are collision-free
+                                  are kept, while at most half the roadmap capacity, when each arm first
+                                  connects them and when the query ends (least recently used ones leave
+                                  beyond 8 GiB per context).  Results then depend on the queries that ran before.
+                                  Default 0 */
 } mrpop_solve_options;
 
 int mrpop_solve_options_default(mrpop_solve_options *o);
@@ -250,6 +264,7 @@ typedef struct mrpop_result_info {
     int edge_table_full;       /* expansions whose duplicate-edge lookup found the table full (checked alone) */
     int connect_skipped;       /* connections longer than 64 chunks per expansion, not attempted */
     int simplify_dropped;      /* simplification candidates beyond the chunk staging, kept as colliding */
+    int roadmap_cached[MRPOP_MAX_ARMS];  /* nodes each arm's roadmap took from the roadmap cache (0: built anew) */
 } mrpop_result_info;
 
 typedef struct mrpop_solution mrpop_solution;

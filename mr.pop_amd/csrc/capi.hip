@@ -304,6 +304,7 @@ int mrpop_destroy(mrpop_ctx *c) {
     for (DevBuf *b : {&c->counters, &c->from, &c->to, &c->out_edge, &c->out_mask, &c->cfg, &c->out_a,
                       &c->out_b})
         if (b->ptr) (void)hipFree(b->ptr);
+    for (auto &kv : c->rm_cache.entries) (void)hipFree(kv.second.mem);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return MRPOP_OK;

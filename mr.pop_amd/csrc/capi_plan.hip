@@ -78,6 +78,8 @@ SolveOptions to_options(const mrpop_solve_options *o) {
         so.probe_samples = o->probe_samples;
         so.graph_rounds = o->graph_rounds;
         so.arm_split = o->arm_split;
+        so.biased_iter = o->biased_iter;
+        so.roadmap_cache = o->roadmap_cache;
     }
     return so;
 }
@@ -234,6 +236,8 @@ int mrpop_solve_options_default(mrpop_solve_options *o) {
     o->probe_samples = d.probe_samples;
     o->graph_rounds = d.graph_rounds;
     o->arm_split = d.arm_split;
+    o->biased_iter = d.biased_iter;
+    o->roadmap_cache = d.roadmap_cache;
     return MRPOP_OK;
 }
 
@@ -287,6 +291,7 @@ int mrpop_solution_info(const mrpop_solution *s, mrpop_result_info *info) {
     info->edge_table_full = r.events[kEvEdgeTable];
     info->connect_skipped = r.events[kEvConnectSkip];
     info->simplify_dropped = r.events[kEvSimplifyDrop];
+    for (int a = 0; a < MRPOP_MAX_ARMS; ++a) info->roadmap_cached[a] = r.roadmap_cached[a];
     return MRPOP_OK;
 }
 
@@ -386,11 +391,12 @@ size_t mrpop_server_bytes(const mrpop_server *srv) { return srv ? srv->engine->d
 // Diagnostics (not part of the public ABI): build the roadmaps of a problem
 // for `rounds` roadmap rounds and copy arm `arm`'s roadmap out (nodes as
 // [n][8] floats, degrees, neighbour lists [n][neighbour_capacity], union-find
-// parents [n] and the arm's start-goal connectivity flag; any output may be
-// NULL); *n gets the node count, at most `cap` nodes are copied.
+// parents [n], the arm's start-goal connectivity flag and the number of
+// rounds of the solve's first roadmap stretch the arm sampled in; any output
+// may be NULL); *n gets the node count, at most `cap` nodes are copied.
 extern "C" int mrpop_diag_roadmap(mrpop_ctx *c, const mrpop_problem *p, const mrpop_settings *settings,
                                   const mrpop_solve_options *options, int rounds, int arm, float *q, int *deg,
-                                  int *nbr, int *uf, int *connected, int *n, int cap) {
+                                  int *nbr, int *uf, int *connected, int *n, int cap, int *stretch_rounds) {
     if (!c || !p || !n || rounds < 0) return MRPOP_ERR_INVALID;
     Context *ctx = (Context *)c;
     if (hipSetDevice(ctx->device) != hipSuccess) return MRPOP_ERR_HIP;
@@ -408,7 +414,10 @@ extern "C" int mrpop_diag_roadmap(mrpop_ctx *c, const mrpop_problem *p, const mr
                             r.goals.data(), r.seed, st, to_options(options), &s))
         return rc;
     if (int rc = engine.debug_roadmap(s, rounds)) return rc;
-    return engine.debug_copy_roadmap(s, arm, q, deg, nbr, uf, connected, n, cap);
+    if (int rc = engine.debug_copy_roadmap(s, arm, q, deg, nbr, uf, connected, n, cap)) return rc;
+    if (stretch_rounds)
+        MRPOP_HIP(ctx, hipMemcpy(stretch_rounds, s->dv.rs_brounds + arm, sizeof(int), hipMemcpyDeviceToHost));
+    return MRPOP_OK;
 }
 
 // Diagnostics (not part of the public ABI): device bytes one problem of

@@ -32,6 +32,8 @@ struct SolveOptions {
     int probe_samples = 0;         // > 0: planner edges are checked at this many samples first (see launch_staged)
     int graph_rounds = 0;          // 1: replay a step's dRRT rounds from a captured HIP graph
     int arm_split = 0;             // 1: one check launch per arm count, not the mixed-arm kernel
+    int biased_iter = 10000;       // BIASED_ITER (roadmap_interior.cuh:24); 0: no biased roadmap samples
+    int roadmap_cache = 0;         // 1: start from, and contribute to, the context's roadmaps of the same scene
 };
 
 struct SeedPath {
@@ -49,6 +51,7 @@ struct SlotResult {
     uint64_t ttfs_ns = 0;          // submit -> first solution
     std::vector<uint64_t> improve_ns;  // submit -> each improving solution
     int passes = 0, rounds = 0, roadmap_rounds = 0;
+    int roadmap_cached[geom::kMaxArms] = {};  // nodes each arm's roadmap started with from the cache
     int roadmap_size[geom::kMaxArms] = {};
     int status = 0;                // MRPOP_OK, MRPOP_ERR_TIMEOUT, MRPOP_ERR_BUDGET, MRPOP_ERR_INTERNAL
     int events[kEvents] = {};      // device event counters (kEv*) at the end of the solve
@@ -83,6 +86,11 @@ struct Slot {
     SlotResult res;
     void *user = nullptr;
     int lane = -1;  // lane running this slot's work, -1 none
+    // roadmap reuse: each arm's cache key, and whether the query's nodes
+    // (known after the insertion round) allow sharing the roadmaps
+    std::vector<float> rm_key[geom::kMaxArms];
+    int rm_cached[geom::kMaxArms] = {};  // nodes taken from the cache per arm
+    unsigned rm_kept = 0;                // arms whose roadmap was kept when it first connected the query
 };
 
 // Per-slot report copied back after each step.
@@ -195,6 +203,8 @@ class Engine {
     int launch_staged(Lane &L, const Staged &st);
     int run_aux(Slot *s, void (*launch)(SlotDev *, const int *, int, hipStream_t));
     void release_pool();
+    int cache_fetch(Slot *s);
+    int cache_store(Slot *s, const int *rm_n, unsigned arms);
     Context *c_;
     std::vector<Slot *> slots_;
     int cap_ = 0;

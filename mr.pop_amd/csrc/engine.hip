@@ -11,7 +11,6 @@ namespace mrpop {
 
 namespace {
 
-constexpr int kRoadmapIterPerBlock = 230;  // ROADMAP_BUILD_ITER (roadmap_interior.cuh:20)
 constexpr int kPathCap = 2048;             // waypoints (MAX_PATH_SIZE floats in the reference)
 constexpr int kMaxConnectChunks = 64;      // chunks of one connection attempt
 constexpr float kFltMax = 3.402823466e38f;
@@ -140,7 +139,7 @@ int Engine::ensure_capacity(int n) {
 size_t Engine::slot_bytes(int A, const mrpop_settings &st, const SolveOptions &opt) const {
     const int D = 7 * A;
     const int B = st.num_new_configs;
-    const int M = kRoadmapIterPerBlock * opt.block_per_robot;
+    const int M = kRoadmapIter * opt.block_per_robot;
     const int rm_cap = opt.roadmap_capacity;
     const int tr_cap = opt.tree_capacity > 0 ? opt.tree_capacity : std::min(st.max_samples, 262144);
     const int Kc = opt.neighbour_candidates, K = opt.neighbour_capacity;
@@ -204,9 +203,11 @@ int Engine::validate(const mrpop_scene &scene, int A, const float *bases, const 
     if (opt.block_per_robot < 1 || opt.roadmap_capacity < 1 + n_goals + 1 || opt.neighbour_capacity < 1 ||
         opt.neighbour_candidates < 1 || opt.neighbour_candidates > 64 || opt.rounds_per_step < 1 ||
         opt.rounds_per_step > 1024 || opt.tree_capacity < 0 ||
-        (opt.tree_capacity == 0 && st.max_samples < 2) || opt.probe_samples < 0 || opt.probe_samples > 64)
-        return c_->fail(MRPOP_ERR_INVALID,
-                        "bad solve options (neighbour_candidates 1..64, rounds_per_step 1..1024, probe_samples 0..64)");
+        (opt.tree_capacity == 0 && st.max_samples < 2) || opt.probe_samples < 0 || opt.probe_samples > 64 ||
+        opt.biased_iter < 0 || (opt.roadmap_cache && opt.roadmap_capacity < 4 * (1 + kMaxGoals)))
+        return c_->fail(MRPOP_ERR_INVALID, "bad solve options (neighbour_candidates 1..64, rounds_per_step 1..1024, "
+                                           "probe_samples 0..64, biased_iter >= 0, roadmap_capacity >= %d with "
+                                           "roadmap_cache)", 4 * (1 + kMaxGoals));
     // the goal tree starts with one root per goal and the start tree with one
     // node; a capacity below n_goals + 1 would not hold the roots
     const long tr_cap = opt.tree_capacity > 0 ? opt.tree_capacity : std::min(st.max_samples, 262144);
@@ -228,6 +229,128 @@ int Engine::validate(const mrpop_scene &scene, int A, const float *bases, const 
     };
     if (!finite(start, 7L * A) || !finite(goals, 7L * A * n_goals) || (bases && !finite(bases, 12L * A)))
         return c_->fail(MRPOP_ERR_INVALID, "start, goals and base poses must be finite");
+    return MRPOP_OK;
+}
+
+// Everything an arm's roadmap depends on: the checker resolution and edge
+// range, the candidate and list sizes, the node capacity, the arm's base
+// pose and every obstacle record.
+std::vector<float> roadmap_key(const mrpop_scene &sc, const float *base12, const mrpop_settings &st,
+                               const SolveOptions &opt) {
+    std::vector<float> k = {(float)st.granularity, st.range, (float)opt.neighbour_candidates,
+                            (float)opt.neighbour_capacity, (float)opt.roadmap_capacity};
+    k.insert(k.end(), base12, base12 + 12);
+    const int counts[5] = {sc.n_spheres, sc.n_capsules, sc.n_z_capsules, sc.n_cuboids, sc.n_z_cuboids};
+    const float *recs[5] = {sc.spheres, sc.capsules, sc.z_capsules, sc.cuboids, sc.z_cuboids};
+    const int width[5] = {MRPOP_SPHERE_FLOATS, MRPOP_CAPSULE_FLOATS, MRPOP_CAPSULE_FLOATS, MRPOP_CUBOID_FLOATS,
+                          MRPOP_CUBOID_FLOATS};
+    for (int t = 0; t < 5; ++t) {
+        k.push_back((float)counts[t]);
+        if (counts[t] > 0) k.insert(k.end(), recs[t], recs[t] + (size_t)counts[t] * width[t]);
+    }
+    return k;
+}
+
+// Copy the cached roadmap of each arm that has one into the slot (its node
+// count included); the first round then inserts the query's nodes after them.
+int Engine::cache_fetch(Slot *s) {
+    if (!s->opt.roadmap_cache) return MRPOP_OK;
+    RoadmapCache &C = c_->rm_cache;
+    const int K = s->dv.rm_K;
+    for (int a = 0; a < s->A; ++a) {
+        auto it = C.entries.find(s->rm_key[a]);
+        if (it == C.entries.end()) {
+            C.misses++;
+            continue;
+        }
+        RoadmapEntry &e = it->second;
+        e.used = ++C.clock;
+        C.hits++;
+        const size_t off = (size_t)a * s->dv.rm_cap;
+        const int n = e.n;
+        MRPOP_HIP(c_, hipMemcpyAsync(s->dv.rm_q + off * 8, e.q, sizeof(float) * 8 * n, hipMemcpyDeviceToDevice,
+                                     c_->stream));
+        MRPOP_HIP(c_, hipMemcpyAsync(s->dv.rm_nbr + off * K, e.nbr, sizeof(int) * (size_t)n * K,
+                                     hipMemcpyDeviceToDevice, c_->stream));
+        MRPOP_HIP(c_, hipMemcpyAsync(s->dv.rm_deg + off, e.deg, sizeof(int) * n, hipMemcpyDeviceToDevice, c_->stream));
+        MRPOP_HIP(c_, hipMemcpyAsync(s->dv.rm_uf + off, e.uf, sizeof(int) * n, hipMemcpyDeviceToDevice, c_->stream));
+        s->rm_cached[a] = n;
+        s->res.roadmap_cached[a] = n;
+        // node count, and the id the query's start will get (the tree roots
+        // are set up before the insertion round runs)
+        MRPOP_HIP(c_, hipMemcpyAsync(s->dv.rm_n + a, &s->rm_cached[a], sizeof(int), hipMemcpyHostToDevice,
+                                     c_->stream));
+        MRPOP_HIP(c_, hipMemcpyAsync(s->dv.rm_root0 + a, &s->rm_cached[a], sizeof(int), hipMemcpyHostToDevice,
+                                     c_->stream));
+    }
+    MRPOP_HIP(c_, hipStreamSynchronize(c_->stream));
+    return MRPOP_OK;
+}
+
+// Keep the slot's roadmaps (rm_n nodes per arm) for later queries: only
+// arms whose inserted start and goals were collision-free, whose roadmap
+// fills at most half the capacity (a query that starts from it can still
+// grow it around its own start and goals), and that have more nodes than
+// the entry already kept.  The slot is on no lane.
+int Engine::cache_store(Slot *s, const int *rm_n, unsigned arms) {
+    // roadmaps whose union-find hit its bound are not trusted
+    if (!s->opt.roadmap_cache || s->res.events[kEvUnionFind] > 0) return MRPOP_OK;
+    RoadmapCache &C = c_->rm_cache;
+    int qvalid[geom::kMaxArms] = {};
+    MRPOP_HIP(c_, hipMemcpyAsync(qvalid, s->dv.rm_qvalid, sizeof(int) * s->A, hipMemcpyDeviceToHost, c_->stream));
+    MRPOP_HIP(c_, hipStreamSynchronize(c_->stream));
+    const int K = s->dv.rm_K;
+    std::vector<void *> old;
+    for (int a = 0; a < s->A; ++a) {
+        const int n = rm_n[a];
+        if (!((arms >> a) & 1u) || !qvalid[a] || n <= 0 || n > s->dv.rm_cap / 2) continue;
+        auto it = C.entries.find(s->rm_key[a]);
+        if (it != C.entries.end() && it->second.n >= n) continue;
+        RoadmapEntry e;
+        e.n = n;
+        e.K = K;
+        const size_t bq = align256(sizeof(float) * 8 * n), bn = align256(sizeof(int) * (size_t)n * K),
+                     bi = align256(sizeof(int) * n);
+        e.bytes = bq + bn + 2 * bi;
+        if (e.bytes > C.budget) continue;
+        if (hipMalloc(&e.mem, e.bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            continue;  // no room: the roadmap is not kept
+        }
+        char *m = (char *)e.mem;
+        e.q = (float *)m;
+        e.nbr = (int *)(m + bq);
+        e.deg = (int *)(m + bq + bn);
+        e.uf = (int *)(m + bq + bn + bi);
+        const size_t off = (size_t)a * s->dv.rm_cap;
+        MRPOP_HIP(c_, hipMemcpyAsync(e.q, s->dv.rm_q + off * 8, sizeof(float) * 8 * n, hipMemcpyDeviceToDevice,
+                                     c_->stream));
+        MRPOP_HIP(c_, hipMemcpyAsync(e.nbr, s->dv.rm_nbr + off * K, sizeof(int) * (size_t)n * K,
+                                     hipMemcpyDeviceToDevice, c_->stream));
+        MRPOP_HIP(c_, hipMemcpyAsync(e.deg, s->dv.rm_deg + off, sizeof(int) * n, hipMemcpyDeviceToDevice, c_->stream));
+        MRPOP_HIP(c_, hipMemcpyAsync(e.uf, s->dv.rm_uf + off, sizeof(int) * n, hipMemcpyDeviceToDevice, c_->stream));
+        e.used = ++C.clock;
+        if (it != C.entries.end()) {
+            C.bytes -= it->second.bytes;
+            old.push_back(it->second.mem);
+            it->second = e;
+        } else {
+            C.entries.emplace(s->rm_key[a], e);
+        }
+        C.bytes += e.bytes;
+        C.stored++;
+    }
+    // least recently used entries leave beyond the budget
+    while (C.bytes > C.budget && !C.entries.empty()) {
+        auto lru = C.entries.begin();
+        for (auto it = C.entries.begin(); it != C.entries.end(); ++it)
+            if (it->second.used < lru->second.used) lru = it;
+        C.bytes -= lru->second.bytes;
+        old.push_back(lru->second.mem);
+        C.entries.erase(lru);
+    }
+    MRPOP_HIP(c_, hipStreamSynchronize(c_->stream));
+    for (void *p : old) (void)hipFree(p);
     return MRPOP_OK;
 }
 
@@ -274,7 +397,7 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     s->id = id;
 
     // ---- device memory
-    const int D = s->D, B = st.num_new_configs, M = kRoadmapIterPerBlock * opt.block_per_robot;
+    const int D = s->D, B = st.num_new_configs, M = kRoadmapIter * opt.block_per_robot;
     const int rm_cap = opt.roadmap_capacity, K = opt.neighbour_capacity, Kc = opt.neighbour_candidates;
     const int tr_cap = opt.tree_capacity > 0 ? opt.tree_capacity : std::min(st.max_samples, 262144);
     const size_t ce_cap = (size_t)B * kMaxConnectChunks, sp_cap = (size_t)kPathCap * 40;
@@ -313,6 +436,9 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     v.rm_connected = small + 8;
     v.re_count = small + 16;
     v.rs_nsamp = small + 24;
+    v.rs_brounds = small + 32;
+    v.rm_root0 = small + 40;
+    v.rm_qvalid = small + 48;
     v.rs_q = (float *)take(sizeof(float) * A * M * 7);
     v.rs_coll = (uint32_t *)take(sizeof(uint32_t) * A * M);
     v.rs_id = (int *)take(sizeof(int) * A * M);
@@ -399,6 +525,15 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     v.seed = seed;
     v.G = st.granularity;
     v.rm_reassign = st.roadmap_reassign ? 1 : 0;
+    v.rm_biased_iter = opt.biased_iter;
+    v.rm_stretch = 1;
+    // roadmap reuse: the query's nodes are inserted by the first round, and
+    // sampled rounds leave room for the nodes of the queries that reuse the
+    // roadmap later
+    v.rm_insert = opt.roadmap_cache ? 1 : 0;
+    v.rm_fill = opt.roadmap_cache ? rm_cap - (1 + kMaxGoals) : rm_cap;
+    if (opt.roadmap_cache)
+        for (int a = 0; a < A; ++a) s->rm_key[a] = roadmap_key(scene, b + 12 * a, st, opt);
     v.B = B;
     v.M = M;
     v.range = st.range;
@@ -449,6 +584,7 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     h_worlds_[id] = w;
     MRPOP_HIP(c_, hipMemcpyAsync(d_worlds_ + id, &h_worlds_[id], sizeof(dev::WorldDesc), hipMemcpyHostToDevice, c_->stream));
     if (int rc = run_aux(s, plan::launch_roadmap_init)) return rc;
+    if (int rc = cache_fetch(s)) return rc;
     slots_.push_back(s);
     s->t_kernel_start = std::chrono::steady_clock::now();
     if (int rc = start_pass(s)) return rc;
@@ -812,19 +948,32 @@ int Engine::process_lane(Lane &L) {
         s->lane = -1;
         if (s->phase == Phase::Done) continue;  // cancelled while running
         const SlotReport &r = L.h_report[k];
+        for (int e = 0; e < kEvents; ++e) s->res.events[e] = r.events[e];
         const bool in_rm = std::find(L.rm.begin(), L.rm.end(), s) != L.rm.end();
         const bool in_dr = !in_rm && std::find(L.dr.begin(), L.dr.end(), s) != L.dr.end();
         if (in_rm) {
             s->dv.rm_round++;
             s->res.roadmap_rounds++;
+            s->dv.rm_insert = 0;  // the query's nodes are in
             bool all_connected = true, at_cap = false;
             for (int a = 0; a < s->A; ++a) {
                 all_connected &= r.rm_connected[a] != 0;
-                at_cap |= r.rm_n[a] >= s->dv.rm_cap;
+                at_cap |= r.rm_n[a] >= s->dv.rm_fill;
                 s->res.roadmap_size[a] = r.rm_n[a];
+            }
+            if (s->opt.roadmap_cache) {
+                // an arm's roadmap is kept when it first connects the query's start and goal
+                unsigned now_connected = 0;
+                for (int a = 0; a < s->A; ++a)
+                    if (r.rm_connected[a] && !((s->rm_kept >> a) & 1u)) now_connected |= 1u << a;
+                if (now_connected) {
+                    if (int rc = cache_store(s, r.rm_n, now_connected)) return rc;
+                    s->rm_kept |= now_connected;
+                }
             }
             if (!s->rm_until_connected || all_connected || at_cap) {
                 s->rm_until_connected = false;
+                s->dv.rm_stretch = 0;  // the first stretch of roadmap rounds is over for good
                 s->phase = Phase::Search;
                 s->dr_round = 0;
                 s->dv.round_reset = 1;  // a fresh dRRT launch
@@ -862,7 +1011,6 @@ int Engine::process_lane(Lane &L) {
                 if (int rc = finish_pass(s, r)) return rc;
             }
         }
-        for (int k = 0; k < kEvents; ++k) s->res.events[k] = r.events[k];
         if (s->phase != Phase::Done && r.events[kEvUnionFind] > 0) {
             // roadmap connectivity can no longer be trusted: end the problem
             s->phase = Phase::Done;
@@ -877,6 +1025,8 @@ int Engine::process_lane(Lane &L) {
             s->res.wall_ns = ns_since(s->t_submit);
             s->res.kernel_ns = ns_since(s->t_kernel_start);
         }
+        if (s->phase == Phase::Done)  // the roadmaps as they ended, for later queries
+            if (int rc = cache_store(s, s->res.roadmap_size, ~0u)) return rc;
     }
     L.table.clear();
     L.rm.clear();

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,29 @@ struct FineQueue {
     unsigned *diag;   // per (sample, arm) phase results, 8 words per sample (diagnostics only)
 };
 
+// One arm's roadmap kept for later queries in the same scene: nodes
+// [n][8], neighbour lists [n][K], degrees and union-find parents [n], in one
+// device allocation.
+struct RoadmapEntry {
+    int n = 0, K = 0;
+    void *mem = nullptr;
+    float *q = nullptr;
+    int *nbr = nullptr, *deg = nullptr, *uf = nullptr;
+    size_t bytes = 0;
+    uint64_t used = 0;
+};
+
+// Roadmaps keyed by everything their nodes and edges depend on (obstacle
+// records, the arm's base pose, granularity, range, neighbour candidates
+// and capacities); least recently used entries leave beyond `budget` bytes.
+struct RoadmapCache {
+    std::map<std::vector<float>, RoadmapEntry> entries;
+    size_t bytes = 0;
+    size_t budget = (size_t)8 << 30;
+    uint64_t clock = 0;
+    long hits = 0, misses = 0, stored = 0;
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -37,6 +61,7 @@ struct Context {
     void *scratch = nullptr;  // this device's static staging array (device address), see capi.hip
     void *diag = nullptr;  // when set, edge checks record per-sample phase results (diagnostics)
     int probe = 0;         // > 0: edge-only checks run a probe pass of this many samples first (diagnostics)
+    RoadmapCache rm_cache; // roadmaps shared by the queries of this context (solve option roadmap_cache)
 
     int fail(int code, const char *fmt, ...) {
         char buf[512];

@@ -21,6 +21,10 @@
 namespace mrpop {
 
 constexpr int kMaxGoals = 8;
+// Roadmap samples one block of the reference draws per round
+// (ROADMAP_BUILD_ITER, roadmap_interior.cuh:20): a round has
+// kRoadmapIter * block_per_robot samples per arm.
+constexpr int kRoadmapIter = 230;
 constexpr float kUnwritten = -9999.0f;  // pop_planner_state.cuh:10
 
 // Slot status codes (device int).
@@ -73,6 +77,16 @@ struct SlotDev {
     int *rm_connected;        // [A]
     int *rs_nsamp;            // [A] samples drawn this round (0: the arm sits the round out)
     int rm_reassign;          // roadmap_reassign: arms already connected draw no more samples
+    int rm_biased_iter;       // every rm_biased_iter-th sample of a block is steered to its nearest node (0: never)
+    int rm_stretch;           // 1 while the first roadmap rounds of the solve run (until the arms connect)
+    int *rs_brounds;          // [A] rounds of that stretch in which the arm drew samples
+    // roadmap reuse across queries of one scene (roadmaps taken from the
+    // context's roadmap cache): the query's start and goals are nodes
+    // rm_root0[a] + k of arm a's roadmap (k = 0 start, 1..n_goals goals)
+    int *rm_root0;            // [A] (0 for a roadmap built from scratch)
+    int rm_insert;            // 1: this round's samples are the query's start and goals (forced in as nodes)
+    int rm_fill;              // node ids sampled rounds may take (rm_cap, less a reserve for later queries)
+    int *rm_qvalid;           // [A] 1 if the inserted start and goals of the arm are collision-free
     // roadmap round staging
     float *rs_q;              // [A][M][7]
     uint32_t *rs_coll;        // [A][M]

@@ -64,8 +64,21 @@ __device__ __forceinline__ bool searching(const SlotDev &s) {
 // ============================================================ roadmaps ====
 // Roadmap of each arm: node 0 = the arm's start slice, nodes 1..n_goals its
 // goal slices (roadmap_interior.cuh:69-88).
+//
+// With roadmap reuse (rm_insert) the start and goals are not written here:
+// the first round inserts them after the nodes copied from the cache.
+// (The host then copies any cached roadmap in and sets its node count.)
 __global__ void k_roadmap_init(SlotDev *slots, const int *ids) {
     SlotDev &s = slots[ids[blockIdx.y]];
+    if (threadIdx.x < s.A) {
+        s.rm_root0[threadIdx.x] = 0;
+        s.rm_n[threadIdx.x] = s.rm_insert ? 0 : 1 + s.n_goals;
+        s.rm_connected[threadIdx.x] = 0;
+        s.rm_qvalid[threadIdx.x] = 0;
+        s.re_count[threadIdx.x] = 0;
+        s.rs_brounds[threadIdx.x] = 0;
+    }
+    if (s.rm_insert) return;
     const float *sg = s.sg;
     for (int t = threadIdx.x; t < s.A * (1 + s.n_goals) * 7; t += blockDim.x) {
         const int a = t / ((1 + s.n_goals) * 7);
@@ -78,35 +91,99 @@ __global__ void k_roadmap_init(SlotDev *slots, const int *ids) {
         s.rm_uf[(size_t)a * s.rm_cap + node] = node;
         s.rm_deg[(size_t)a * s.rm_cap + node] = 0;
     }
-    if (threadIdx.x < s.A) {
-        s.rm_n[threadIdx.x] = 1 + s.n_goals;
-        s.rm_connected[threadIdx.x] = 0;
-        s.re_count[threadIdx.x] = 0;
-    }
 }
 
 // M uniform samples per arm in the joint limits (roadmap_interior.cuh:105-112).
 // They are validated as G = 1 edges q -> q by the edge checker.
-__global__ void k_rm_sample(SlotDev *slots, const int *ids) {
+//
+// Biased steering (roadmap_interior.cuh:117-165): while the first roadmap
+// rounds of the solve run, a reference block's iteration counter counts its
+// samples, and every BIASED_ITER-th sample is pulled to within `range` of
+// its nearest roadmap node, q <- n + (q - n) * min(1, range / |q - n|).
+// Sample i of a round is iteration i mod 230 of block i / 230, so its
+// counter is 230 * (earlier rounds of the stretch the arm sampled in) +
+// (i mod 230) + 1.  The nearest node (squared distance accumulated with one
+// fused multiply-add per joint, lowest id among equals) is searched by the
+// whole wave over the nodes the arm had at the round's start.
+__global__ __launch_bounds__(256) void k_rm_sample(SlotDev *slots, const int *ids) {
     SlotDev &s = slots[ids[blockIdx.y]];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= s.A * s.M) return;
-    const int a = t / s.M, i = t - a * s.M;
+    const int lane = threadIdx.x & 63;
+    const bool live = t < s.A * s.M;
+    const int a = live ? t / s.M : 0, i = live ? t - a * s.M : 0;
     // roadmap_reassign (roadmap_interior.cuh:316-327): once an arm's start and
     // goal are connected, its share of the round goes to the arms still
     // unconnected -- here the arm draws no samples and its checks drop out
-    const bool skip = s.rm_reassign && s.rm_connected[a];
-    if (i == 0) s.rs_nsamp[a] = skip ? 0 : s.M;
-    if (skip) {
+    const bool skip = live && s.rm_reassign && s.rm_connected[a];
+    const bool draw = live && !skip;
+    if (s.rm_insert) {
+        // insertion round: sample k <= n_goals is the query's node k (start,
+        // then goals); the checker validates only those
+        if (!live) return;
+        if (i == 0) s.rs_nsamp[a] = 1 + s.n_goals;
+        if (i <= s.n_goals) {
+            float *dst = s.rs_q + ((size_t)a * s.M + i) * 7;
+            for (int j = 0; j < 7; ++j) dst[j] = s.sg[(size_t)i * s.D + 7 * a + j];
+        }
+        s.rs_coll[(size_t)a * s.M + i] = i <= s.n_goals ? 0u : 1u;
+        return;
+    }
+    if (live && i == 0) s.rs_nsamp[a] = skip ? 0 : s.M;
+    float q[7] = {};
+    if (draw) {
+        const uint4 r0 = slot_random(s.seed, kStreamRoadmap, (unsigned)(s.rm_round * kMaxArms + a), (unsigned)i, 0);
+        const uint4 r1 = slot_random(s.seed, kStreamRoadmap, (unsigned)(s.rm_round * kMaxArms + a), (unsigned)i, 1);
+        const unsigned bits[7] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
+#pragma unroll
+        for (int j = 0; j < 7; ++j) q[j] = __builtin_fmaf(u01(bits[j]), kScale[j], kLow[j]);
+    }
+    bool biased = false;
+    if (draw && s.rm_stretch && s.rm_biased_iter > 0) {
+        const long long it = (long long)s.rs_brounds[a] * kRoadmapIter + (i % kRoadmapIter) + 1;
+        biased = it % s.rm_biased_iter == 0;
+    }
+    // every lane of the wave reaches this point (no early exit above)
+    unsigned long long pend = __ballot(biased);
+    while (pend) {
+        const int src = __ffsll((long long)pend) - 1;
+        pend &= pend - 1;
+        const int sa = __shfl(a, src);
+        float sq[7];
+#pragma unroll
+        for (int j = 0; j < 7; ++j) sq[j] = __shfl(q[j], src);
+        const float *rq = s.rm_q + (size_t)sa * s.rm_cap * 8;
+        const int n = s.rm_n[sa];
+        float bd = FLT_MAX;
+        int bi = 0x7fffffff;
+        for (int r = lane; r < n; r += 64) {
+            const float *p = rq + (size_t)r * 8;
+            float d = 0.f;
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const float x = p[j] - sq[j];
+                d = __builtin_fmaf(x, x, d);
+            }
+            if (d < bd) {
+                bd = d;
+                bi = r;
+            }
+        }
+        wave_argmin(bd, bi);
+        if (lane == src) {
+            const float *p = rq + (size_t)bi * 8;
+            const float sc = fminf(1.0f, s.range / sqrtf(bd));  // a node at the sample itself: 1
+#pragma unroll
+            for (int j = 0; j < 7; ++j) q[j] = __builtin_fmaf(q[j] - p[j], sc, p[j]);
+        }
+    }
+    if (!live) return;
+    if (!draw) {
         s.rs_coll[(size_t)a * s.M + i] = 1u;  // no new nodes
         return;
     }
-    const uint4 r0 = slot_random(s.seed, kStreamRoadmap, (unsigned)(s.rm_round * kMaxArms + a), (unsigned)i, 0);
-    const uint4 r1 = slot_random(s.seed, kStreamRoadmap, (unsigned)(s.rm_round * kMaxArms + a), (unsigned)i, 1);
-    const unsigned bits[7] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z};
-    float *q = s.rs_q + ((size_t)a * s.M + i) * 7;
+    float *dst = s.rs_q + ((size_t)a * s.M + i) * 7;
 #pragma unroll
-    for (int j = 0; j < 7; ++j) q[j] = __builtin_fmaf(u01(bits[j]), kScale[j], kLow[j]);
+    for (int j = 0; j < 7; ++j) dst[j] = q[j];
     s.rs_coll[(size_t)a * s.M + i] = 0u;  // the checker only sets bits
 }
 
@@ -122,12 +199,24 @@ __global__ void k_rm_assign(SlotDev *slots, const int *ids) {
         s.re_count[a] = 0;
         s.rs_base[a] = base;     // first id of this round's new nodes
         s.rl_ntouched[a] = 0;
+        if (s.rm_stretch && s.rs_nsamp[a] > 0 && !s.rm_insert) s.rs_brounds[a] += 1;  // the round is over for k_rm_sample
+        if (s.rm_insert) {
+            // the query's nodes go in whatever the checker found (a query
+            // assumes a free start and goals); whether they are free decides
+            // whether the roadmap may be shared later
+            int ok = 1;
+            for (int k = 0; k <= s.n_goals; ++k) ok &= s.rs_coll[(size_t)a * s.M + k] == 0u;
+            s.rm_qvalid[a] = ok;
+            s.rm_root0[a] = base;
+        }
     }
     __syncthreads();
+    const int limit = s.rm_insert ? s.rm_cap : s.rm_fill;
     const int per = (s.M + 255) / 256;
     const int i0 = threadIdx.x * per;
     int cnt = 0;
-    for (int i = i0; i < min(i0 + per, s.M); ++i) cnt += s.rs_coll[(size_t)a * s.M + i] == 0u;
+    for (int i = i0; i < min(i0 + per, s.M); ++i)
+        cnt += s.rs_coll[(size_t)a * s.M + i] == 0u || (s.rm_insert && i <= s.n_goals);
     sh[threadIdx.x] = cnt;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
@@ -139,7 +228,7 @@ __global__ void k_rm_assign(SlotDev *slots, const int *ids) {
     int id = base + sh[threadIdx.x] - cnt;
     for (int i = i0; i < min(i0 + per, s.M); ++i) {
         int out = -1;
-        if (s.rs_coll[(size_t)a * s.M + i] == 0u && id < s.rm_cap) {
+        if ((s.rs_coll[(size_t)a * s.M + i] == 0u || (s.rm_insert && i <= s.n_goals)) && id < limit) {
             out = id++;
             const float *q = s.rs_q + ((size_t)a * s.M + i) * 7;
             float *dst = s.rm_q + ((size_t)a * s.rm_cap + out) * 8;
@@ -150,7 +239,7 @@ __global__ void k_rm_assign(SlotDev *slots, const int *ids) {
         s.rs_id[(size_t)a * s.M + i] = out;
     }
     __syncthreads();
-    if (threadIdx.x == 255) s.rm_n[a] = min(base + sh[255], s.rm_cap);
+    if (threadIdx.x == 255) s.rm_n[a] = max(base, min(base + sh[255], limit));
 }
 
 // Radius neighbours of each new node among all nodes with a smaller id
@@ -158,6 +247,12 @@ __global__ void k_rm_assign(SlotDev *slots, const int *ids) {
 // (roadmap_interior.cuh:183-197).  Candidates are staged at fixed rows
 // sample*Kc + k; k_rm_compact packs them in sample order.  One wave per
 // (slot, arm, sample).
+//
+// The query's start and goal nodes come first, then the other nodes in id
+// order.  In a roadmap built for the query they are nodes 0..n_goals, so
+// this is plain id order; in a roadmap taken from the cache they are its
+// newest nodes, and every new node still tries them first, as it would in a
+// roadmap of its own.
 __global__ __launch_bounds__(64) void k_rm_radius(SlotDev *slots, const int *ids) {
     SlotDev &s = slots[ids[blockIdx.y]];
     const int t = blockIdx.x;
@@ -178,10 +273,11 @@ __global__ __launch_bounds__(64) void k_rm_radius(SlotDev *slots, const int *ids
     const float r2 = s.range * s.range;
     int *stage = s.rs_cand + ((size_t)a * s.M + i) * s.rm_Kc;
     int found = 0;
-    for (int base = 0; base < id && found < s.rm_Kc; base += 64) {
-        const int n = base + lane;
+    const int root0 = s.rm_root0[a], root1 = root0 + 1 + s.n_goals;  // the query's nodes [root0, root1)
+    if (root0 > 0) {
+        const int n = root0 + lane;
         bool near = false;
-        if (n < id) {
+        if (n < root1 && n < id) {
             const float *p = rq + (size_t)n * 8;
             float d = 0.f;
 #pragma unroll
@@ -189,7 +285,100 @@ __global__ __launch_bounds__(64) void k_rm_radius(SlotDev *slots, const int *ids
                 const float x = p[j] - q[j];
                 d = __builtin_fmaf(x, x, d);
             }
-            near = d < r2;  // l2 distance < range
+            near = d < r2 && d > 0.f;
+        }
+        const unsigned long long bal = __ballot(near);
+        if (near) {
+            const int pos = __popcll(bal & ((1ull << lane) - 1ull));
+            if (pos < s.rm_Kc) stage[pos] = n;
+        }
+        found = __popcll(bal);
+    }
+    auto sqd = [&](int n) {
+        const float *p = rq + (size_t)n * 8;
+        float d = 0.f;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            const float x = p[j] - q[j];
+            d = __builtin_fmaf(x, x, d);
+        }
+        return d;
+    };
+    if (s.rm_insert && root0 > 0) {
+        // A query's node inserted into a roadmap from the cache: of the other
+        // nodes within range it takes the nearest (by squared distance, then
+        // id), not the oldest -- it gets no second chance as the roadmap
+        // grows around it the way a start node built with the roadmap does.
+        // The (rem)-th smallest distance is found by a radix select on the
+        // float bits (non-negative floats order as their bit patterns).
+        const int rem = s.rm_Kc - found;
+        int in_range = 0;
+        for (int n = lane; n < id; n += 64)
+            if (n < root0 || n >= root1) {
+                const float d = sqd(n);
+                in_range += d < r2 && d > 0.f;
+            }
+        for (int off = 32; off > 0; off >>= 1) in_range += __shfl_xor(in_range, off);
+        unsigned thr = 0xffffffffu;  // take keys below thr, and the first `eq` keys equal to it
+        int eq = 0;
+        if (rem > 0 && in_range > rem) {
+            __shared__ int hist[256];
+            unsigned prefix = 0, mask = 0;
+            int k = rem;  // rank (1-based) of the wanted key among those matching the prefix
+            for (int shift = 24; shift >= 0; shift -= 8) {
+                for (int b = lane; b < 256; b += 64) hist[b] = 0;
+                __syncthreads();
+                for (int n = lane; n < id; n += 64) {
+                    if (n >= root0 && n < root1) continue;
+                    const float d = sqd(n);
+                    const unsigned key = __float_as_uint(d);
+                    if (d < r2 && d > 0.f && (key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1);
+                }
+                __syncthreads();
+                int digit = 0, below = 0;
+                if (lane == 0) {
+                    while (below + hist[digit] < k) below += hist[digit++];
+                }
+                digit = __shfl(digit, 0);
+                below = __shfl(below, 0);
+                k -= below;
+                prefix |= (unsigned)digit << shift;
+                mask |= 255u << shift;
+                __syncthreads();
+            }
+            thr = prefix;
+            eq = k;
+        }
+        int taken_eq = 0;
+        for (int base = 0; base < id && found < s.rm_Kc; base += 64) {
+            const int n = base + lane;
+            bool lt = false, is_eq = false;
+            if (n < id && (n < root0 || n >= root1)) {
+                const float d = sqd(n);
+                const unsigned key = __float_as_uint(d);
+                lt = d < r2 && d > 0.f && key < thr;
+                is_eq = d < r2 && d > 0.f && key == thr;
+            }
+            const unsigned long long beq = __ballot(is_eq);
+            const int eq_pos = taken_eq + __popcll(beq & ((1ull << lane) - 1ull));
+            const bool take = lt || (is_eq && eq_pos < eq);
+            const unsigned long long bal = __ballot(take);
+            if (take) {
+                const int pos = found + __popcll(bal & ((1ull << lane) - 1ull));
+                if (pos < s.rm_Kc) stage[pos] = n;
+            }
+            found += __popcll(bal);
+            taken_eq += __popcll(beq);
+        }
+        if (lane == 0) cnt[i] = min(found, s.rm_Kc);
+        return;
+    }
+    for (int base = 0; base < id && found < s.rm_Kc; base += 64) {
+        const int n = base + lane;
+        bool near = false;
+        if (n < id && (root0 == 0 || n < root0 || n >= root1)) {
+            const float d = sqd(n);
+            near = d < r2 && d > 0.f;  // l2 distance < range; a node repeating v is no neighbour
         }
         const unsigned long long bal = __ballot(near);
         if (near) {
@@ -402,9 +591,10 @@ __global__ void k_rm_connected(SlotDev *slots, const int *ids) {
     const int a = threadIdx.x;
     if (a >= s.A) return;
     int *uf = s.rm_uf + (size_t)a * s.rm_cap;
-    const int r0 = uf_find(uf, 0, s.events);
+    const int root0 = s.rm_root0[a];
+    const int r0 = uf_find(uf, root0, s.events);
     int c = 0;
-    for (int g = 0; g < s.n_goals; ++g) c |= uf_find(uf, 1 + g, s.events) == r0;
+    for (int g = 0; g < s.n_goals; ++g) c |= uf_find(uf, root0 + 1 + g, s.events) == r0;
     s.rm_connected[a] = c;
 }
 
@@ -432,7 +622,8 @@ __global__ void k_tree_reset(SlotDev *slots, const int *ids) {
         const int tr = t == 0 ? 0 : 1, i = t == 0 ? 0 : t - 1;
         s.tr_parent[(size_t)tr * s.tr_cap + i] = i;
         s.tr_cost[(size_t)tr * s.tr_cap + i] = 0.f;
-        for (int a = 0; a < s.A; ++a) s.tr_rid[((size_t)tr * s.tr_cap + i) * s.A + a] = t;  // roadmap node t
+        for (int a = 0; a < s.A; ++a)  // the query's roadmap node t
+            s.tr_rid[((size_t)tr * s.tr_cap + i) * s.A + a] = s.rm_root0[a] + t;
     }
     for (int b = threadIdx.x; b < s.B; b += blockDim.x) s.ex_tree[b] = b < s.B / 2 ? 0 : 1;
     if (threadIdx.x == 0) {
@@ -625,7 +816,10 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
                 d2 = __builtin_fmaf(x, x, d2);
             }
             const float d = sqrtf(d2);
-            if ((s.optimize && d > bound) || d > s.range) continue;
+            // a neighbour at the arm's own configuration (a node that repeats
+            // it) is no move: every arm must move, as the self-edge excluded
+            // above (utils.cuh:708-712)
+            if ((s.optimize && d > bound) || d > s.range || d2 == 0.f) continue;
             const float ang = edge_angle(&vn[7 * a], &q[7 * a], p);
             if (ang < bang || (ang == bang && r < bidx)) {
                 bang = ang;

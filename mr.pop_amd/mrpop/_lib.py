@@ -37,7 +37,8 @@ class SolveOptions(ctypes.Structure):
     """mrpop_solve_options (include/mrpop_c.h)."""
     _fields_ = [("block_per_robot", I), ("time_budget_s", ctypes.c_double), ("roadmap_capacity", I),
                 ("tree_capacity", I), ("neighbour_capacity", I), ("neighbour_candidates", I), ("rounds_per_step", I),
-                ("probe_samples", I), ("graph_rounds", I), ("arm_split", I)]
+                ("probe_samples", I), ("graph_rounds", I), ("arm_split", I), ("biased_iter", I),
+                ("roadmap_cache", I)]
 
 
 class CProblem(ctypes.Structure):
@@ -53,7 +54,7 @@ class ResultInfo(ctypes.Structure):
                 ("goal_tree_size", I), ("iters", I), ("wall_ns", U64), ("kernel_ns", U64), ("copy_ns", U64),
                 ("ttfs_ns", U64), ("n_seed_paths", I), ("passes", I), ("rounds", I), ("roadmap_rounds", I),
                 ("roadmap_size", I * 5), ("status", I), ("union_find_bound", I), ("edge_table_full", I),
-                ("connect_skipped", I), ("simplify_dropped", I)]
+                ("connect_skipped", I), ("simplify_dropped", I), ("roadmap_cached", I * 5)]
 
 
 class MrpopError(RuntimeError):
@@ -276,8 +277,9 @@ class Solution:
         self.dim = dim
         info = ResultInfo()
         lib().mrpop_solution_info(handle, ctypes.byref(info))
-        self.info = {f: getattr(info, f) for f, _ in ResultInfo._fields_ if f != "roadmap_size"}
+        self.info = {f: getattr(info, f) for f, _ in ResultInfo._fields_ if f not in ("roadmap_size", "roadmap_cached")}
         self.info["roadmap_size"] = list(info.roadmap_size)
+        self.info["roadmap_cached"] = list(info.roadmap_cached)
         self.solved = bool(info.solved)
         self.cost = float(info.cost)
         self.path = self._path(handle, -1)

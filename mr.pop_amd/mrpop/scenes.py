@@ -425,13 +425,13 @@ def reachable(rng: np.random.Generator, scene: Scene, base: np.ndarray, q: np.nd
 
 
 def shelf_cell_configs(rng: np.random.Generator, scene: Scene, bases: np.ndarray, count: int, checker: "Checker",
-                       tries: int = 64, batch: int = 2048, per_cell: int = 64) -> np.ndarray:
+                       tries: int = 64, batch: int = 2048) -> np.ndarray:
     """`count` collision-free multi-arm configurations with every arm's hand
     inside a shelf cell, the arms in distinct cells (configs[2] of the
     benchmark: end-effectors reaching into random shelf cells).  Per arm and
     cell a pool of in-cell arm configurations is drawn by rejection on the
-    hand position; up to `per_cell` of them (a random subset) that are free
-    for the arm alone and reachable (see `reachable`) form the cell's pool;
+    hand position; those that are free for the arm alone and reachable (see
+    `reachable`) form the cell's pool;
     composites of pool members over random cell assignments are then checked
     whole."""
     n_arms = len(bases)
@@ -441,7 +441,6 @@ def shelf_cell_configs(rng: np.random.Generator, scene: Scene, bases: np.ndarray
         row = []
         for pl in _cell_pools(bases[a], cells):
             if len(pl):
-                pl = pl[rng.permutation(len(pl))[:per_cell]]
                 pl = pl[~checker(scene, 1, bases[a:a + 1], pl)]  # free on its own (environment + self)
                 pl = pl[reachable(rng, scene, bases[a], pl, checker)]
             row.append(pl)

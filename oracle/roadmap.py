@@ -12,7 +12,9 @@ A roadmap round (reference: roadmap_interior.cuh:94-339) for each arm:
     (roadmap_interior.cuh:183-197; here the first `Kc` of them in id order)
     and checks each edge neighbour -> new node at `granularity` samples
     (roadmap_interior.cuh:199-258);
-  * links the collision-free edges in both directions.
+  * links the collision-free edges in both directions;
+  * during the solve's first stretch of rounds, steers every BIASED_ITER-th
+    sample of a block towards its nearest node (roadmap_interior.cuh:117-165).
 Every neighbour list is therefore the node's accepted neighbours in
 ascending id order, of which the first `K` are kept; the degree counts all.
 Distances are evaluated in float32 exactly as the device does (one fused
@@ -68,6 +70,36 @@ def roadmap_samples(seed: int, rm_round: int, arm: int, M: int, scale, low) -> n
     return out
 
 
+ROADMAP_ITER = 230   # samples of one reference block per round (ROADMAP_BUILD_ITER, roadmap_interior.cuh:20)
+
+
+def biased_indices(M: int, stretch_round: int, biased_iter: int) -> list:
+    """Samples of a round that the reference's BIASED_ITER branch steers
+    (roadmap_interior.cuh:114-117): sample i is iteration i mod 230 of block
+    i // 230, whose counter has reached 230 * stretch_round + (i mod 230) + 1
+    (stretch_round = earlier rounds of the solve's first roadmap stretch in
+    which the arm sampled)."""
+    if biased_iter <= 0:
+        return []
+    return [i for i in range(M)
+            if (stretch_round * ROADMAP_ITER + i % ROADMAP_ITER + 1) % biased_iter == 0]
+
+
+def steer(sample: np.ndarray, nodes: np.ndarray, rng: float) -> np.ndarray:
+    """The biased sample (roadmap_interior.cuh:118-164): its nearest node n
+    (squared float32 distance, lowest id among equals) and
+    q <- n + (q - n) * min(1, range / |q - n|), each coordinate one fused
+    multiply-add; the square root and the division are correctly rounded
+    float32 operations."""
+    d = sqdist32(nodes, sample)
+    bi = int(np.argmin(d))
+    dist = np.sqrt(np.float32(d[bi]))
+    with np.errstate(divide="ignore"):
+        sc = np.float32(min(np.float32(1.0), np.float32(np.float32(rng) / dist)))
+    nn = nodes[bi].astype(np.float32)
+    return np.array([fma32(np.float32(sample[j] - nn[j]), sc, nn[j]) for j in range(7)], np.float32)
+
+
 def sqdist32(p: np.ndarray, q: np.ndarray) -> np.ndarray:
     """Squared distances of rows p [n, 7] to q [7] accumulated as the device
     does: d = fma(x, x, d) over the coordinates, x = p - q in float32.  The
@@ -83,12 +115,13 @@ def sqdist32(p: np.ndarray, q: np.ndarray) -> np.ndarray:
 
 def radius_candidates(nodes: np.ndarray, v: int, rng: float, kc: int) -> list:
     """The first kc nodes of id < v within rng of node v (squared float32
-    distance below rng^2, as the device compares)."""
+    distance below rng^2, as the device compares); a node at v's own
+    configuration is not a neighbour."""
     if v == 0:
         return []
     d = sqdist32(nodes[:v], nodes[v])
     r2 = np.float32(np.float32(rng) * np.float32(rng))
-    return [int(u) for u in np.nonzero(d < r2)[0][:kc]]
+    return [int(u) for u in np.nonzero((d < r2) & (d > 0))[0][:kc]]
 
 
 def expected_lists(n_nodes: int, accepted_edges, K: int):

@@ -190,8 +190,9 @@ def expand(s: Slot, b: int, q=None, c_rand=None):
             if r == rid:
                 continue
             p = s.rm_q[a, r, :7]
-            dr = sqrt32(sqdist(vn[None, 7 * a:7 * a + 7], p)[0])
-            if dr > np.float32(s.range):
+            d2 = sqdist(vn[None, 7 * a:7 * a + 7], p)[0]
+            dr = sqrt32(d2)
+            if dr > np.float32(s.range) or d2 == 0:  # a node repeating the arm's configuration is no move
                 continue
             c.append((angle64(vn[7 * a:7 * a + 7], q[7 * a:7 * a + 7], p), r, dr))
         if not c:

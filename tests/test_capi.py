@@ -40,6 +40,17 @@ def test_settings_defaults_match_pop_settings():
         assert getattr(s, k) == pytest.approx(v, rel=1e-6), k
 
 
+def test_solve_options_defaults():
+    """Solve options of this library: the roadmap steering period is the
+    reference's BIASED_ITER (roadmap_interior.cuh:24), roadmap reuse across
+    queries is off, block_per_robot is the harness's 8 and the roadmap
+    capacity MAX_ROADMAP_SIZE (roadmap_interior.cuh:21)."""
+    o = _lib.solve_options_default()
+    assert o.biased_iter == 10000 and o.roadmap_cache == 0
+    assert o.block_per_robot == 8 and o.roadmap_capacity == 100000
+    assert o.graph_rounds == 0 and o.probe_samples == 0
+
+
 def test_default_bases():
     tables = json.loads((GOLDEN / "panda_tables.json").read_text())
     for A in (2, 3, 4, 5):

@@ -327,7 +327,9 @@ def test_invalid_problems_are_rejected(ctx):
            (p, ok[0], _options(tree_capacity=1)), (p, _settings(max_samples=1), ok[1]),
            (dataclasses.replace(p, goals=np.stack([p.goals[0]] * 3)), ok[0], _options(tree_capacity=3)),
            (dataclasses.replace(p, goals=np.stack([p.goals[0]] * 3)), _settings(max_samples=2), ok[1]),
-           (dataclasses.replace(p, start=nan_start), *ok), (dataclasses.replace(p, goals=inf_goal), *ok)]
+           (dataclasses.replace(p, start=nan_start), *ok), (dataclasses.replace(p, goals=inf_goal), *ok),
+           # steering period, and a roadmap too small to keep room for another query's nodes
+           (p, ok[0], _options(biased_iter=-1)), (p, ok[0], _options(roadmap_cache=1, roadmap_capacity=35))]
     cases = [(p, *ok)] + bad + [(PROBLEMS[3], *ok)]
     probs = [c[0] for c in cases]
     sols = _lib.solve_batch(ctx, probs, [c[1] for c in cases], [c[2] for c in cases])

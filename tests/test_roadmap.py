@@ -48,10 +48,12 @@ def test_u01_range_and_resolution():
 
 def test_radius_candidates_take_the_first_in_id_order():
     nodes = np.zeros((6, 7), np.float32)
-    nodes[:, 0] = [0.0, 5.0, 0.5, 0.9, 0.2, 0.0]
+    nodes[:, 0] = [0.1, 5.0, 0.5, 0.9, 0.2, 0.0]
     assert RM.radius_candidates(nodes, 5, 1.0, 8) == [0, 2, 3, 4]
     assert RM.radius_candidates(nodes, 5, 1.0, 2) == [0, 2]
     assert RM.radius_candidates(nodes, 5, 0.9, 8) == [0, 2, 4]  # strictly closer than the range
+    nodes[0, 0] = 0.0  # a node repeating node 5's configuration is no neighbour
+    assert RM.radius_candidates(nodes, 5, 1.0, 8) == [2, 3, 4]
 
 
 def test_start_component_matches_connected_components():
@@ -83,9 +85,10 @@ def test_start_component_matches_connected_components():
     assert RM.start_component(4, [(1, 2), (2, 3)]).tolist() == [True, False, False, False]
 
 
-def _dump(ctx, p, st, op, rounds, arm, cap, K):
+def _dump(ctx, p, st, op, rounds, arm, cap, K, with_stretch=False):
     lib = _lib.lib()
-    lib.mrpop_diag_roadmap.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 6 + [ctypes.c_int]
+    lib.mrpop_diag_roadmap.argtypes = ([ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 6 +
+                                       [ctypes.c_int, ctypes.c_void_p])
     from mrpop._lib import _ProblemRef
     ref = _ProblemRef(p)
     q = np.zeros((cap, 8), np.float32)
@@ -94,11 +97,13 @@ def _dump(ctx, p, st, op, rounds, arm, cap, K):
     uf = np.zeros(cap, np.int32)
     conn = ctypes.c_int()
     n = ctypes.c_int()
+    stretch = ctypes.c_int()
     ctx.check(lib.mrpop_diag_roadmap(ctx.handle, ctypes.byref(ref.c), ctypes.byref(st), ctypes.byref(op), rounds, arm,
                                      q.ctypes.data, deg.ctypes.data, nbr.ctypes.data, uf.ctypes.data,
-                                     ctypes.byref(conn), ctypes.byref(n), cap))
+                                     ctypes.byref(conn), ctypes.byref(n), cap, ctypes.byref(stretch)))
     m = n.value
-    return q[:m, :7].copy(), deg[:m].copy(), nbr[:m].copy(), uf[:m].copy(), bool(conn.value)
+    out = q[:m, :7].copy(), deg[:m].copy(), nbr[:m].copy(), uf[:m].copy(), bool(conn.value)
+    return out + (stretch.value,) if with_stretch else out
 
 
 def _uf_roots(parent):
@@ -134,19 +139,63 @@ def _check_components(uf, connected, n_nodes, accepted, n_goals):
     return int(label0.sum()), len(smallest)
 
 
+def test_biased_sample_indices_follow_the_block_counter():
+    """BIASED_ITER bookkeeping (roadmap_interior.cuh:24, 114-117): a block's
+    counter is incremented before the test, so with 230 samples per round
+    and BIASED_ITER 10000 the first steered sample is iteration 110 of the
+    44th round (43 * 230 + 110 = 10000), the next iteration 220 of round 87
+    (86 * 230 + 220 = 20000); every block of the round steers the same
+    iteration."""
+    M = 230 * 8
+    for r in range(43):
+        assert RM.biased_indices(M, r, 10000) == []
+    assert RM.biased_indices(M, 43, 10000) == [109 + 230 * b for b in range(8)]
+    assert RM.biased_indices(M, 86, 10000) == [219 + 230 * b for b in range(8)]
+    assert RM.biased_indices(M, 86, 0) == []
+    assert RM.biased_indices(230, 0, 1) == list(range(230))
+
+
+def test_steered_sample_geometry():
+    """The steered sample against float64 geometry: it lies on the segment
+    from its nearest node (brute force over the nodes) towards the sample,
+    at distance min(|q - n|, range) from the node, within float32 rounding;
+    a sample within range of its nearest node is unchanged."""
+    rng = np.random.default_rng(11)
+    for trial in range(200):
+        nodes = rng.uniform(-3, 3, (int(rng.integers(1, 40)), 7)).astype(np.float32)
+        q = rng.uniform(-3, 3, 7).astype(np.float32)
+        r = float(rng.uniform(0.2, 3.0))
+        d = np.linalg.norm(nodes.astype(np.float64) - q, axis=1)
+        nn = nodes[int(np.argmin(d))].astype(np.float64)
+        out = RM.steer(q, nodes, r).astype(np.float64)
+        v = q - nn
+        along = out - nn
+        want = min(np.linalg.norm(v), np.float32(r))
+        assert abs(np.linalg.norm(along) - want) < 1e-5 * (1 + want), trial
+        if np.linalg.norm(v) > 0:
+            cos = along @ v / (np.linalg.norm(along) * np.linalg.norm(v))
+            assert cos > 1 - 1e-6, trial
+        if d.min() <= np.float32(r) * (1 - 1e-6):
+            assert np.allclose(out, q, atol=1e-6), trial
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind,seed,arm,n_goals", [("plain2", 4, 0, 1), ("bin4", 1, 2, 1), ("table3", 3, 0, 1),
-                                                   ("bin4", 9, 3, 1), ("shelf5", 2, 4, 1), ("shelf5", 6, 1, 1),
-                                                   ("table3", 4, 1, 3), ("shelf5", 8, 2, 4)])
-def test_roadmap_rounds_match_the_restatement(ctx, kind, seed, arm, n_goals):
+@pytest.mark.parametrize("kind,seed,arm,n_goals,biased", [
+    ("plain2", 4, 0, 1, 10000), ("bin4", 1, 2, 1, 100), ("table3", 3, 0, 1, 10000), ("bin4", 9, 3, 1, 10000),
+    ("shelf5", 2, 4, 1, 57), ("shelf5", 6, 1, 1, 10000), ("table3", 4, 1, 3, 31), ("shelf5", 8, 2, 4, 10000)])
+def test_roadmap_rounds_match_the_restatement(ctx, kind, seed, arm, n_goals, biased):
+    """biased: the steering period (BIASED_ITER); the reference's 10000 is
+    not reached in three rounds, so cases with short periods steer samples
+    in every round of the first stretch."""
     p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx), n_goals=n_goals)
     G, rng, kc, K, bpr, rounds = 8, 2.0, 4, 6, 1, 3
     M = 230 * bpr
     st = _lib.settings(granularity=G, range=rng)
     op = _lib.solve_options_default(block_per_robot=bpr, neighbour_candidates=kc, neighbour_capacity=K,
-                                    roadmap_capacity=4096)
-    print(f"[roadmap] {kind} seed {seed} arm {arm} goals {n_goals}", flush=True)
-    q, deg, nbr, uf, connected = _dump(ctx, p, st, op, rounds, arm, 4096, K)
+                                    roadmap_capacity=4096, biased_iter=biased)
+    print(f"[roadmap] {kind} seed {seed} arm {arm} goals {n_goals} biased_iter {biased}", flush=True)
+    q, deg, nbr, uf, connected, stretch = _dump(ctx, p, st, op, rounds, arm, 4096, K, with_stretch=True)
+    assert 1 <= stretch <= rounds  # round 0 always belongs to the first stretch
     scale, low = geometry.table("kScale"), geometry.table("kLow")
     # nodes 0 .. n_goals: start and goal slices of the arm
     n_fixed = 1 + len(p.goals)
@@ -155,8 +204,15 @@ def test_roadmap_rounds_match_the_restatement(ctx, kind, seed, arm, n_goals):
     nodes = list(fixed)
     accepted = []
     exempt = 0
+    moved = 0
     for r in range(rounds):
         samples = RM.roadmap_samples(p.seed, r, arm, M, scale, low)
+        if r < stretch:  # rounds of the first stretch: biased samples move towards the roadmap
+            arr0 = np.array(nodes, np.float32)
+            for i in RM.biased_indices(M, r, biased):
+                steered = RM.steer(samples[i], arr0, rng)
+                moved += int(not np.array_equal(steered, samples[i]))
+                samples[i] = steered
         v = O.check_edges(p.scene, O.MODE_SINGLE, p.bases, 1, samples, samples, arm=arm)
         first_new = len(nodes)
         for i in range(M):
@@ -188,8 +244,11 @@ def test_roadmap_rounds_match_the_restatement(ctx, kind, seed, arm, n_goals):
         assert list(nbr[i][:len(l)]) == l, f"node {i}: neighbour list differs"
     assert len(accepted) > 100 and max(degs) > K  # the test exercises truncation
     n_start, n_comp = _check_components(uf, connected, len(nodes), accepted, len(p.goals))
+    if biased < 10000:
+        assert moved > 0, "no sample was steered"
     print(f"[roadmap] {len(nodes)} nodes, {len(accepted)} edges, {exempt} near-contact samples, "
-          f"{n_comp} components, {n_start} nodes with the start, connected {connected}", flush=True)
+          f"{n_comp} components, {n_start} nodes with the start, connected {connected}, "
+          f"{stretch} stretch rounds, {moved} samples steered", flush=True)
 
 
 @pytest.mark.gpu
