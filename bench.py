@@ -142,6 +142,7 @@ def _stats(sols, ttfs, wall, budget_s=None):
            "ttfs_p50_s": float(np.percentile(t, 50)) if len(t) else None,
            "ttfs_p99_s": float(np.percentile(t, 99)) if len(t) else None,
            "median_cost": float(np.median(costs)),
+           "cost_p90": float(np.percentile(costs, 90)),
            "timeouts": sum(s.info["status"] == 4 for s in sols)}
     if budget_s is not None:
         full = np.concatenate([t, np.full(len(sols) - len(t), float(budget_s))])
@@ -167,14 +168,17 @@ def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_r
     if batch_probe:
         tuned["probe_samples"] = batch_probe
     if tune:
-        calib = plan_problems(checker, 16, first_seed=1000)
+        calib = plan_problems(checker, max(16, n_batch), first_seed=1000)  # as many as the measured batch
         print(f"[bench] autotune on {len(calib)} calibration problems", file=sys.stderr, flush=True)
-        res = autotune.tune(ctx, calib, common, op_common)
+        res = autotune.tune(ctx, calib, common, dict(op_common, time_budget_s=5.0),
+                            log=lambda m: print(f"[bench] autotune trial {m}", file=sys.stderr, flush=True))
         tuned = res.best
         out["autotune"] = res.as_dict()
         print(f"[bench] autotune: {tuned} in {res.seconds:.1f} s", file=sys.stderr, flush=True)
-    st_batch = _lib.settings(num_new_configs=tuned["num_new_configs"], **common)
-    op_batch = _lib.solve_options_default(**op_common, **{k: v for k, v in tuned.items() if k != "num_new_configs"})
+    tuned_st = {k: v for k, v in tuned.items() if k in autotune.SETTING_KEYS}
+    op_kw = {k: v for k, v in tuned.items() if k not in autotune.SETTING_KEYS}
+    st_batch = _lib.settings(**dict(common, **tuned_st))
+    op_batch = _lib.solve_options_default(**op_common, **op_kw)
     out["settings"]["batched"] = tuned
     # plain: one problem at a time, all submitted at t = 0 (TTFS includes the queue)
     plain_sols, plain_ttfs, own_ttfs = [], [], []
@@ -209,18 +213,58 @@ def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_r
         # the batched configuration with a fixed 2 s budget per problem and
         # the long optimisation passes of the 4-/5-arm configurations
         bprobs = plan_problems(checker, n_budget)
-        st_b = _lib.settings(num_new_configs=tuned["num_new_configs"], **dict(common, optimize_iters=400000))
-        op_b = _lib.solve_options_default(time_budget_s=2.0,
-                                          **{k: v for k, v in tuned.items() if k != "num_new_configs"})
+        st_b = _lib.settings(**dict(common, optimize_iters=400000, **tuned_st))
+        op_b = _lib.solve_options_default(time_budget_s=2.0, **op_kw)
         t0 = time.perf_counter()
         sols = _lib.solve_batch(ctx, bprobs, [st_b] * n_budget, [op_b] * n_budget)
         out["batched_2s_budget"] = _stats(sols, [s.ttfs_s for s in sols if s.solved], time.perf_counter() - t0, 2.0)
         print(f"[bench] batched, 2 s budget: {out['batched_2s_budget']}", file=sys.stderr, flush=True)
-    op_kw = {k: v for k, v in tuned.items() if k != "num_new_configs"}
     out["serving"] = [serving_bench(ctx, checker, rate, serve_s, st_batch, op_kw) for rate in serve_rates]
     if n_scene:
         out["scenes"] = scene_bench(ctx, checker, n_scene, max(1, n_scene_plain), op_kw, kinds=scene_kinds)
     out["two_arm_plain"] = two_arm_plain_bench(ctx, checker)
+    out["same_scene"] = same_scene_bench()
+    return out
+
+
+def same_scene_bench(budget_s=30.0):
+    """The reference's own evaluation: the 50 queries of each fixture scene
+    (scripts/panda_four_problems.json, scripts/panda_five_problems.json --
+    one scene per file) solved one after another in the plain form at its
+    harness settings (nnc 300, G 64, range 2.0, rrtc_iter 3,
+    scripts/evaluate_mr.cu:162-166; optimisation passes of 1500 rounds
+    instead of 400 000 so that the 100 queries finish in seconds), without
+    roadmap reuse (every query builds its roadmaps), and with roadmap_cache
+    in one context: every query starts from the roadmaps the earlier queries
+    of the scene left."""
+    from mrpop import _lib, geometry, scenes
+    st = _lib.settings(num_new_configs=300, granularity=64, range=2.0, rrtc_iter=3, optimize_iters=1500)
+    out = []
+    for name, A in (("panda_four", 4), ("panda_five", 5)):
+        probs = scenes.load_problem_file(REPO / "tests" / "golden" / f"{name}_problems.json",
+                                         geometry.default_bases(A))
+        res = {"scene": f"{name} fixture scene", "arms": A, "queries": len(probs),
+               "settings": {"num_new_configs": 300, "granularity": 64, "range": 2.0, "rrtc_iter": 3,
+                            "optimize_iters": 1500, "time_budget_s": budget_s}}
+        for cache in (0, 1):
+            ctx = _lib.Context(0)
+            op = _lib.solve_options_default(time_budget_s=budget_s, roadmap_cache=cache)
+            sols, own = [], []
+            t0 = time.perf_counter()
+            for p in probs:
+                s = _lib.solve(ctx, p, st, op)
+                sols.append(s)
+                if s.solved:
+                    own.append(s.ttfs_s)
+            wall = time.perf_counter() - t0
+            ctx.close()
+            r = _stats(sols, own, wall, budget_s)
+            r["roadmap_rounds_mean"] = float(np.mean([s.info["roadmap_rounds"] for s in sols]))
+            r["roadmap_nodes_built_mean"] = float(np.mean([sum(s.info["roadmap_size"][:A]) -
+                                                           sum(s.info["roadmap_cached"][:A]) for s in sols]))
+            res["roadmap_reuse" if cache else "fresh_roadmaps"] = r
+        print(f"[bench] same scene: {res}", file=sys.stderr, flush=True)
+        out.append(res)
     return out
 
 

@@ -447,6 +447,16 @@ extern "C" int mrpop_diag_phs_transform(int D, const float *f1, const float *f2,
     return MRPOP_OK;
 }
 
+// Diagnostics (not part of the public ABI): the lane deadline of a context
+// (seconds a planner step may take before its problems end with
+// MRPOP_ERR_TIMEOUT).  Tests set it tiny to take the give-up path with
+// kernels that still complete.
+extern "C" int mrpop_diag_set_lane_deadline(mrpop_ctx *c, double seconds) {
+    if (!c || !(seconds >= 0)) return MRPOP_ERR_INVALID;
+    ((Context *)c)->lane_deadline_s = seconds;
+    return MRPOP_OK;
+}
+
 // Diagnostics (not part of the public ABI): the planner's counter-based
 // generator, Philox4x32-10, evaluated on the host.
 extern "C" void mrpop_diag_philox(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]) {

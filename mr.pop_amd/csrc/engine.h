@@ -187,7 +187,10 @@ class Engine {
         };
         std::vector<Graph> graphs;
         uint64_t graph_clock = 0;
+        bool abandoned = false;  // its work outlived the deadline; its problems were ended
     };
+    int wait_lane(Lane &L);
+    void abandon_lane(Lane &L);
     void drop_graphs(Lane &L);
     int ensure_capacity(int n);
     int ensure_lanes();
@@ -207,6 +210,7 @@ class Engine {
     int cache_store(Slot *s, const int *rm_n, unsigned arms);
     Context *c_;
     std::vector<Slot *> slots_;
+    std::vector<std::pair<int, void *>> graveyard_;  // (lane, block) of slots released while their lane was stuck
     int cap_ = 0;
     Lane lanes_[2];  // 0: roadmap rounds, 1: dRRT rounds and simplification
     dev::WorldDesc *d_worlds_ = nullptr, *h_worlds_ = nullptr;

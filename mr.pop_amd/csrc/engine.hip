@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "engine.h"
 
@@ -64,6 +65,7 @@ Engine::~Engine() {
         if (L.stream) (void)hipStreamDestroy(L.stream);
     }
     for (auto &kv : pool_) (void)hipFree(kv.second);
+    for (auto &g : graveyard_) (void)hipFree(g.second);  // the lanes' streams have drained above
     for (void *p : {(void *)d_worlds_, (void *)d_aux_slot_, (void *)d_aux_id_})
         if (p) (void)hipFree(p);
     for (void *p : {(void *)h_worlds_, (void *)h_aux_slot_})
@@ -602,6 +604,18 @@ void Engine::release_pool() {
 void Engine::release(Slot *s) {
     auto it = std::find(slots_.begin(), slots_.end(), s);
     if (it == slots_.end()) return;
+    if (s->lane >= 0 && lanes_[s->lane].abandoned) {
+        // its lane's work may still use the memory and may never finish:
+        // keep the block aside until the lane rejoins, instead of waiting
+        Lane &L = lanes_[s->lane];
+        for (Slot *&t : L.table)
+            if (t == s) t = nullptr;
+        graveyard_.emplace_back(s->lane, s->mem);
+        bytes_in_use_ -= s->mem_bytes;
+        slots_.erase(it);
+        delete s;
+        return;
+    }
     if (s->lane >= 0) {
         // its lane may still run kernels on the slot's memory; its report is dropped
         Lane &L = lanes_[s->lane];
@@ -1035,10 +1049,46 @@ int Engine::process_lane(Lane &L) {
     return MRPOP_OK;
 }
 
+// Wait for a lane's step, at most the context's lane deadline: its kernels
+// are bounded, so a step that runs far beyond it is stuck.  Polls the event
+// (a blocking wait could not be given up).
+int Engine::wait_lane(Lane &L) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spin = 0;; ++spin) {
+        const hipError_t e = hipEventQuery(L.done);
+        if (e == hipSuccess) return MRPOP_OK;
+        if (e != hipErrorNotReady) return c_->fail(MRPOP_ERR_HIP, "lane step: %s", hipGetErrorString(e));
+        if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > c_->lane_deadline_s)
+            return MRPOP_ERR_TIMEOUT;
+        if (spin > 64) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+// A step outlived the deadline: its problems end with MRPOP_ERR_TIMEOUT.
+// Their memory stays with the lane until its work completes (release()
+// waits for the lane's stream), and the lane takes no new work until then.
+void Engine::abandon_lane(Lane &L) {
+    L.abandoned = true;
+    for (Slot *s : L.table) {
+        if (!s || s->phase == Phase::Done) continue;
+        s->phase = Phase::Done;
+        s->res.status = MRPOP_ERR_TIMEOUT;
+        s->res.wall_ns = ns_since(s->t_submit);
+        s->res.kernel_ns = ns_since(s->t_kernel_start);
+    }
+    c_->fail(MRPOP_ERR_TIMEOUT, "a planner step did not complete within %.3g s", c_->lane_deadline_s);
+}
+
 int Engine::drain() {
     for (Lane &L : lanes_)
         if (L.busy) {
-            MRPOP_HIP(c_, hipEventSynchronize(L.done));
+            const int w = wait_lane(L);
+            if (w == MRPOP_ERR_TIMEOUT) {
+                abandon_lane(L);
+                return w;
+            }
+            if (w) return w;
+            L.abandoned = false;
             if (int rc = process_lane(L)) return rc;
         }
     return MRPOP_OK;
@@ -1057,6 +1107,21 @@ int Engine::step() {
         }
     }
     Lane &R = lanes_[0], &S = lanes_[1];
+    // a lane given up on rejoins once its work has completed (its problems
+    // already ended)
+    for (Lane *L : {&R, &S})
+        if (L->abandoned && hipEventQuery(L->done) == hipSuccess) {
+            L->abandoned = false;
+            const int li = (int)(L - lanes_);
+            for (auto g = graveyard_.begin(); g != graveyard_.end();)
+                if (g->first == li) {
+                    (void)hipFree(g->second);
+                    g = graveyard_.erase(g);
+                } else {
+                    ++g;
+                }
+            if (int rc = process_lane(*L)) return rc;
+        }
     if (!R.busy) {
         // slots still looking for their first solution go first; the roadmap
         // rounds of optimising passes wait for a step without them
@@ -1083,12 +1148,20 @@ int Engine::step() {
         if (int rc = launch_lane(S)) return rc;
     }
     // wait for the search lane (short steps) when it runs, else for the roadmap lane
-    Lane *first = S.busy ? &S : (R.busy ? &R : nullptr);
-    if (!first) return MRPOP_OK;
-    MRPOP_HIP(c_, hipEventSynchronize(first->done));
+    Lane *first = S.busy && !S.abandoned ? &S : (R.busy && !R.abandoned ? &R : nullptr);
+    if (!first) {
+        if (R.abandoned || S.abandoned) std::this_thread::sleep_for(std::chrono::microseconds(100));
+        return MRPOP_OK;
+    }
+    const int w = wait_lane(*first);
+    if (w == MRPOP_ERR_TIMEOUT) {
+        abandon_lane(*first);  // its problems end; the others go on
+        return MRPOP_OK;
+    }
+    if (w) return w;
     if (int rc = process_lane(*first)) return rc;
     Lane &other = first == &S ? R : S;
-    if (other.busy && hipEventQuery(other.done) == hipSuccess)
+    if (other.busy && !other.abandoned && hipEventQuery(other.done) == hipSuccess)
         if (int rc = process_lane(other)) return rc;
     return MRPOP_OK;
 }

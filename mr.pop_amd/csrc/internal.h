@@ -62,6 +62,9 @@ struct Context {
     void *diag = nullptr;  // when set, edge checks record per-sample phase results (diagnostics)
     int probe = 0;         // > 0: edge-only checks run a probe pass of this many samples first (diagnostics)
     RoadmapCache rm_cache; // roadmaps shared by the queries of this context (solve option roadmap_cache)
+    // a planner step whose kernels have not completed after this many seconds
+    // is taken to be stuck: its problems end with MRPOP_ERR_TIMEOUT
+    double lane_deadline_s = 120.0;
 
     int fail(int code, const char *fmt, ...) {
         char buf[512];

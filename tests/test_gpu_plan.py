@@ -411,3 +411,26 @@ def test_reference_fixtures_all_solved(ctx):
     for p, s in zip(probs, sols):
         assert s.info["union_find_bound"] == 0 and s.info["edge_table_full"] == 0
         check_solution(p, s, granularity=HARNESS["granularity"], rng=HARNESS["range"])
+
+
+def test_stuck_step_ends_its_problems_with_timeout():
+    """A planner step whose kernels outlive the context's lane deadline ends
+    its problems with ERR_TIMEOUT instead of blocking the caller.  The
+    condition is forced from the host: a deadline of 0 s while the kernels
+    run and complete normally.  The context plans normally afterwards."""
+    import ctypes
+    lib = _lib.lib()
+    lib.mrpop_diag_set_lane_deadline.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    c = _lib.Context(0)
+    try:
+        assert lib.mrpop_diag_set_lane_deadline(c.handle, 0.0) == _lib.OK
+        t0 = time.perf_counter()
+        sol = _lib.solve(c, PROBLEMS[0], _settings(), _options())
+        assert sol.info["status"] == _lib.ERR_TIMEOUT and not sol.solved
+        assert time.perf_counter() - t0 < 10.0
+        assert lib.mrpop_diag_set_lane_deadline(c.handle, 120.0) == _lib.OK
+        sol = _lib.solve(c, PROBLEMS[0], _settings(), _options())
+        assert sol.info["status"] == _lib.OK
+        check_solution(PROBLEMS[0], sol)
+    finally:
+        c.close()
