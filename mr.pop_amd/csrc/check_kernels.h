@@ -439,6 +439,95 @@ constexpr float max_block_radius() {
 constexpr float kMaxBlockRadius = max_block_radius();
 constexpr float kBallMargin = 1e-4f;
 
+// Sub-blocks of the fine arm-arm stage: each block's fine spheres in groups
+// of at most six that lie close together, with a ball holding each group
+// (centre = mean of the offsets, radius = largest |offset - centre| + r,
+// plus kBallMargin for float rounding).  Two groups whose balls miss each
+// other hold no pair of overlapping spheres, so their pairs are skipped.
+// The hand's 18-sphere block is a 6 x 3 grid: its groups are pairs of
+// columns (radius 5.3 cm against 10.5 cm for the block); the 12-sphere block
+// of joint 5 splits into its two upper spheres, two lower ones and the two
+// four-sphere rails.
+constexpr int kSubBlocks = 16;
+inline constexpr unsigned char kSubIdx[kFineSpheres] = {
+    0,                                  // block 0
+    1, 2, 3, 4,                         // 1
+    5, 6, 7, 8,                         // 2
+    9, 10, 11, 12,                      // 3
+    13, 14, 15, 16,                     // 4
+    17, 18, 19, 20, 21, 22, 25, 26, 23, 24, 27, 28,        // 5: four groups
+    29, 30, 31,                         // 6
+    32, 33, 34, 35, 36,                 // 7
+    37, 38, 43, 44, 49, 50, 39, 40, 45, 46, 51, 52, 41, 42, 47, 48, 53, 54,  // 8: three groups
+    55, 56,                             // 9
+    57, 58};                            // 10
+inline constexpr unsigned char kSubFirst[kSubBlocks] = {0, 1, 5, 9, 13, 17, 19, 21, 25, 29, 32, 37, 43, 49, 55, 57};
+inline constexpr unsigned char kSubCount[kSubBlocks] = {1, 4, 4, 4, 4, 2, 2, 4, 4, 3, 5, 6, 6, 6, 2, 2};
+inline constexpr unsigned char kBlockSub0[kCoarseSpheres] = {0, 1, 2, 3, 4, 5, 9, 10, 11, 14, 15};  // first group
+inline constexpr unsigned char kBlockSubs[kCoarseSpheres] = {1, 1, 1, 1, 1, 4, 1, 1, 3, 1, 1};      // groups
+
+constexpr double csqrt(double x) {
+    double r = x > 1.0 ? x : 1.0;
+    for (int i = 0; i < 64; ++i) r = 0.5 * (r + x / r);
+    return r;
+}
+struct SubBall {
+    float c[kSubBlocks][3];
+    float r[kSubBlocks];
+    unsigned char idx[kFineSpheres], first[kSubBlocks], count[kSubBlocks];
+    unsigned char sub0[kCoarseSpheres], subs[kCoarseSpheres];
+};
+constexpr SubBall build_sub_balls() {
+    SubBall b{};
+    for (int i = 0; i < kFineSpheres; ++i) b.idx[i] = kSubIdx[i];
+    for (int g = 0; g < kSubBlocks; ++g) {
+        b.first[g] = kSubFirst[g];
+        b.count[g] = kSubCount[g];
+    }
+    for (int k = 0; k < kCoarseSpheres; ++k) {
+        b.sub0[k] = kBlockSub0[k];
+        b.subs[k] = kBlockSubs[k];
+    }
+    for (int g = 0; g < kSubBlocks; ++g) {
+        double c[3] = {0, 0, 0};
+        for (int m = 0; m < kSubCount[g]; ++m)
+            for (int d = 0; d < 3; ++d) c[d] += kFineSphere[kSubIdx[kSubFirst[g] + m]][d];
+        for (int d = 0; d < 3; ++d) c[d] /= kSubCount[g];
+        double r = 0;
+        for (int m = 0; m < kSubCount[g]; ++m) {
+            const float *f = kFineSphere[kSubIdx[kSubFirst[g] + m]];
+            const double dx = f[0] - c[0], dy = f[1] - c[1], dz = f[2] - c[2];
+            const double e = csqrt(dx * dx + dy * dy + dz * dz) + f[3];
+            r = e > r ? e : r;
+        }
+        for (int d = 0; d < 3; ++d) b.c[g][d] = (float)c[d];
+        b.r[g] = (float)(r + kBallMargin);
+    }
+    return b;
+}
+static __constant__ SubBall kSub = build_sub_balls();
+constexpr bool sub_blocks_partition_blocks() {
+    // every block's groups are consecutive and list exactly its fine spheres
+    int seen[kFineSpheres] = {};
+    for (int k = 0; k < kCoarseSpheres; ++k) {
+        int total = 0;
+        for (int g = kBlockSub0[k]; g < kBlockSub0[k] + kBlockSubs[k]; ++g) {
+            if (g >= kSubBlocks || kSubFirst[g] != kSubFirst[kBlockSub0[k]] + total) return false;
+            for (int m = 0; m < kSubCount[g]; ++m) {
+                const int i = kSubIdx[kSubFirst[g] + m];
+                if (i < kBlockFirst[k] || i >= kBlockFirst[k] + kBlockCount[k]) return false;
+                ++seen[i];
+            }
+            total += kSubCount[g];
+        }
+        if (total != kBlockCount[k]) return false;
+    }
+    for (int i = 0; i < kFineSpheres; ++i)
+        if (seen[i] != 1) return false;
+    return true;
+}
+static_assert(sub_blocks_partition_blocks(), "fine-sphere groups partition each block");
+
 // Shared memory of one tile of NT items (sample, arm); KH = arm-arm
 // half-pairs per item.
 template <int KH, int NT> struct TileLds {
@@ -913,6 +1002,10 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
                 if (!((gate >> (p < 7 ? p : 7)) & 1u)) clo &= ~(0x7ffull << (11 * t));
             }
             if (!((gate >> 5) & 1u)) chi = 0;  // row 5 exists only for i0 == 0 (block 5)
+            // frames of the last joints walked: consecutive candidate blocks
+            // mostly lie on the same joints (the hand's four blocks on 7)
+            Frame Ta, Tb;
+            int ja = -1, jb = -1;
             while (clo | chi) {
                 int t, kb;
                 if (clo) {
@@ -926,18 +1019,39 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
                     t = 5;
                 }
                 const int p = i0 + t;
-                const Frame Ta = frame_rt(w, eb, a, 7 * a, e, k, p < 7 ? p : 7);
-                const Frame Tb = frame_rt(w, eb, b, 7 * b, e, k, kb < 7 ? kb : 7);
-                const int p0 = kTab.block_first[p], pn = kTab.block_count[p];
-                const int k0 = kTab.block_first[kb], kn = kTab.block_count[kb];
-                for (int i = p0; i < p0 + pn; ++i) {
-                    float ax, ay, az;
-                    centre_rt(Ta, i, ax, ay, az);
-                    const float ar = kTab.r[i];
-                    for (int j = k0; j < k0 + kn; ++j) {
-                        float x, y, z;
-                        centre_rt(Tb, j, x, y, z);
-                        finehit |= sphere_sphere(ax, ay, az, ar, x, y, z, kTab.r[j]) < 0.f;
+                if ((p < 7 ? p : 7) != ja) {
+                    ja = p < 7 ? p : 7;
+                    Ta = frame_rt(w, eb, a, 7 * a, e, k, ja);
+                }
+                if ((kb < 7 ? kb : 7) != jb) {
+                    jb = kb < 7 ? kb : 7;
+                    Tb = frame_rt(w, eb, b, 7 * b, e, k, jb);
+                }
+                // groups of block p against groups of block kb; only
+                // groups whose balls meet have their sphere pairs tested
+                const int ga0 = kSub.sub0[p], ga1 = ga0 + kSub.subs[p];
+                const int gb0 = kSub.sub0[kb], gb1 = gb0 + kSub.subs[kb];
+                for (int ga = ga0; ga < ga1; ++ga) {
+                    float gx, gy, gz;
+                    centre(Ta, kSub.c[ga][0], kSub.c[ga][1], kSub.c[ga][2], gx, gy, gz);
+                    for (int gb = gb0; gb < gb1; ++gb) {
+                        float hx, hy, hz;
+                        centre(Tb, kSub.c[gb][0], kSub.c[gb][1], kSub.c[gb][2], hx, hy, hz);
+                        if (sphere_sphere(gx, gy, gz, kSub.r[ga], hx, hy, hz, kSub.r[gb]) >= 0.f) continue;
+                        for (int m = kSub.first[ga]; m < kSub.first[ga] + kSub.count[ga]; ++m) {
+                            const int i = kSub.idx[m];
+                            float ax, ay, az;
+                            centre_rt(Ta, i, ax, ay, az);
+                            const float ar = kTab.r[i];
+                            // a sphere that misses group gb's ball misses its spheres
+                            if (sphere_sphere(ax, ay, az, ar, hx, hy, hz, kSub.r[gb]) >= 0.f) continue;
+                            for (int mb = kSub.first[gb]; mb < kSub.first[gb] + kSub.count[gb]; ++mb) {
+                                const int j = kSub.idx[mb];
+                                float x, y, z;
+                                centre_rt(Tb, j, x, y, z);
+                                finehit |= sphere_sphere(ax, ay, az, ar, x, y, z, kTab.r[j]) < 0.f;
+                            }
+                        }
                     }
                 }
                 if (finehit) done[le] = 1u;

@@ -7,6 +7,7 @@
 // results are committed in expansion order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdint>
 
@@ -41,7 +42,7 @@ __device__ __forceinline__ int wave_min_int(int v) {
     return v;
 }
 
-// Squared distance of a D-vector in global memory to one in LDS (fixed order).
+// Squared distance of two D-vectors, accumulated in a fixed order.
 __device__ __forceinline__ float sqdist(const float *a, const float *b, int D) {
     float s = 0.f;
     for (int d = 0; d < D; ++d) {
@@ -53,6 +54,20 @@ __device__ __forceinline__ float sqdist(const float *a, const float *b, int D) {
 
 __device__ __forceinline__ float *tree_q(const SlotDev &s, int t, int i) {
     return s.tr_q + ((size_t)t * s.tr_cap + i) * s.D;
+}
+
+// Tree scans read 64 rows of D floats at a time: the wave copies the chunk
+// rows[base .. base + c) (c * D consecutive floats, so the loads coalesce)
+// into LDS, and lane i then reads row base + i from there -- the rows lanes
+// would otherwise load themselves lie D floats apart, one cache line per
+// lane and load.  The chunk never reaches past row base + c - 1 < n <= the
+// tree's capacity.  Lane i keeps handling rows i, i + 64, ..., as a
+// per-lane strided scan does, so the results (and ties) do not change.
+constexpr int kScanRows = 64;
+__device__ __forceinline__ void stage_rows(const float *rows, int base, int c, int D, float *stage, int lane) {
+    const float *src = rows + (size_t)base * D;
+    const int nf = c * D;
+    for (int x = lane; x < nf; x += 64) stage[x] = src[x];
 }
 
 // A dRRT round runs for a slot while its pass is unsolved and the launch has
@@ -253,10 +268,7 @@ __global__ void k_rm_assign(SlotDev *slots, const int *ids) {
 // this is plain id order; in a roadmap taken from the cache they are its
 // newest nodes, and every new node still tries them first, as it would in a
 // roadmap of its own.
-__global__ __launch_bounds__(64) void k_rm_radius(SlotDev *slots, const int *ids) {
-    SlotDev &s = slots[ids[blockIdx.y]];
-    const int t = blockIdx.x;
-    if (t >= s.A * s.M) return;
+__device__ __forceinline__ void rm_radius_one(SlotDev &s, const int t) {
     const int a = t / s.M, i = t - a * s.M;
     const int id = s.rs_id[(size_t)a * s.M + i];
     const int lane = threadIdx.x;
@@ -389,6 +401,10 @@ __global__ __launch_bounds__(64) void k_rm_radius(SlotDev *slots, const int *ids
     }
     if (lane == 0) cnt[i] = min(found, s.rm_Kc);
 }
+__global__ __launch_bounds__(64) void k_rm_radius(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    for (int t = blockIdx.x; t < s.A * s.M; t += gridDim.x) rm_radius_one(s, t);  // a wave per item, striding
+}
 
 // First packed edge of each sample: the staged candidates of an arm form
 // one edge list neighbour -> new node in sample order (roadmap_interior.cuh:
@@ -422,10 +438,7 @@ __global__ __launch_bounds__(1024) void k_rm_compact(SlotDev *slots, const int *
 
 // Write the packed edges of one sample: lane k is candidate k.  One wave per
 // (slot, arm, sample).
-__global__ __launch_bounds__(64) void k_rm_pack(SlotDev *slots, const int *ids) {
-    SlotDev &s = slots[ids[blockIdx.y]];
-    const int t = blockIdx.x;
-    if (t >= s.A * s.M) return;
+__device__ __forceinline__ void rm_pack_one(SlotDev &s, const int t) {
     const int a = t / s.M, i = t - a * s.M;
     const int k = threadIdx.x;
     if (k >= s.rs_ncand[(size_t)a * s.M + i]) return;
@@ -442,6 +455,10 @@ __global__ __launch_bounds__(64) void k_rm_pack(SlotDev *slots, const int *ids) 
     s.re_pair[e * 2 + 0] = n;
     s.re_pair[e * 2 + 1] = id;
     s.re_coll[e] = 0u;  // the checker only sets bits
+}
+__global__ __launch_bounds__(64) void k_rm_pack(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    for (int t = blockIdx.x; t < s.A * s.M; t += gridDim.x) rm_pack_one(s, t);  // a wave per item, striding
 }
 
 // Parents always have smaller ids than their children, so the walk ends
@@ -492,10 +509,7 @@ __device__ __forceinline__ void uf_union(int *uf, int a, int b, int *events) {
 //                    bitset claimed for every older endpoint u;
 //   k_rm_link_bits   one wave per sample: set bit (v - base) in u's bitset;
 //   k_rm_link_append waves over the claimed bitsets: append, count, clear.
-__global__ __launch_bounds__(64) void k_rm_link_own(SlotDev *slots, const int *ids) {
-    SlotDev &s = slots[ids[blockIdx.y]];
-    const int t = blockIdx.x;
-    if (t >= s.A * s.M) return;
+__device__ __forceinline__ void rm_link_own_one(SlotDev &s, const int t) {
     const int a = t / s.M, i = t - a * s.M;
     const int n = s.rs_ncand[(size_t)a * s.M + i];
     if (n == 0) return;
@@ -526,11 +540,12 @@ __global__ __launch_bounds__(64) void k_rm_link_own(SlotDev *slots, const int *i
     // v is new: nothing else writes its degree before k_rm_link_append
     if (lane == 0) deg[v] = __popcll(bal);
 }
-
-__global__ __launch_bounds__(64) void k_rm_link_bits(SlotDev *slots, const int *ids) {
+__global__ __launch_bounds__(64) void k_rm_link_own(SlotDev *slots, const int *ids) {
     SlotDev &s = slots[ids[blockIdx.y]];
-    const int t = blockIdx.x;
-    if (t >= s.A * s.M) return;
+    for (int t = blockIdx.x; t < s.A * s.M; t += gridDim.x) rm_link_own_one(s, t);  // a wave per item, striding
+}
+
+__device__ __forceinline__ void rm_link_bits_one(SlotDev &s, const int t) {
     const int a = t / s.M, i = t - a * s.M;
     const int n = s.rs_ncand[(size_t)a * s.M + i];
     const int lane = threadIdx.x;
@@ -541,6 +556,10 @@ __global__ __launch_bounds__(64) void k_rm_link_bits(SlotDev *slots, const int *
     const int k = s.rl_slot[(size_t)a * s.rm_cap + u];
     const int j = v - s.rs_base[a];
     atomicOr(&s.rl_bits[((size_t)a * s.rl_cap + k) * s.rl_words + (j >> 5)], 1u << (j & 31));
+}
+__global__ __launch_bounds__(64) void k_rm_link_bits(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    for (int t = blockIdx.x; t < s.A * s.M; t += gridDim.x) rm_link_bits_one(s, t);  // a wave per item, striding
 }
 
 __global__ __launch_bounds__(256) void k_rm_link_append(SlotDev *slots, const int *ids) {
@@ -701,6 +720,7 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
     __shared__ float vn[35];
     __shared__ float nq[35];
     __shared__ int nrid[kMaxArms];
+    __shared__ float stage[kScanRows * 35];
 
     // ---- tree choice (pop_planner.cu:366-376)
     int t = round == 0 ? (b < s.B / 2 ? 0 : 1) : s.ex_tree[b];  // first iteration of a launch
@@ -780,14 +800,21 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
 
     // ---- nearest node of tree t (pop_planner.cu:472-508)
     const int nt = s.tr_n[t];
+    const float *rows = tree_q(s, t, 0);
     float bd = FLT_MAX;
     int bi = 0;
-    for (int i = lane; i < nt; i += 64) {
-        if (s.optimize && !(s.tr_cost[(size_t)t * s.tr_cap + i] < c_rand)) continue;
-        const float d = sqdist(tree_q(s, t, i), q, D);
-        if (d < bd) {
-            bd = d;
-            bi = i;
+    for (int base = 0; base < nt; base += kScanRows) {
+        const int c = min(kScanRows, nt - base);
+        __syncthreads();  // the previous chunk has been read
+        stage_rows(rows, base, c, D, stage, lane);
+        __syncthreads();
+        const int i = base + lane;
+        if (lane < c && !(s.optimize && !(s.tr_cost[(size_t)t * s.tr_cap + i] < c_rand))) {
+            const float d = sqdist(stage + lane * D, q, D);
+            if (d < bd) {
+                bd = d;
+                bi = i;
+            }
         }
     }
     wave_argmin(bd, bi);
@@ -841,12 +868,17 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
     __syncthreads();
 
     // ---- existing node within squared distance 1e-5 (pop_planner.cu:526-533)
+    // the lowest index within squared distance 1e-5: once a chunk holds one,
+    // later chunks hold only higher indices
     int exist = 0x7fffffff;
-    for (int i = lane; i < nt; i += 64)
-        if (sqdist(tree_q(s, t, i), nq, D) <= 1e-5f) {
-            exist = i;
-            break;
-        }
+    for (int base = 0; base < nt; base += kScanRows) {
+        const int c = min(kScanRows, nt - base);
+        __syncthreads();
+        stage_rows(rows, base, c, D, stage, lane);
+        __syncthreads();
+        if (lane < c && sqdist(stage + lane * D, nq, D) <= 1e-5f) exist = base + lane;
+        if (__ballot(exist != 0x7fffffff)) break;
+    }
     exist = wave_min_int(exist);
     const float new_cost = s.tr_cost[(size_t)t * s.tr_cap + near] + dnear;  // (pop_planner.cu:541)
     if (exist != 0x7fffffff) {
@@ -993,18 +1025,26 @@ __global__ __launch_bounds__(64) void k_dr_connect(SlotDev *slots, const int *id
     if (s.ex_state[b] != 1 || idx < 0) return;
     const int t = s.ex_tree[b], o = 1 - t, D = s.D;
     __shared__ float q[35];
+    __shared__ float stage[kScanRows * 35];
     if (lane < D) q[lane] = tree_q(s, t, idx)[lane];
     __syncthreads();
     const float best = *s.best_cost;
     const float ub = best - s.tr_cost[(size_t)t * s.tr_cap + idx];
     const int no = s.tr_n[o];
+    const float *rows = tree_q(s, o, 0);
     float bd = FLT_MAX;
     int bi = 0x7fffffff;
-    for (int i = lane; i < no; i += 64) {
-        const float d = sqdist(tree_q(s, o, i), q, D);
-        if (d < bd && (!s.optimize || sqrtf(d) < ub)) {
-            bd = d;
-            bi = i;
+    for (int base = 0; base < no; base += kScanRows) {
+        const int c = min(kScanRows, no - base);
+        __syncthreads();
+        stage_rows(rows, base, c, D, stage, lane);
+        __syncthreads();
+        if (lane < c) {
+            const float d = sqdist(stage + lane * D, q, D);
+            if (d < bd && (!s.optimize || sqrtf(d) < ub)) {
+                bd = d;
+                bi = base + lane;
+            }
         }
     }
     wave_argmin(bd, bi);
@@ -1414,16 +1454,22 @@ void launch_rm_sample(SlotDev *d, const int *ids, int n, int max_am, hipStream_t
 void launch_rm_assign(SlotDev *d, const int *ids, int n, hipStream_t stream) {
     MRPOP_L(k_rm_assign, dim3(kMaxArms, n), dim3(256), d, ids);
 }
+// Waves of the per-(arm, sample) roadmap kernels per slot: one per item for
+// a few slots, at most about 16 K waves over all slots for many (each wave
+// then takes every gridDim.x-th item), since a launch of one short wave per
+// item of 50 five-arm slots (460 K waves) is bound by wave dispatch.
+static int rm_waves(int max_am, int n) { return std::max(64, std::min(max_am, 16384 / std::max(1, n))); }
+
 void launch_rm_radius(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream) {
-    MRPOP_L(k_rm_radius, dim3(max_am, n), dim3(64), d, ids);
+    MRPOP_L(k_rm_radius, dim3(rm_waves(max_am, n), n), dim3(64), d, ids);
 }
 void launch_rm_compact(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream) {
     MRPOP_L(k_rm_compact, dim3(kMaxArms, n), dim3(1024), d, ids);
-    MRPOP_L(k_rm_pack, dim3(max_am, n), dim3(64), d, ids);
+    MRPOP_L(k_rm_pack, dim3(max_am, n), dim3(64), d, ids);  // short stores: a wave per item is faster here
 }
 void launch_rm_link(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream) {
-    MRPOP_L(k_rm_link_own, dim3(max_am, n), dim3(64), d, ids);
-    MRPOP_L(k_rm_link_bits, dim3(max_am, n), dim3(64), d, ids);
+    MRPOP_L(k_rm_link_own, dim3(rm_waves(max_am, n), n), dim3(64), d, ids);
+    MRPOP_L(k_rm_link_bits, dim3(rm_waves(max_am, n), n), dim3(64), d, ids);
     MRPOP_L(k_rm_link_append, dim3(64, kMaxArms, n), dim3(256), d, ids);
 }
 void launch_rm_connected(SlotDev *d, const int *ids, int n, hipStream_t stream) {
