@@ -457,6 +457,17 @@ extern "C" int mrpop_diag_set_lane_deadline(mrpop_ctx *c, double seconds) {
     return MRPOP_OK;
 }
 
+// Diagnostics (not part of the public ABI): the tree size from which dRRT
+// expansions run in the five-launch form (0: always, < 0: never) and the
+// least tree rows a block of its scans takes.  Both forms give the same
+// results; tests compare them.
+extern "C" int mrpop_diag_set_expand_split(mrpop_ctx *c, int min_nodes, int slice_rows) {
+    if (!c || slice_rows < 1) return MRPOP_ERR_INVALID;
+    ((Context *)c)->expand_split_nodes = min_nodes;
+    ((Context *)c)->expand_slice_rows = slice_rows;
+    return MRPOP_OK;
+}
+
 // Diagnostics (not part of the public ABI): the planner's counter-based
 // generator, Philox4x32-10, evaluated on the host.
 extern "C" void mrpop_diag_philox(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]) {

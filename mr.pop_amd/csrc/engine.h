@@ -80,6 +80,7 @@ struct Slot {
     bool seeking = true;   // the pass looks for a first solution (no optimise mode, no hyperspheroid)
     bool pass_solved = false;
     int dr_round = 0, dr_budget = 0;
+    int tree_n = 0;  // larger tree of the dRRT launch at the last report (0 after a reset)
     int sp_stage = 0;
     float best_cost = 3.402823466e38f;
     std::chrono::steady_clock::time_point t_submit, t_kernel_start;

@@ -180,6 +180,7 @@ size_t Engine::slot_bytes(int A, const mrpop_settings &st, const SolveOptions &o
     L.add(sizeof(int) * B * A);                                             // ex_rid
     L.add(sizeof(unsigned long long) * edge_hash_size(B));                  // ex_hash
     L.add(sizeof(float) * (size_t)B * (D + 1));                             // ex_sample, ex_crand
+    L.add(sizeof(unsigned long long) * B + sizeof(int) * B);                // ex_nnkey, ex_live
     L.add(sizeof(float) * (size_t)B * D * 2 + sizeof(int) * B * 2);         // ed_from/to, ed_b, ed_coll
     L.add(sizeof(int) * B * 3 + sizeof(float) * B);                         // cn_*
     L.add(sizeof(float) * ce_cap * D * 2);                                  // ce_from/to
@@ -490,6 +491,8 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     v.ex_hmask = (int)edge_hash_size(B) - 1;
     v.ex_sample = (float *)take(sizeof(float) * (size_t)B * (D + 1));
     v.ex_crand = v.ex_sample + (size_t)B * D;
+    v.ex_nnkey = (unsigned long long *)take(sizeof(unsigned long long) * B + sizeof(int) * B);
+    v.ex_live = (int *)(v.ex_nnkey + B);
     float *ed = (float *)take(sizeof(float) * (size_t)B * D * 2 + sizeof(int) * B * 2);
     v.ed_from = ed;
     v.ed_to = ed + (size_t)B * D;
@@ -832,13 +835,25 @@ int Engine::launch_lane(Lane &L) {
     if (!L.dr.empty()) {
         int *d = upload_ids(L, L.dr);
         const int n = (int)L.dr.size();
-        int max_b = 0, rounds = 1 << 30;
+        int max_b = 0, rounds = 1 << 30, tree_n = 0;
         bool reset = false;
         for (Slot *s : L.dr) {
             max_b = std::max(max_b, s->dv.B);
             rounds = std::min(rounds, std::max(1, s->opt.rounds_per_step));
+            if (s->dv.round_reset) s->tree_n = 0;
+            tree_n = std::max(tree_n, s->tree_n);
             reset |= s->dv.round_reset != 0;
             s->dv.round_reset = 0;  // applied by this step's k_dr_begin
+        }
+        // Large trees: the tree scans run a thread per expansion, each block
+        // over a slice of the trees, with about 2048 blocks in all and slices
+        // of at least expand_slice_rows rows.  The slice count only changes
+        // the speed.
+        int splits = 0;
+        if (c_->expand_split_nodes >= 0 && tree_n >= c_->expand_split_nodes) {
+            const int groups = (max_b + 63) / 64 * n;
+            const int rows = c_->expand_slice_rows;
+            splits = std::max(1, std::min((2048 + groups - 1) / groups, (tree_n + rows - 1) / rows));
         }
         std::vector<const dev::EdgeGroup *> g1, g2;
         for (Slot *s : L.dr) {
@@ -852,7 +867,7 @@ int Engine::launch_lane(Lane &L) {
         if (reset) plan::launch_dr_begin(ds, d, n, st);
         auto enqueue_rounds = [&]() -> int {
             for (int r = 0; r < rounds; ++r) {
-                plan::launch_dr_expand(ds, d, n, max_b, st);
+                plan::launch_dr_expand(ds, d, n, max_b, splits, st);
                 if (int rc = launch_staged(L, s1)) return rc;
                 plan::launch_dr_commit(ds, d, n, st);
                 plan::launch_dr_connect(ds, d, n, max_b, st);
@@ -871,7 +886,7 @@ int Engine::launch_lane(Lane &L) {
             // sequence captured once replays for as long as the launch
             // arguments it bakes in stay the same.  It holds kernel nodes only.
             std::vector<uint64_t> key = {(uint64_t)(uintptr_t)ds, (uint64_t)(uintptr_t)d, (uint64_t)n,
-                                         (uint64_t)max_b, (uint64_t)rounds, (uint64_t)(uintptr_t)L.d_wave,
+                                         (uint64_t)max_b, (uint64_t)rounds, (uint64_t)splits, (uint64_t)(uintptr_t)L.d_wave,
                                          (uint64_t)(uintptr_t)d_worlds_};
             for (const Staged *sg : {&s1, &s2}) {
                 key.push_back((uint64_t)sg->mode);
@@ -996,6 +1011,7 @@ int Engine::process_lane(Lane &L) {
         } else if (in_dr) {
             s->dr_round = r.ctr[plan_ctr_round];
             s->res.rounds = r.ctr[plan_ctr_run];
+            s->tree_n = std::max(r.tr_n[0], r.tr_n[1]);
             if (r.status > 0) {
                 s->pass_solved = true;
                 s->best_cost = r.best_cost;

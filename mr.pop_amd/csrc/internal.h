@@ -65,6 +65,10 @@ struct Context {
     // a planner step whose kernels have not completed after this many seconds
     // is taken to be stuck: its problems end with MRPOP_ERR_TIMEOUT
     double lane_deadline_s = 120.0;
+    // dRRT expansions use the five-launch form (thread-per-expansion tree
+    // scans) once a tree of the launch holds this many nodes; < 0: never
+    int expand_split_nodes = 1024;
+    int expand_slice_rows = 256;  // least tree rows per block of the five-launch scans
 
     int fail(int code, const char *fmt, ...) {
         char buf[512];

@@ -124,6 +124,8 @@ struct SlotDev {
     int *ex_dup;              // [B] expansion whose identical edge is checked for b (-1 none)
     float *ex_sample;         // [B][D] the round's sample of expansion b
     float *ex_crand;          // [B] its cost bound (optimise passes; FLT_MAX otherwise; -1 if rejected)
+    unsigned long long *ex_nnkey;  // [B] five-launch form: (squared distance bits << 32 | index) of the nearest node
+    int *ex_live;             // [B] five-launch form: 1 sampled, 2 stepped, 0 ended
     unsigned long long *ex_hash;  // [ex_hmask+1] edges of the round: (round tag << 32) | b
     int ex_hmask;
     unsigned long long *rw_key;  // [tr_cap*2] rewire arbitration (cost bits << 32 | b)
@@ -206,7 +208,10 @@ void launch_rm_compact(SlotDev *d, const int *ids, int n, int max_am, hipStream_
 void launch_rm_link(SlotDev *d, const int *ids, int n, int max_am, hipStream_t stream);
 void launch_rm_connected(SlotDev *d, const int *ids, int n, hipStream_t stream);
 void launch_dr_begin(SlotDev *d, const int *ids, int n, hipStream_t stream);
-void launch_dr_expand(SlotDev *d, const int *ids, int n, int max_b, hipStream_t stream);
+// splits > 0: the five-launch form, whose tree scans run a thread per
+// expansion over `splits` slices of each tree (large trees); 0: one wave per
+// expansion.  Both give the same results.
+void launch_dr_expand(SlotDev *d, const int *ids, int n, int max_b, int splits, hipStream_t stream);
 void launch_dr_commit(SlotDev *d, const int *ids, int n, hipStream_t stream);
 void launch_dr_connect(SlotDev *d, const int *ids, int n, int max_b, hipStream_t stream);
 void launch_dr_finish(SlotDev *d, const int *ids, int n, hipStream_t stream);

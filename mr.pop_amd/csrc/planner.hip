@@ -705,23 +705,30 @@ __device__ int claim_edge(const SlotDev &s, unsigned tag, int b, int t, int near
     return -1;
 }
 
-// One dRRT expansion per wave (pop_planner.cu:350-622 up to the edge check):
-// pick the tree, sample (uniform or PHS), nearest tree node, per-arm roadmap
-// neighbour of minimum angle, duplicate test, and the arm-arm edge to check.
-__global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids) {
-    SlotDev &s = slots[ids[blockIdx.y]];
-    const int b = blockIdx.x;
-    if (b >= s.B || !searching(s)) return;
-    const int lane = threadIdx.x;
-    const int D = s.D, A = s.A;
+// One dRRT expansion (pop_planner.cu:350-622 up to the edge check): pick the
+// tree, sample (uniform or PHS), nearest tree node, per-arm roadmap neighbour
+// of minimum angle, duplicate test, and the arm-arm edge to check.  The
+// stages below run in one wave per expansion (k_dr_expand), or, when the
+// trees are large, as five launches in which the two tree scans (nearest
+// node, existing node) run a thread per expansion over tree rows shared by
+// a block of 64 expansions (k_dr_sample, k_dr_scan<false>, k_dr_step,
+// k_dr_scan<true>, k_dr_cand).  Both forms compute the same values in the same order.
+struct ExpandLds {
+    float q[35];
+    float vn[35];
+    float nq[35];
+    int nrid[kMaxArms];
+    float stage[kScanRows * 35];
+};
+
+// (1) Tree choice, sample, informed rejection and cost bound.  Writes
+// ex_tree, ex_sample, ex_crand (-1: rejected); false if the sample is
+// rejected.
+__device__ bool expand_sample(SlotDev &s, int b, int lane, ExpandLds &L, int &t_out, float &c_rand_out) {
+    const int D = s.D;
     const int round = s.ctr[kCtrRound];
     const unsigned rng_round = (unsigned)s.ctr[kCtrRng];
-    __shared__ float q[35];
-    __shared__ float vn[35];
-    __shared__ float nq[35];
-    __shared__ int nrid[kMaxArms];
-    __shared__ float stage[kScanRows * 35];
-
+    float *q = L.q, *nq = L.nq;
     // ---- tree choice (pop_planner.cu:366-376)
     int t = round == 0 ? (b < s.B / 2 ? 0 : 1) : s.ex_tree[b];  // first iteration of a launch
     if (round > 0 && s.balance == 2) {
@@ -779,7 +786,7 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
             d1 = x1 * x1;
         }
         const float g0 = sqrtf(wave_sum(d0)), g1 = sqrtf(wave_sum(d1));
-        if (g0 + g1 >= best) return;
+        if (g0 + g1 >= best) return false;
     }
     // ---- cost bound of the sample (pop_planner.cu:459-469)
     float c_rand = FLT_MAX;
@@ -797,8 +804,16 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
         c_rand = u01(cr.x) * c_range + g_hat;
     }
     if (lane == 0) s.ex_crand[b] = c_rand;
+    t_out = t;
+    c_rand_out = c_rand;
+    return true;
+}
 
-    // ---- nearest node of tree t (pop_planner.cu:472-508)
+// (2) Nearest node of tree t to L.q (pop_planner.cu:472-508), by the wave:
+// lane i scans rows i, i + 64, ...; the least squared distance, lowest index
+// among equals.
+__device__ void expand_nn_wave(SlotDev &s, int t, float c_rand, int lane, ExpandLds &L, int &near, float &dsq) {
+    const int D = s.D;
     const int nt = s.tr_n[t];
     const float *rows = tree_q(s, t, 0);
     float bd = FLT_MAX;
@@ -806,11 +821,11 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
     for (int base = 0; base < nt; base += kScanRows) {
         const int c = min(kScanRows, nt - base);
         __syncthreads();  // the previous chunk has been read
-        stage_rows(rows, base, c, D, stage, lane);
+        stage_rows(rows, base, c, D, L.stage, lane);
         __syncthreads();
         const int i = base + lane;
         if (lane < c && !(s.optimize && !(s.tr_cost[(size_t)t * s.tr_cap + i] < c_rand))) {
-            const float d = sqdist(stage + lane * D, q, D);
+            const float d = sqdist(L.stage + lane * D, L.q, D);
             if (d < bd) {
                 bd = d;
                 bi = i;
@@ -818,12 +833,18 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
         }
     }
     wave_argmin(bd, bi);
-    const int near = bi;
-    const float dnear = sqrtf(bd);  // distance from the nearest node to the sample
+    near = bi;
+    dsq = bd;
+}
+
+// (3) Per-arm roadmap step of minimum angle from tree node `near` toward L.q
+// (utils.cuh:688-759): L.vn <- the node, L.nq / L.nrid <- the new
+// composite.  false if some arm has no admissible neighbour.
+__device__ bool expand_step(SlotDev &s, int t, int near, float c_rand, int lane, ExpandLds &L) {
+    const int D = s.D, A = s.A;
+    float *q = L.q, *vn = L.vn, *nq = L.nq;
     if (lane < D) vn[lane] = tree_q(s, t, near)[lane];
     __syncthreads();
-
-    // ---- per-arm roadmap step of minimum angle (utils.cuh:688-759)
     float bound = s.optimize ? c_rand - s.tr_cost[(size_t)t * s.tr_cap + near] : FLT_MAX;
     for (int a = 0; a < A; ++a) {
         const int rid = s.tr_rid[((size_t)t * s.tr_cap + near) * A + a];
@@ -854,9 +875,9 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
             }
         }
         wave_argmin(bang, bidx);
-        if (bidx == 0x7fffffff) return;  // this arm has no admissible neighbour
+        if (bidx == 0x7fffffff) return false;  // this arm has no admissible neighbour
         if (lane < 7) nq[7 * a + lane] = rq[(size_t)bidx * 8 + lane];
-        if (lane == 0) nrid[a] = bidx;
+        if (lane == 0) L.nrid[a] = bidx;
         __syncthreads();
         float d2 = 0.f;
         for (int k = 0; k < 7; ++k) {
@@ -866,41 +887,55 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
         bound -= sqrtf(d2);
     }
     __syncthreads();
+    return true;
+}
 
-    // ---- existing node within squared distance 1e-5 (pop_planner.cu:526-533)
-    // the lowest index within squared distance 1e-5: once a chunk holds one,
-    // later chunks hold only higher indices
+// (4) The lowest index of a node of tree t within squared distance 1e-5 of
+// L.nq (pop_planner.cu:526-533), by the wave; 0x7fffffff if none.  Once a
+// chunk holds one, later chunks hold only higher indices.
+__device__ int expand_exist_wave(SlotDev &s, int t, int lane, ExpandLds &L) {
+    const int D = s.D;
+    const int nt = s.tr_n[t];
+    const float *rows = tree_q(s, t, 0);
     int exist = 0x7fffffff;
     for (int base = 0; base < nt; base += kScanRows) {
         const int c = min(kScanRows, nt - base);
         __syncthreads();
-        stage_rows(rows, base, c, D, stage, lane);
+        stage_rows(rows, base, c, D, L.stage, lane);
         __syncthreads();
-        if (lane < c && sqdist(stage + lane * D, nq, D) <= 1e-5f) exist = base + lane;
+        if (lane < c && sqdist(L.stage + lane * D, L.nq, D) <= 1e-5f) exist = base + lane;
         if (__ballot(exist != 0x7fffffff)) break;
     }
-    exist = wave_min_int(exist);
+    return wave_min_int(exist);
+}
+
+// (5) The candidate: edge near -> new for the arm-arm check, unless the new
+// node exists and is not cheaper, or is above the best cost.  L.vn, L.nq,
+// L.nrid hold the step's result.
+__device__ void expand_candidate(SlotDev &s, int b, int t, int near, float dnear, int exist, int lane, ExpandLds &L) {
+    const int D = s.D, A = s.A;
+    const unsigned rng_round = (unsigned)s.ctr[kCtrRng];
+    const float best = *s.best_cost;
     const float new_cost = s.tr_cost[(size_t)t * s.tr_cap + near] + dnear;  // (pop_planner.cu:541)
     if (exist != 0x7fffffff) {
         if (!s.optimize || new_cost >= s.tr_cost[(size_t)t * s.tr_cap + exist]) return;
     }
     if (s.optimize && new_cost >= best) return;
 
-    // ---- candidate: edge near -> new for the arm-arm check
     if (lane == 0) {
         s.ex_state[b] = exist != 0x7fffffff ? 2 : 1;
         s.ex_near[b] = near;
         s.ex_exist[b] = exist;
         s.ex_cost[b] = new_cost;
     }
-    if (lane < A) s.ex_rid[b * A + lane] = nrid[lane];
-    if (lane < D) s.ex_q[(size_t)b * D + lane] = nq[lane];
+    if (lane < A) s.ex_rid[b * A + lane] = L.nrid[lane];
+    if (lane < D) s.ex_q[(size_t)b * D + lane] = L.nq[lane];
     __syncthreads();
     // The edge is fixed by (t, near, nrid[]): an expansion of this round with
     // the same triple has the same edge and so the same verdict, and the
     // edge is checked once.
     int dup = -1;
-    if (lane == 0) dup = claim_edge(s, rng_round + 1u, b, t, near, nrid);
+    if (lane == 0) dup = claim_edge(s, rng_round + 1u, b, t, near, L.nrid);
     dup = __shfl(dup, 0);
     if (dup >= 0) {
         if (lane == 0) s.ex_dup[b] = dup;
@@ -910,13 +945,188 @@ __global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids
     if (lane == 0) slot = atomicAdd(s.ed_count, 1);
     slot = __shfl(slot, 0);
     if (lane < D) {
-        s.ed_from[(size_t)slot * D + lane] = vn[lane];
-        s.ed_to[(size_t)slot * D + lane] = nq[lane];
+        s.ed_from[(size_t)slot * D + lane] = L.vn[lane];
+        s.ed_to[(size_t)slot * D + lane] = L.nq[lane];
     }
     if (lane == 0) {
         s.ed_b[slot] = b;
         s.ed_coll[slot] = 0u;  // the checker only sets bits
     }
+}
+
+// All five stages in one wave per expansion.
+__global__ __launch_bounds__(64) void k_dr_expand(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int b = blockIdx.x;
+    if (b >= s.B || !searching(s)) return;
+    const int lane = threadIdx.x;
+    __shared__ ExpandLds L;
+    int t, near;
+    float c_rand, dsq;
+    if (!expand_sample(s, b, lane, L, t, c_rand)) return;
+    expand_nn_wave(s, t, c_rand, lane, L, near, dsq);
+    if (!expand_step(s, t, near, c_rand, lane, L)) return;
+    const int exist = expand_exist_wave(s, t, lane, L);
+    expand_candidate(s, b, t, near, sqrtf(dsq), exist, lane, L);
+}
+
+// ---- the same stages over five launches, for large trees
+__global__ __launch_bounds__(64) void k_dr_sample(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int b = blockIdx.x;
+    if (b >= s.B || !searching(s)) return;
+    __shared__ ExpandLds L;
+    int t;
+    float c_rand;
+    const bool ok = expand_sample(s, b, threadIdx.x, L, t, c_rand);
+    if (threadIdx.x == 0) {
+        s.ex_live[b] = ok ? 1 : 0;
+        s.ex_nnkey[b] = ~0ull;
+        s.ex_exist[b] = 0x7fffffff;
+    }
+}
+
+// Thread-per-expansion scans.  A block holds 64 expansions of one slot and
+// one of `splits` contiguous slices of each tree (whole chunks of 64 rows);
+// the rows are staged a chunk at a time in LDS and every thread compares
+// them with its own query, held in registers, in index order.  Slices merge
+// by atomicMin: on (squared distance bits, index) for the nearest node (a
+// non-negative float orders as its bits, so the least distance wins and the
+// lowest index among equals, as in the wave scan), on the index for the
+// existing node.  A squared distance only grows term by term (fma of a
+// square, rounded upward or to nearest, never below the partial sum), so a
+// row whose first arms already reach the bound cannot pass and its remaining
+// terms are skipped.
+template <int D, bool EXIST>
+__device__ void scan_block(SlotDev &s, int b, int split, int splits, float *stage, float *cstage) {
+    const int lane = threadIdx.x;
+    const bool live = b < s.B && s.ex_live[b] == (EXIST ? 2 : 1);
+    const int t_me = live ? s.ex_tree[b] : -1;
+    const float c_rand = live && !EXIST ? s.ex_crand[b] : 0.f;
+    const float *src = EXIST ? s.ex_q : s.ex_sample;
+    float q[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = live ? src[(size_t)b * D + d] : 0.f;
+    float bd = FLT_MAX;
+    int bi = 0x7fffffff;
+    for (int t = 0; t < 2; ++t) {
+        if (!__syncthreads_or(t_me == t)) continue;
+        const int nt = s.tr_n[t];
+        const int per = ((nt + kScanRows - 1) / kScanRows + splits - 1) / splits * kScanRows;
+        const int lo = split * per, hi = min(nt, lo + per);
+        const float *rows = tree_q(s, t, 0);
+        const float *cost = s.tr_cost + (size_t)t * s.tr_cap;
+        for (int base = lo; base < hi; base += kScanRows) {
+            const int c = min(kScanRows, hi - base);
+            __syncthreads();  // the previous chunk has been read
+            stage_rows(rows, base, c, D, stage, lane);
+            if (!EXIST && s.optimize && lane < c) cstage[lane] = cost[base + lane];
+            __syncthreads();
+            if (t_me == t && !(EXIST && bi != 0x7fffffff)) {
+                for (int r = 0; r < c; ++r) {
+                    if (!EXIST && s.optimize && !(cstage[r] < c_rand)) continue;
+                    const float *row = stage + r * D;
+                    float acc = 0.f;
+                    bool out = false;
+#pragma unroll
+                    for (int a = 0; a < D / 7; ++a) {
+                        if (!out) {
+#pragma unroll
+                            for (int k = 0; k < 7; ++k) {
+                                const float x = row[7 * a + k] - q[7 * a + k];
+                                acc = __builtin_fmaf(x, x, acc);
+                            }
+                            if (a + 1 < D / 7) out = EXIST ? acc > 1e-5f : acc >= bd;
+                        }
+                    }
+                    if (out) continue;
+                    if (EXIST) {
+                        if (acc <= 1e-5f) {
+                            bi = base + r;
+                            break;
+                        }
+                    } else if (acc < bd) {
+                        bd = acc;
+                        bi = base + r;
+                    }
+                }
+            }
+            // the existing-node scan of a slice ends once every expansion of
+            // the block on this tree has its node
+            if (EXIST && __syncthreads_and(t_me != t || bi != 0x7fffffff)) break;
+        }
+    }
+    if (!live || bi == 0x7fffffff) return;
+    if (EXIST)
+        atomicMin(s.ex_exist + b, bi);
+    else
+        atomicMin(s.ex_nnkey + b, ((unsigned long long)__float_as_uint(bd) << 32) | (unsigned)bi);
+}
+
+template <bool EXIST>
+__global__ __launch_bounds__(64) void k_dr_scan(SlotDev *slots, const int *ids, int splits) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int group = blockIdx.x / splits, split = blockIdx.x % splits;
+    const int b0 = group * 64;
+    if (b0 >= s.B || !searching(s)) return;
+    __shared__ float stage[kScanRows * 35];
+    __shared__ float cstage[kScanRows];
+    const int b = b0 + threadIdx.x;
+    switch (s.D) {
+    case 14: scan_block<14, EXIST>(s, b, split, splits, stage, cstage); break;
+    case 21: scan_block<21, EXIST>(s, b, split, splits, stage, cstage); break;
+    case 28: scan_block<28, EXIST>(s, b, split, splits, stage, cstage); break;
+    default: scan_block<35, EXIST>(s, b, split, splits, stage, cstage); break;
+    }
+}
+
+// The nearest node of expansion b from the merged key: index 0 at distance
+// FLT_MAX when no node passed the cost bound, as the wave scan leaves it.
+__device__ __forceinline__ void nn_from_key(const SlotDev &s, int b, int &near, float &dsq) {
+    const unsigned long long k = s.ex_nnkey[b];
+    near = k == ~0ull ? 0 : (int)(unsigned)k;
+    dsq = k == ~0ull ? FLT_MAX : __uint_as_float((unsigned)(k >> 32));
+}
+
+__global__ __launch_bounds__(64) void k_dr_step(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int b = blockIdx.x;
+    if (b >= s.B || !searching(s) || s.ex_live[b] != 1) return;
+    const int lane = threadIdx.x;
+    __shared__ ExpandLds L;
+    const int D = s.D;
+    if (lane < D) L.q[lane] = s.ex_sample[(size_t)b * D + lane];
+    __syncthreads();
+    int near;
+    float dsq;
+    nn_from_key(s, b, near, dsq);
+    const int t = s.ex_tree[b];
+    const bool ok = expand_step(s, t, near, s.ex_crand[b], lane, L);
+    if (ok) {
+        if (lane < s.A) s.ex_rid[b * s.A + lane] = L.nrid[lane];
+        if (lane < D) s.ex_q[(size_t)b * D + lane] = L.nq[lane];
+    }
+    if (lane == 0) s.ex_live[b] = ok ? 2 : 0;
+}
+
+__global__ __launch_bounds__(64) void k_dr_cand(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int b = blockIdx.x;
+    if (b >= s.B || !searching(s) || s.ex_live[b] != 2) return;
+    const int lane = threadIdx.x;
+    __shared__ ExpandLds L;
+    const int D = s.D, A = s.A;
+    int near;
+    float dsq;
+    nn_from_key(s, b, near, dsq);
+    const int t = s.ex_tree[b];
+    if (lane < D) {
+        L.vn[lane] = tree_q(s, t, near)[lane];
+        L.nq[lane] = s.ex_q[(size_t)b * D + lane];
+    }
+    if (lane < A) L.nrid[lane] = s.ex_rid[b * A + lane];
+    __syncthreads();
+    expand_candidate(s, b, t, near, sqrtf(dsq), s.ex_exist[b], lane, L);
 }
 
 // Commit the checked expansions in expansion order (pop_planner.cu:571-622):
@@ -1478,8 +1688,17 @@ void launch_rm_connected(SlotDev *d, const int *ids, int n, hipStream_t stream) 
 void launch_dr_begin(SlotDev *d, const int *ids, int n, hipStream_t stream) {
     MRPOP_L(k_dr_begin, dim3(n), dim3(64), d, ids);
 }
-void launch_dr_expand(SlotDev *d, const int *ids, int n, int max_b, hipStream_t stream) {
-    MRPOP_L(k_dr_expand, dim3(max_b, n), dim3(64), d, ids);
+void launch_dr_expand(SlotDev *d, const int *ids, int n, int max_b, int splits, hipStream_t stream) {
+    if (splits <= 0) {
+        MRPOP_L(k_dr_expand, dim3(max_b, n), dim3(64), d, ids);
+        return;
+    }
+    const int blocks = (max_b + 63) / 64 * splits;
+    MRPOP_L(k_dr_sample, dim3(max_b, n), dim3(64), d, ids);
+    MRPOP_L(k_dr_scan<false>, dim3(blocks, n), dim3(64), d, ids, splits);
+    MRPOP_L(k_dr_step, dim3(max_b, n), dim3(64), d, ids);
+    MRPOP_L(k_dr_scan<true>, dim3(blocks, n), dim3(64), d, ids, splits);
+    MRPOP_L(k_dr_cand, dim3(max_b, n), dim3(64), d, ids);
 }
 void launch_dr_commit(SlotDev *d, const int *ids, int n, hipStream_t stream) {
     MRPOP_L(k_dr_commit, dim3(1, n), dim3(256), d, ids);

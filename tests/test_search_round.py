@@ -102,6 +102,25 @@ def test_record_path_on_a_two_goal_tree():
 
 # ---------------------------------------------------------------- GPU ----
 
+# Expansion forms: "wave" = one wave per expansion; "split" = the five-launch
+# form from the first round, with tree slices of 64 rows so that even these
+# small trees are scanned by several blocks per expansion group.  The default
+# switches from the first to the second at 1024 tree nodes.
+_FORMS = {"wave": (-1, 256), "split": (0, 64), "default": (1024, 256)}
+
+
+class _expand_form:
+    def __init__(self, ctx, form):
+        self.ctx, self.form = ctx, form
+        self.lib = _lib.lib()
+        self.lib.mrpop_diag_set_expand_split.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+
+    def __enter__(self):
+        self.ctx.check(self.lib.mrpop_diag_set_expand_split(self.ctx.handle, *_FORMS[self.form]))
+
+    def __exit__(self, *exc):
+        self.ctx.check(self.lib.mrpop_diag_set_expand_split(self.ctx.handle, *_FORMS["default"]))
+
 class _DiagSlot:
     def __init__(self, ctx, problem, st, op):
         self.lib = _lib.lib()
@@ -403,8 +422,9 @@ def _check_round(p, ds, before, after, settings, optimize=False, phs=None):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", ["wave", "split"])
 @pytest.mark.parametrize("kind,seed", [("plain2", 4), ("table3", 2), ("bin4", 1), ("shelf5", 3)])
-def test_search_rounds_match_the_restatement(ctx, kind, seed):
+def test_search_rounds_match_the_restatement(ctx, kind, seed, form):
     p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
     st = _lib.settings(num_new_configs=48, granularity=8, range=2.0)
     op = _lib.solve_options_default(block_per_robot=1, neighbour_candidates=8, neighbour_capacity=16,
@@ -420,15 +440,17 @@ def test_search_rounds_match_the_restatement(ctx, kind, seed):
                 break
         stats = []
         keep = False  # the first round of a launch (round 0), then rounds continuing it
-        for r in range(6):
-            before = ds.state()
-            if before["status"] != 0:
-                break
-            print(f"[search] {kind} seed {seed} round {before['round']} trees {before['tr_n'].tolist()}", flush=True)
-            ds.search(1, keep)
-            keep = True
-            stats.append(_check_round(p, ds, before, ds.state(), st))
-        print(f"[search] {kind}: {stats}", flush=True)
+        with _expand_form(ctx, form):
+            for r in range(6):
+                before = ds.state()
+                if before["status"] != 0:
+                    break
+                print(f"[search] {kind} seed {seed} {form} round {before['round']} trees {before['tr_n'].tolist()}",
+                      flush=True)
+                ds.search(1, keep)
+                keep = True
+                stats.append(_check_round(p, ds, before, ds.state(), st))
+        print(f"[search] {kind} {form}: {stats}", flush=True)
         assert sum(x["edges"] for x in stats) > 0 and sum(x["inserted"] for x in stats) > 0
     finally:
         ds.close()
@@ -529,6 +551,14 @@ def test_rounds_and_simplification_sweep(ctx, kind, seed):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["plain2", "table3", "bin4", "shelf5"])
+def test_rounds_and_simplification_sweep_five_launch_form(ctx, kind):
+    """The same sweep with every expansion in the five-launch form."""
+    with _expand_form(ctx, "split"):
+        _sweep(ctx, scenes.make_problem(kind, 10, _lib.gpu_checker(ctx)))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("kind,seed", [("table3", 5), ("bin4", 6), ("shelf5", 7)])
 def test_multi_goal_rounds_match_the_restatement(ctx, kind, seed):
     """Three goals: the goal tree starts with three roots, and a recorded
@@ -584,8 +614,9 @@ def _sweep(ctx, p):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", ["wave", "split"])
 @pytest.mark.parametrize("kind,seed", [("table3", 2), ("shelf5", 3)])
-def test_optimise_pass_rounds_match_the_restatement(ctx, kind, seed):
+def test_optimise_pass_rounds_match_the_restatement(ctx, kind, seed, form):
     """An AO-x optimise pass (P3 PHS sampling, P4 cost-filtered nearest node,
     P5 bounded roadmap step, P6 rewire): after the first pass solves, the
     next pass starts with the best cost as bound and its rounds are restated
@@ -608,16 +639,17 @@ def test_optimise_pass_rounds_match_the_restatement(ctx, kind, seed):
         assert float(best) > cmin and np.linalg.norm(phs[0][:, 0]) == pytest.approx(float(best) / 2, rel=1e-5)
         stats = []
         keep = False
-        for _ in range(6):
-            before = ds.state()
-            if before["status"] != 0:
-                break
-            if not keep:
-                before["round"] = 0  # a fresh launch starts its round count at 0
-            ds.search(1, keep)
-            keep = True
-            stats.append(_check_round(p, ds, before, ds.state(), st, optimize=True, phs=phs))
-        print(f"[optimise] {kind}: {stats}", flush=True)
+        with _expand_form(ctx, form):
+            for _ in range(6):
+                before = ds.state()
+                if before["status"] != 0:
+                    break
+                if not keep:
+                    before["round"] = 0  # a fresh launch starts its round count at 0
+                ds.search(1, keep)
+                keep = True
+                stats.append(_check_round(p, ds, before, ds.state(), st, optimize=True, phs=phs))
+        print(f"[optimise] {kind} {form}: {stats}", flush=True)
         # samples come from the informed set, so rejections are rare (clamping
         # to the joint box only); rewires must occur somewhere in the set
         assert sum(x["candidates"] for x in stats) > 0
@@ -626,3 +658,23 @@ def test_optimise_pass_rounds_match_the_restatement(ctx, kind, seed):
             assert sum(x["rewired"] for x in stats) > 0
     finally:
         ds.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,seed", [("bin4", 2), ("shelf5", 4)])
+def test_whole_solves_agree_between_expansion_forms(ctx, kind, seed):
+    """A whole solve (first pass, simplification, optimise passes), bounded by
+    rounds rather than time, gives the same path and cost with every
+    expansion in the wave form as in the five-launch form."""
+    p = scenes.make_problem(kind, seed, _lib.gpu_checker(ctx))
+    st = _lib.settings(num_new_configs=256, granularity=32, range=2.0, rrtc_iter=2, optimize_iters=300)
+    op = _lib.solve_options_default(time_budget_s=120.0)
+    sols = {}
+    for form in ("wave", "split"):
+        with _expand_form(ctx, form):
+            print(f"[forms] {kind} seed {seed}: {form}", flush=True)
+            sols[form] = _lib.solve(ctx, p, st, op)
+    a, b = sols["wave"], sols["split"]
+    assert a.solved and b.solved
+    assert a.cost == b.cost and np.array_equal(a.path, b.path), "the forms disagree"
+    assert a.info["rounds"] == b.info["rounds"] and a.info["passes"] == b.info["passes"]
