@@ -151,7 +151,8 @@ def _stats(sols, ttfs, wall, budget_s=None):
 
 
 def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_rates, serve_s, tune=True,
-                   n_budget=0, batch_kc=16, batch_probe=0, n_scene=0, n_scene_plain=8, scene_kinds=SCENE_KINDS):
+                   n_budget=0, batch_kc=16, batch_probe=0, n_scene=0, n_scene_plain=8, scene_kinds=SCENE_KINDS,
+                   extras=True):
     """Plain vs batched planning on the same seeded problems, a budgeted
     batch, and serving."""
     from mrpop import _lib, autotune
@@ -222,8 +223,9 @@ def planning_bench(ctx, checker, n_plain, n_batch, plain_nnc, batch_nnc, serve_r
     out["serving"] = [serving_bench(ctx, checker, rate, serve_s, st_batch, op_kw) for rate in serve_rates]
     if n_scene:
         out["scenes"] = scene_bench(ctx, checker, n_scene, max(1, n_scene_plain), op_kw, kinds=scene_kinds)
-    out["two_arm_plain"] = two_arm_plain_bench(ctx, checker)
-    out["same_scene"] = same_scene_bench()
+    if extras:
+        out["two_arm_plain"] = two_arm_plain_bench(ctx, checker)
+        out["same_scene"] = same_scene_bench()
     return out
 
 
@@ -429,6 +431,8 @@ def main():
                     help="seeded problems per scene of the 4-arm bin / 5-arm shelf-cell lines (0: skip)")
     ap.add_argument("--scene-plain", type=int, default=8, help="of which the plain form runs this many")
     ap.add_argument("--scenes", type=str, default=",".join(SCENE_KINDS), help="scenes of the per-scene lines")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the two-arm plain line and the same-scene fixture lines (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 256 MiB three times and exit: the reference kernel that scales the HBM counters")
     args = ap.parse_args()
@@ -458,7 +462,8 @@ def main():
         print(json.dumps(planning_bench(ctx, checker, args.plain_problems, args.batch_problems, args.plain_nnc,
                                         args.batch_nnc, rates, args.serve_seconds, not args.no_tune,
                                         args.budget_problems, args.batch_candidates, args.batch_probe,
-                                        args.scene_problems, args.scene_plain, args.scenes.split(","))),
+                                        args.scene_problems, args.scene_plain, args.scenes.split(","),
+                                        not args.no_extras)),
               flush=True)
         return
     problem = scenes.make_problem("bin4", 0, checker)
@@ -554,7 +559,8 @@ def main():
         out["planning"] = planning_bench(ctx, checker, args.plain_problems, args.batch_problems, args.plain_nnc,
                                          args.batch_nnc, rates, args.serve_seconds, not args.no_tune,
                                          args.budget_problems, args.batch_candidates, args.batch_probe,
-                                         args.scene_problems, args.scene_plain, args.scenes.split(","))
+                                         args.scene_problems, args.scene_plain, args.scenes.split(","),
+                                         not args.no_extras)
     print(json.dumps(out), flush=True)
 
 

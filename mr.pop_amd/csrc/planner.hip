@@ -671,19 +671,21 @@ __device__ __forceinline__ float edge_angle(const float *v, const float *q, cons
     return acosf(c);
 }
 
-// Enters expansion b's edge (tree t, tree node near, roadmap nodes rid[]) in
-// the round's edge table.  Returns -1 if b is the first expansion of the
-// round with this edge, else that expansion.  An entry belongs to the
-// round whose tag it carries; entries of earlier rounds count as free.  b's
-// own fields are published before its entry: ex_tree and ex_near were
-// stored by this thread, ex_rid by lanes of this wave before the caller's
-// barrier.
-__device__ int claim_edge(const SlotDev &s, unsigned tag, int b, int t, int near, const int *rid) {
+// Enters candidate b's edge (tree ex_tree[b], tree node ex_near[b], roadmap
+// nodes ex_rid[b]) in the round's edge table.  Returns -1 if b is the first
+// candidate of the round with this edge, else that candidate.  An entry
+// belongs to the round whose tag it carries; entries of earlier rounds count
+// as free.  It runs in a launch of its own, after the launches that stored
+// the candidates' fields, so an entry's owner's fields are read with plain
+// loads and no fence (the table entries alone change during the launch).
+__device__ int claim_edge(const SlotDev &s, unsigned tag, int b) {
+    const int A = s.A;
+    const int t = s.ex_tree[b], near = s.ex_near[b];
+    const int *rid = s.ex_rid + (size_t)b * A;
     unsigned h = (unsigned)near * 0x9E3779B1u ^ (unsigned)t * 0x85EBCA77u;
-    for (int a = 0; a < s.A; ++a) h = (h ^ (unsigned)rid[a]) * 0xC2B2AE3Du + 0x27D4EB2Fu;
+    for (int a = 0; a < A; ++a) h = (h ^ (unsigned)rid[a]) * 0xC2B2AE3Du + 0x27D4EB2Fu;
     h ^= h >> 15;
     const unsigned long long mine = ((unsigned long long)tag << 32) | (unsigned)b;
-    __threadfence();  // release b's fields before its entry
     for (int p = 0; p <= s.ex_hmask; ++p) {
         unsigned long long *e = s.ex_hash + ((h + (unsigned)p) & (unsigned)s.ex_hmask);
         unsigned long long cur = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -693,16 +695,45 @@ __device__ int claim_edge(const SlotDev &s, unsigned tag, int b, int t, int near
             cur = prev;
         }
         const int w = (int)(unsigned)cur;
-        __threadfence();  // acquire w's fields
-        bool same = __hip_atomic_load(s.ex_tree + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == t &&
-                    __hip_atomic_load(s.ex_near + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == near;
-        for (int a = 0; a < s.A && same; ++a)
-            same = __hip_atomic_load(s.ex_rid + (size_t)w * s.A + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-                   rid[a];
+        bool same = s.ex_tree[w] == t && s.ex_near[w] == near;
+        for (int a = 0; a < A && same; ++a) same = s.ex_rid[(size_t)w * A + a] == rid[a];
         if (same) return w;
     }
     atomicAdd(&s.events[kEvEdgeTable], 1);  // not reached: the table has more entries than a round has expansions
     return -1;
+}
+
+// The round's edges, a thread per expansion: each candidate's edge is
+// claimed in the edge table; an expansion of this round with the same edge
+// (tree, near, roadmap nodes) has the same verdict, so the edge is checked
+// once and later candidates link to its first one (ex_dup).  The first ones
+// take edge slots, a wave at a time.
+__global__ __launch_bounds__(64) void k_dr_claim(SlotDev *slots, const int *ids) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    if ((int)blockIdx.x * 64 >= s.B || !searching(s)) return;
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x * 64 + lane;
+    const unsigned rng_round = (unsigned)s.ctr[kCtrRng];
+    const bool cand = b < s.B && s.ex_state[b] != 0;
+    int dup = -1;
+    if (cand) dup = claim_edge(s, rng_round + 1u, b);
+    if (cand && dup >= 0) s.ex_dup[b] = dup;
+    const bool first = cand && dup < 0;
+    const unsigned long long m = __ballot(first);
+    int base = 0;
+    if (lane == 0 && m) base = atomicAdd(s.ed_count, __popcll(m));
+    base = __shfl(base, 0);
+    if (!first) return;
+    const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+    const int D = s.D;
+    const float *from = tree_q(s, s.ex_tree[b], s.ex_near[b]);
+    const float *to = s.ex_q + (size_t)b * D;
+    for (int d = 0; d < D; ++d) {
+        s.ed_from[(size_t)slot * D + d] = from[d];
+        s.ed_to[(size_t)slot * D + d] = to[d];
+    }
+    s.ed_b[slot] = b;
+    s.ed_coll[slot] = 0u;  // the checker only sets bits
 }
 
 // One dRRT expansion (pop_planner.cu:350-622 up to the edge check): pick the
@@ -910,11 +941,10 @@ __device__ int expand_exist_wave(SlotDev &s, int t, int lane, ExpandLds &L) {
 }
 
 // (5) The candidate: edge near -> new for the arm-arm check, unless the new
-// node exists and is not cheaper, or is above the best cost.  L.vn, L.nq,
-// L.nrid hold the step's result.
+// node exists and is not cheaper, or is above the best cost.  L.nq and
+// L.nrid hold the step's result.  k_dr_claim then enters its edge.
 __device__ void expand_candidate(SlotDev &s, int b, int t, int near, float dnear, int exist, int lane, ExpandLds &L) {
     const int D = s.D, A = s.A;
-    const unsigned rng_round = (unsigned)s.ctr[kCtrRng];
     const float best = *s.best_cost;
     const float new_cost = s.tr_cost[(size_t)t * s.tr_cap + near] + dnear;  // (pop_planner.cu:541)
     if (exist != 0x7fffffff) {
@@ -930,28 +960,6 @@ __device__ void expand_candidate(SlotDev &s, int b, int t, int near, float dnear
     }
     if (lane < A) s.ex_rid[b * A + lane] = L.nrid[lane];
     if (lane < D) s.ex_q[(size_t)b * D + lane] = L.nq[lane];
-    __syncthreads();
-    // The edge is fixed by (t, near, nrid[]): an expansion of this round with
-    // the same triple has the same edge and so the same verdict, and the
-    // edge is checked once.
-    int dup = -1;
-    if (lane == 0) dup = claim_edge(s, rng_round + 1u, b, t, near, L.nrid);
-    dup = __shfl(dup, 0);
-    if (dup >= 0) {
-        if (lane == 0) s.ex_dup[b] = dup;
-        return;
-    }
-    int slot = 0;
-    if (lane == 0) slot = atomicAdd(s.ed_count, 1);
-    slot = __shfl(slot, 0);
-    if (lane < D) {
-        s.ed_from[(size_t)slot * D + lane] = L.vn[lane];
-        s.ed_to[(size_t)slot * D + lane] = L.nq[lane];
-    }
-    if (lane == 0) {
-        s.ed_b[slot] = b;
-        s.ed_coll[slot] = 0u;  // the checker only sets bits
-    }
 }
 
 // All five stages in one wave per expansion.
@@ -1120,10 +1128,7 @@ __global__ __launch_bounds__(64) void k_dr_cand(SlotDev *slots, const int *ids) 
     float dsq;
     nn_from_key(s, b, near, dsq);
     const int t = s.ex_tree[b];
-    if (lane < D) {
-        L.vn[lane] = tree_q(s, t, near)[lane];
-        L.nq[lane] = s.ex_q[(size_t)b * D + lane];
-    }
+    if (lane < D) L.nq[lane] = s.ex_q[(size_t)b * D + lane];
     if (lane < A) L.nrid[lane] = s.ex_rid[b * A + lane];
     __syncthreads();
     expand_candidate(s, b, t, near, sqrtf(dsq), s.ex_exist[b], lane, L);
@@ -1689,16 +1694,17 @@ void launch_dr_begin(SlotDev *d, const int *ids, int n, hipStream_t stream) {
     MRPOP_L(k_dr_begin, dim3(n), dim3(64), d, ids);
 }
 void launch_dr_expand(SlotDev *d, const int *ids, int n, int max_b, int splits, hipStream_t stream) {
+    const int groups = (max_b + 63) / 64;
     if (splits <= 0) {
         MRPOP_L(k_dr_expand, dim3(max_b, n), dim3(64), d, ids);
-        return;
+    } else {
+        MRPOP_L(k_dr_sample, dim3(max_b, n), dim3(64), d, ids);
+        MRPOP_L(k_dr_scan<false>, dim3(groups * splits, n), dim3(64), d, ids, splits);
+        MRPOP_L(k_dr_step, dim3(max_b, n), dim3(64), d, ids);
+        MRPOP_L(k_dr_scan<true>, dim3(groups * splits, n), dim3(64), d, ids, splits);
+        MRPOP_L(k_dr_cand, dim3(max_b, n), dim3(64), d, ids);
     }
-    const int blocks = (max_b + 63) / 64 * splits;
-    MRPOP_L(k_dr_sample, dim3(max_b, n), dim3(64), d, ids);
-    MRPOP_L(k_dr_scan<false>, dim3(blocks, n), dim3(64), d, ids, splits);
-    MRPOP_L(k_dr_step, dim3(max_b, n), dim3(64), d, ids);
-    MRPOP_L(k_dr_scan<true>, dim3(blocks, n), dim3(64), d, ids, splits);
-    MRPOP_L(k_dr_cand, dim3(max_b, n), dim3(64), d, ids);
+    MRPOP_L(k_dr_claim, dim3(groups, n), dim3(64), d, ids);
 }
 void launch_dr_commit(SlotDev *d, const int *ids, int n, hipStream_t stream) {
     MRPOP_L(k_dr_commit, dim3(1, n), dim3(256), d, ids);
