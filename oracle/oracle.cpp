@@ -31,6 +31,8 @@
 // The per-sample thread re-apportioning of utils.cuh:477-517 is a scheduling
 // detail; for a single configuration (every sample the same) it gives every
 // sample its share, so the per-configuration verdict is the composite below.
+// Along an edge it can leave a sample's fine stages without a thread:
+// orc_check_edges_ref_schedule restates it to measure what that changes.
 
 #include <algorithm>
 #include <cmath>
@@ -197,8 +199,27 @@ struct SampleGeo {
     ArmSpheres arm[kMaxArms];
 };
 
+// Per-sample figures the reference's thread schedule reads (utils.cuh:477-498):
+// fine spheres behind the env-flagged and self-flagged joints, summed over the
+// arms (panda_four.cuh:216-225, 264-272; panda.cuh:704-711, 739-745, one
+// count per flagged joint from the joint-to-sphere-count table), and which of
+// the two fine stages reports a collision.
+struct Detail {
+    int env_spheres = 0, self_spheres = 0;
+    bool env_hit = false, self_hit = false;
+};
+
+// Fine spheres per joint: the joint-to-sphere-count table of the reference
+// (panda_multi_geometry.cuh:362-370, panda.cuh:441-450) is the histogram of
+// the sphere-to-joint table.
+int joint_sphere_count(int j) {
+    int n = 0;
+    for (int i = 0; i < kFineSpheres; ++i) n += kFineJoint[i] == j;
+    return n;
+}
+
 bool composite(const mrpop_scene &scene, int mode, const SampleGeo &g, int bias, double eps,
-               double *min_abs_clear) {
+               double *min_abs_clear, Detail *det = nullptr) {
     const int A = g.n_arms;
     auto note = [&](double c) {
         if (min_abs_clear) *min_abs_clear = std::min(*min_abs_clear, std::fabs(c));
@@ -236,6 +257,12 @@ bool composite(const mrpop_scene &scene, int mode, const SampleGeo &g, int bias,
                         note(t.c);
                         if (t.hit(bias, eps)) selfflag[a][kCoarseJoint[i]] = 1;
                     }
+    if (det)
+        for (int a = 0; a < A; ++a)
+            for (int j = 0; j < kJoints; ++j) {
+                if (envflag[a][j]) det->env_spheres += joint_sphere_count(j);
+                if (selfflag[a][j]) det->self_spheres += joint_sphere_count(j);
+            }
     bool hit = false;
     // fine env: spheres 1..58 on env-flagged joints
     if (mode != kModeDrrt)
@@ -246,6 +273,8 @@ bool composite(const mrpop_scene &scene, int mode, const SampleGeo &g, int bias,
                 note(t.c);
                 hit |= t.hit(bias, eps);
             }
+    const bool env_hit = hit;
+    hit = false;
     // fine self
     for (int a = 0; a < A; ++a)
         for (int r = 0; r < kFineSelfRanges; ++r) {
@@ -279,7 +308,11 @@ bool composite(const mrpop_scene &scene, int mode, const SampleGeo &g, int bias,
                         hit |= t.hit(bias, eps);
                     }
             }
-    return hit;
+    if (det) {
+        det->env_hit = env_hit;
+        det->self_hit = hit;
+    }
+    return env_hit || hit;
 }
 
 void sample_geo(int mode, int n_arms, int arm, const float *bases, const double *q, SampleGeo &g) {
@@ -490,6 +523,26 @@ void orc_check_edges(const mrpop_scene *scene, int mode, int n_arms, int arm, co
     }
 }
 
+// The reference's per-edge thread schedule of the fine stages
+// (utils.cuh:477-517 for FULL and SINGLE, 653-676 for DRRT): a block has 4*G
+// threads; sample i gets (int)((float)n_i / (float)sum(n) * (float)(4*G))
+// of them, n_i being its env (or self) sphere count, and the last sample also
+// gets whatever the floors leave over.  Thread t works on the sample whose
+// running share total first exceeds t, so a sample with a share of 0 has no
+// thread and its fine stage never runs.  Division and products are IEEE
+// float, as the reference is built without fast math (CMakeLists.txt:9).
+void ref_thread_shares(const int *count, int G, int *share) {
+    const int threads = 4 * G;
+    int total = 0, used = 0;
+    for (int i = 0; i < G; ++i) total += count[i];
+    for (int i = 0; i < G; ++i) {
+        // an all-zero stage is never run (its any-flag is clear); 0/0 would be NaN
+        share[i] = total > 0 ? (int)((float)count[i] / (float)total * (float)threads) : 0;
+        used += share[i];
+    }
+    share[G - 1] += threads - used;
+}
+
 // FP32 mirror of the reference checker with its early exits: the CPU
 // baseline.  Samples of an edge are checked in order k = 1..G and the edge
 // stops at its first colliding sample.  n_threads <= 1 runs serially.
@@ -512,6 +565,54 @@ void orc_check_edges_f32(const mrpop_scene *scene, int mode, int n_arms, int arm
         out[e] = hit;
     }
 }
+
+// Edge verdicts (FP64, nominal) under the reference's thread schedule: a
+// sample's fine env (self) stage counts only if its env (self) share of the
+// block's threads is positive (ref_thread_shares).  out_sched: per-edge
+// verdict under the schedule; out_all: every sample checked (the nominal
+// verdict of orc_check_edges); mask_dropped: bit k-1 for the colliding
+// samples whose colliding stages the schedule left without a thread.
+// Any out pointer may be NULL.
+void orc_check_edges_ref_schedule(const mrpop_scene *scene, int mode, int n_arms, int arm, const float *bases, int G,
+                                  int n_edges, const float *from, const float *to, uint8_t *out_sched,
+                                  uint8_t *out_all, uint64_t *mask_dropped) {
+    const int D = dof(mode, n_arms);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int e = 0; e < n_edges; ++e) {
+        Detail det[64];
+        int env_n[64], self_n[64], env_share[64], self_share[64];
+        double q[7 * kMaxArms];
+        bool all = false;
+        for (int k = 1; k <= G; ++k) {
+            for (int d = 0; d < D; ++d) {
+                double f = from[(size_t)e * D + d], t = to[(size_t)e * D + d];
+                q[d] = f + k * ((t - f) / G);
+            }
+            SampleGeo g;
+            sample_geo(mode, n_arms, arm, bases, q, g);
+            det[k - 1] = Detail{};
+            all |= composite(*scene, mode, g, 0, 0.0, nullptr, &det[k - 1]);
+            env_n[k - 1] = det[k - 1].env_spheres;
+            self_n[k - 1] = det[k - 1].self_spheres;
+        }
+        ref_thread_shares(env_n, G, env_share);
+        ref_thread_shares(self_n, G, self_share);
+        bool sched = false;
+        uint64_t dropped = 0;
+        for (int k = 0; k < G; ++k) {
+            const bool env_run = det[k].env_hit && env_share[k] > 0;
+            const bool self_run = det[k].self_hit && self_share[k] > 0;
+            sched |= env_run || self_run;
+            if ((det[k].env_hit || det[k].self_hit) && !env_run && !self_run) dropped |= 1ull << k;
+        }
+        if (out_sched) out_sched[e] = sched;
+        if (out_all) out_all[e] = all;
+        if (mask_dropped) mask_dropped[e] = dropped;
+    }
+}
+
+// The schedule's shares for given per-sample sphere counts (known-answer tests).
+void orc_ref_thread_shares(const int *count, int G, int *share) { ref_thread_shares(count, G, share); }
 
 // Geometry tables as compiled into this library (for the table fixture test).
 void orc_tables(float *joint_tf /*8*12*/, float *coarse /*11*4*/, int *coarse_joint, float *fine /*59*4*/,

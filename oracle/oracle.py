@@ -44,6 +44,9 @@ def lib():
                                       P, P, ctypes.c_double, P, P, P, P, P, P]
         L.orc_check_edges_f32.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, ctypes.c_int,
                                           ctypes.c_int, P, P, P, ctypes.c_int]
+        L.orc_check_edges_ref_schedule.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, ctypes.c_int,
+                                                   ctypes.c_int, P, P, P, P, P]
+        L.orc_ref_thread_shares.argtypes = [P, ctypes.c_int, P]
         L.orc_tables.argtypes = [P] * 11
         _lib = L
     return _lib
@@ -132,6 +135,31 @@ def check_edges_f32(scene, mode, bases, G, frm, to, arm=0, threads=1):
     lib().orc_check_edges_f32(ctypes.byref(cs), mode, len(bases), arm, bases.ctypes.data, G, n, frm.ctypes.data,
                               to.ctypes.data, out.ctypes.data, int(threads))
     return out.astype(bool)
+
+
+def ref_thread_shares(counts):
+    """Threads of the reference's fine stages per sample for per-sample sphere
+    counts (utils.cuh:477-498): int32[G]."""
+    c = np.ascontiguousarray(counts, np.int32)
+    out = np.zeros(len(c), np.int32)
+    lib().orc_ref_thread_shares(c.ctypes.data, len(c), out.ctypes.data)
+    return out
+
+
+def check_edges_ref_schedule(scene, mode, bases, G, frm, to, arm=0):
+    """FP64 nominal edge verdicts with and without the reference's fine-stage
+    thread schedule: dict sched (bool[n]), all (bool[n]), dropped (uint64[n]:
+    colliding samples whose fine stages got no thread)."""
+    bases = _f32(bases).reshape(-1, 12)
+    frm, to = _f32(frm), _f32(to)
+    n = frm.shape[0]
+    cs = scene.as_c()
+    sched, full = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+    dropped = np.zeros(n, np.uint64)
+    lib().orc_check_edges_ref_schedule(ctypes.byref(cs), mode, len(bases), arm, bases.ctypes.data, G, n,
+                                       frm.ctypes.data, to.ctypes.data, sched.ctypes.data, full.ctypes.data,
+                                       dropped.ctypes.data)
+    return {"sched": sched.astype(bool), "all": full.astype(bool), "dropped": dropped}
 
 
 def config_checker(scene, n_arms, bases, cfgs):
