@@ -238,7 +238,10 @@ def same_scene_bench(budget_s=30.0):
     instead of 400 000 so that the 100 queries finish in seconds), without
     roadmap reuse (every query builds its roadmaps), and with roadmap_cache
     in one context: every query starts from the roadmaps the earlier queries
-    of the scene left."""
+    of the scene left.  The batched form then plans the 50 queries of the
+    scene together (fresh roadmaps, all submitted at t = 0); a query's result
+    is the same alone and in the batch, so solve rate and costs match the
+    fresh plain line and the batch changes throughput and TTFS only."""
     from mrpop import _lib, geometry, scenes
     st = _lib.settings(num_new_configs=300, granularity=64, range=2.0, rrtc_iter=3, optimize_iters=1500)
     out = []
@@ -248,23 +251,41 @@ def same_scene_bench(budget_s=30.0):
         res = {"scene": f"{name} fixture scene", "arms": A, "queries": len(probs),
                "settings": {"num_new_configs": 300, "granularity": 64, "range": 2.0, "rrtc_iter": 3,
                             "optimize_iters": 1500, "time_budget_s": budget_s}}
+        fresh = None
         for cache in (0, 1):
             ctx = _lib.Context(0)
             op = _lib.solve_options_default(time_budget_s=budget_s, roadmap_cache=cache)
-            sols, own = [], []
+            sols, own, queued = [], [], []
             t0 = time.perf_counter()
             for p in probs:
+                start = time.perf_counter() - t0
                 s = _lib.solve(ctx, p, st, op)
                 sols.append(s)
                 if s.solved:
                     own.append(s.ttfs_s)
+                    queued.append(start + s.ttfs_s)
             wall = time.perf_counter() - t0
             ctx.close()
             r = _stats(sols, own, wall, budget_s)
+            r.update(_ttfs_pcts(own, "ttfs_own"), **_ttfs_pcts(queued, "ttfs_queued"))
             r["roadmap_rounds_mean"] = float(np.mean([s.info["roadmap_rounds"] for s in sols]))
             r["roadmap_nodes_built_mean"] = float(np.mean([sum(s.info["roadmap_size"][:A]) -
                                                            sum(s.info["roadmap_cached"][:A]) for s in sols]))
             res["roadmap_reuse" if cache else "fresh_roadmaps"] = r
+            if not cache:
+                fresh = sols
+        ctx = _lib.Context(0)
+        op = _lib.solve_options_default(time_budget_s=budget_s)
+        t0 = time.perf_counter()
+        bsols = _lib.solve_batch(ctx, probs, [st] * len(probs), [op] * len(probs))
+        wall = time.perf_counter() - t0
+        ctx.close()
+        bt = [s.ttfs_s for s in bsols if s.solved]
+        b = _stats(bsols, bt, wall, budget_s)
+        b["same_cost_as_fresh_plain"] = sum(a.solved == c.solved and (not a.solved or a.cost == c.cost)
+                                            for a, c in zip(fresh, bsols))
+        res["batched"] = b
+        res["batched_vs_plain_problems_per_s"] = b["problems_per_s"] / res["fresh_roadmaps"]["problems_per_s"]
         print(f"[bench] same scene: {res}", file=sys.stderr, flush=True)
         out.append(res)
     return out

@@ -63,11 +63,32 @@ __device__ __forceinline__ float *tree_q(const SlotDev &s, int t, int i) {
 // lane and load.  The chunk never reaches past row base + c - 1 < n <= the
 // tree's capacity.  Lane i keeps handling rows i, i + 64, ..., as a
 // per-lane strided scan does, so the results (and ties) do not change.
+// Staged rows lie S >= D floats apart.  Where lane i reads row i, S is odd
+// (wave_stride): with an even stride (D = 14 or 28) the lanes' reads of one
+// column would fall on 2 or 4 times fewer LDS banks.  Where every lane reads
+// the same row (a broadcast), S = D.  D | 1 <= 35 for D <= 35, so a chunk
+// fits the kScanRows * 35 floats of each stage buffer.
 constexpr int kScanRows = 64;
-__device__ __forceinline__ void stage_rows(const float *rows, int base, int c, int D, float *stage, int lane) {
+__device__ __forceinline__ int wave_stride(int D) { return D | 1; }
+__device__ __forceinline__ void stage_rows(const float *rows, int base, int c, int D, int S, float *stage, int lane) {
     const float *src = rows + (size_t)base * D;
     const int nf = c * D;
-    for (int x = lane; x < nf; x += 64) stage[x] = src[x];
+    if (S == D) {
+        for (int x = lane; x < nf; x += 64) stage[x] = src[x];
+        return;
+    }
+    // element x = r * D + col goes to r * S + col; x advances by 64 = dr * D + dc
+    const int dr = 64 / D, dc = 64 - dr * D;
+    int r = lane / D, col = lane - r * D;
+    for (int x = lane; x < nf; x += 64) {
+        stage[r * S + col] = src[x];
+        r += dr;
+        col += dc;
+        if (col >= D) {
+            col -= D;
+            ++r;
+        }
+    }
 }
 
 // A dRRT round runs for a slot while its pass is unsolved and the launch has
@@ -844,7 +865,7 @@ __device__ bool expand_sample(SlotDev &s, int b, int lane, ExpandLds &L, int &t_
 // lane i scans rows i, i + 64, ...; the least squared distance, lowest index
 // among equals.
 __device__ void expand_nn_wave(SlotDev &s, int t, float c_rand, int lane, ExpandLds &L, int &near, float &dsq) {
-    const int D = s.D;
+    const int D = s.D, S = wave_stride(D);
     const int nt = s.tr_n[t];
     const float *rows = tree_q(s, t, 0);
     float bd = FLT_MAX;
@@ -852,11 +873,11 @@ __device__ void expand_nn_wave(SlotDev &s, int t, float c_rand, int lane, Expand
     for (int base = 0; base < nt; base += kScanRows) {
         const int c = min(kScanRows, nt - base);
         __syncthreads();  // the previous chunk has been read
-        stage_rows(rows, base, c, D, L.stage, lane);
+        stage_rows(rows, base, c, D, S, L.stage, lane);
         __syncthreads();
         const int i = base + lane;
         if (lane < c && !(s.optimize && !(s.tr_cost[(size_t)t * s.tr_cap + i] < c_rand))) {
-            const float d = sqdist(L.stage + lane * D, L.q, D);
+            const float d = sqdist(L.stage + lane * S, L.q, D);
             if (d < bd) {
                 bd = d;
                 bi = i;
@@ -925,16 +946,16 @@ __device__ bool expand_step(SlotDev &s, int t, int near, float c_rand, int lane,
 // L.nq (pop_planner.cu:526-533), by the wave; 0x7fffffff if none.  Once a
 // chunk holds one, later chunks hold only higher indices.
 __device__ int expand_exist_wave(SlotDev &s, int t, int lane, ExpandLds &L) {
-    const int D = s.D;
+    const int D = s.D, S = wave_stride(D);
     const int nt = s.tr_n[t];
     const float *rows = tree_q(s, t, 0);
     int exist = 0x7fffffff;
     for (int base = 0; base < nt; base += kScanRows) {
         const int c = min(kScanRows, nt - base);
         __syncthreads();
-        stage_rows(rows, base, c, D, L.stage, lane);
+        stage_rows(rows, base, c, D, S, L.stage, lane);
         __syncthreads();
-        if (lane < c && sqdist(L.stage + lane * D, L.nq, D) <= 1e-5f) exist = base + lane;
+        if (lane < c && sqdist(L.stage + lane * S, L.nq, D) <= 1e-5f) exist = base + lane;
         if (__ballot(exist != 0x7fffffff)) break;
     }
     return wave_min_int(exist);
@@ -1027,7 +1048,7 @@ __device__ void scan_block(SlotDev &s, int b, int split, int splits, float *stag
         for (int base = lo; base < hi; base += kScanRows) {
             const int c = min(kScanRows, hi - base);
             __syncthreads();  // the previous chunk has been read
-            stage_rows(rows, base, c, D, stage, lane);
+            stage_rows(rows, base, c, D, D, stage, lane);
             if (!EXIST && s.optimize && lane < c) cstage[lane] = cost[base + lane];
             __syncthreads();
             if (t_me == t && !(EXIST && bi != 0x7fffffff)) {
@@ -1238,7 +1259,7 @@ __global__ __launch_bounds__(64) void k_dr_connect(SlotDev *slots, const int *id
     if (lane == 0) s.cn_near[b] = -1;
     const int idx = s.ex_index[b];
     if (s.ex_state[b] != 1 || idx < 0) return;
-    const int t = s.ex_tree[b], o = 1 - t, D = s.D;
+    const int t = s.ex_tree[b], o = 1 - t, D = s.D, S = wave_stride(D);
     __shared__ float q[35];
     __shared__ float stage[kScanRows * 35];
     if (lane < D) q[lane] = tree_q(s, t, idx)[lane];
@@ -1252,10 +1273,10 @@ __global__ __launch_bounds__(64) void k_dr_connect(SlotDev *slots, const int *id
     for (int base = 0; base < no; base += kScanRows) {
         const int c = min(kScanRows, no - base);
         __syncthreads();
-        stage_rows(rows, base, c, D, stage, lane);
+        stage_rows(rows, base, c, D, S, stage, lane);
         __syncthreads();
         if (lane < c) {
-            const float d = sqdist(stage + lane * D, q, D);
+            const float d = sqdist(stage + lane * S, q, D);
             if (d < bd && (!s.optimize || sqrtf(d) < ub)) {
                 bd = d;
                 bi = base + lane;
