@@ -189,8 +189,10 @@ class Engine {
         std::vector<Graph> graphs;
         uint64_t graph_clock = 0;
         bool abandoned = false;  // its work outlived the deadline; its problems were ended
+        std::chrono::steady_clock::time_point t_launch;  // when its running step was enqueued
     };
     int wait_lane(Lane &L);
+    int wait_any(Lane *const *cands, int n, Lane **done);
     void abandon_lane(Lane &L);
     void drop_graphs(Lane &L);
     int ensure_capacity(int n);
@@ -213,7 +215,17 @@ class Engine {
     std::vector<Slot *> slots_;
     std::vector<std::pair<int, void *>> graveyard_;  // (lane, block) of slots released while their lane was stuck
     int cap_ = 0;
-    Lane lanes_[2];  // 0: roadmap rounds, 1: dRRT rounds and simplification
+    // Lane 0 runs roadmap rounds; lanes 1..kSearchLanes run dRRT rounds and
+    // simplification, a slot on lane 1 + id % kSearchLanes.  A round is a
+    // chain of small dependent launches that leave most of the GPU idle, so
+    // the search lanes' chains run side by side on their own streams.  Two
+    // search lanes: the 64-problem mixed batch plans 18.5 -> 21.3 problems/s,
+    // the reference's 50-query fixture batches 9.3 -> 11.3 (4 arms) and 9.2 ->
+    // 11.2 (5 arms); three ran slower than two (17.6 / 9.5 / 9.1: with the
+    // context and roadmap streams, five streams over the process's four
+    // hardware queues).
+    static constexpr int kSearchLanes = 2;
+    Lane lanes_[1 + kSearchLanes];
     dev::WorldDesc *d_worlds_ = nullptr, *h_worlds_ = nullptr;
     SlotDev *d_aux_slot_ = nullptr, *h_aux_slot_ = nullptr;  // one-slot table for set-up kernels
     int *d_aux_id_ = nullptr;
@@ -222,7 +234,7 @@ class Engine {
     // search-lane steps while first solutions are pending; optimise passes
     // run on one of every kOptimiseShare of them
     static constexpr uint64_t kOptimiseShare = 4;
-    uint64_t search_steps_ = 0;
+    uint64_t search_steps_[kSearchLanes] = {};
     // released slot blocks by size, reused by later problems (serving)
     static constexpr size_t kPoolBytes = size_t(16) << 30;
     std::multimap<size_t, void *> pool_;

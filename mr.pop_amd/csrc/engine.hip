@@ -964,6 +964,7 @@ int Engine::launch_lane(Lane &L) {
     MRPOP_HIP(c_, hipGetLastError());
     MRPOP_HIP(c_, hipEventRecord(L.done, st));
     L.busy = true;
+    L.t_launch = std::chrono::steady_clock::now();
     return MRPOP_OK;
 }
 
@@ -1080,6 +1081,29 @@ int Engine::wait_lane(Lane &L) {
     }
 }
 
+// Wait until one of the busy lanes `cands` completes its step (*done = that
+// lane, MRPOP_OK) or one has run past the context's lane deadline since its
+// launch (*done = that lane, MRPOP_ERR_TIMEOUT).  Polls the events.
+int Engine::wait_any(Lane *const *cands, int n, Lane **done) {
+    for (unsigned spin = 0;; ++spin) {
+        const auto now = std::chrono::steady_clock::now();
+        for (int i = 0; i < n; ++i) {
+            Lane &L = *cands[i];
+            const hipError_t e = hipEventQuery(L.done);
+            if (e == hipSuccess) {
+                *done = &L;
+                return MRPOP_OK;
+            }
+            if (e != hipErrorNotReady) return c_->fail(MRPOP_ERR_HIP, "lane step: %s", hipGetErrorString(e));
+            if (std::chrono::duration<double>(now - L.t_launch).count() > c_->lane_deadline_s) {
+                *done = &L;
+                return MRPOP_ERR_TIMEOUT;
+            }
+        }
+        if (spin > 64) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
 // A step outlived the deadline: its problems end with MRPOP_ERR_TIMEOUT.
 // Their memory stays with the lane until its work completes (release()
 // waits for the lane's stream), and the lane takes no new work until then.
@@ -1122,13 +1146,13 @@ int Engine::step() {
             s->res.kernel_ns = ns_since(s->t_kernel_start);
         }
     }
-    Lane &R = lanes_[0], &S = lanes_[1];
+    Lane &R = lanes_[0];
     // a lane given up on rejoins once its work has completed (its problems
     // already ended)
-    for (Lane *L : {&R, &S})
-        if (L->abandoned && hipEventQuery(L->done) == hipSuccess) {
-            L->abandoned = false;
-            const int li = (int)(L - lanes_);
+    for (Lane &L : lanes_)
+        if (L.abandoned && hipEventQuery(L.done) == hipSuccess) {
+            L.abandoned = false;
+            const int li = (int)(&L - lanes_);
             for (auto g = graveyard_.begin(); g != graveyard_.end();)
                 if (g->first == li) {
                     (void)hipFree(g->second);
@@ -1136,7 +1160,7 @@ int Engine::step() {
                 } else {
                     ++g;
                 }
-            if (int rc = process_lane(*L)) return rc;
+            if (int rc = process_lane(L)) return rc;
         }
     if (!R.busy) {
         // slots still looking for their first solution go first; the roadmap
@@ -1148,37 +1172,47 @@ int Engine::step() {
                 if (s->lane < 0 && s->phase == Phase::Roadmap) R.rm.push_back(s);
         if (int rc = launch_lane(R)) return rc;
     }
-    if (!S.busy) {
-        // Time to first solution first: while some problem is still searching
-        // for its first path, the optimise passes of solved problems join the
-        // search launch on one step in kOptimiseShare, so a hard problem's
-        // rounds are not slowed by every other problem's optimisation.
-        bool seekers = false;
-        for (Slot *s : slots_) seekers |= s->lane < 0 && s->phase == Phase::Search && s->seeking;
-        const bool optimisers = !seekers || (search_steps_++ % kOptimiseShare) == 0;
+    // Time to first solution first: while some problem is still searching
+    // for its first path, the optimise passes of solved problems join a
+    // search lane's launch on one step in kOptimiseShare, so a hard problem's
+    // rounds are not slowed by every other problem's optimisation.
+    bool seekers = false;
+    for (Slot *s : slots_) seekers |= s->phase == Phase::Search && s->seeking;
+    for (int k = 0; k < kSearchLanes; ++k) {
+        Lane &S = lanes_[1 + k];
+        if (S.busy) continue;
+        const bool optimisers = !seekers || (search_steps_[k]++ % kOptimiseShare) == 0;
         for (Slot *s : slots_) {
-            if (s->lane >= 0) continue;
+            if (s->lane >= 0 || s->id % kSearchLanes != k) continue;
             if (s->phase == Phase::Search && (s->seeking || optimisers)) S.dr.push_back(s);
             else if (s->phase == Phase::Simplify) S.sp.push_back(s);
         }
         if (int rc = launch_lane(S)) return rc;
     }
-    // wait for the search lane (short steps) when it runs, else for the roadmap lane
-    Lane *first = S.busy && !S.abandoned ? &S : (R.busy && !R.abandoned ? &R : nullptr);
-    if (!first) {
-        if (R.abandoned || S.abandoned) std::this_thread::sleep_for(std::chrono::microseconds(100));
+    // wait for a search lane (short steps) when one runs, else for the
+    // roadmap lane; lanes that have completed meanwhile are processed too
+    Lane *cands[1 + kSearchLanes];
+    int n = 0;
+    for (int k = 1; k <= kSearchLanes; ++k)
+        if (lanes_[k].busy && !lanes_[k].abandoned) cands[n++] = &lanes_[k];
+    if (!n && R.busy && !R.abandoned) cands[n++] = &R;
+    if (!n) {
+        bool abandoned = false;
+        for (Lane &L : lanes_) abandoned |= L.abandoned;
+        if (abandoned) std::this_thread::sleep_for(std::chrono::microseconds(100));
         return MRPOP_OK;
     }
-    const int w = wait_lane(*first);
+    Lane *first = nullptr;
+    const int w = wait_any(cands, n, &first);
     if (w == MRPOP_ERR_TIMEOUT) {
         abandon_lane(*first);  // its problems end; the others go on
         return MRPOP_OK;
     }
     if (w) return w;
     if (int rc = process_lane(*first)) return rc;
-    Lane &other = first == &S ? R : S;
-    if (other.busy && !other.abandoned && hipEventQuery(other.done) == hipSuccess)
-        if (int rc = process_lane(other)) return rc;
+    for (Lane &L : lanes_)
+        if (&L != first && L.busy && !L.abandoned && hipEventQuery(L.done) == hipSuccess)
+            if (int rc = process_lane(L)) return rc;
     return MRPOP_OK;
 }
 
