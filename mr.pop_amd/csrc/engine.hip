@@ -182,7 +182,7 @@ size_t Engine::slot_bytes(int A, const mrpop_settings &st, const SolveOptions &o
     L.add(sizeof(float) * (size_t)B * (D + 1));                             // ex_sample, ex_crand
     L.add(sizeof(unsigned long long) * B + sizeof(int) * B);                // ex_nnkey, ex_live
     L.add(sizeof(float) * (size_t)B * D * 2 + sizeof(int) * B * 2);         // ed_from/to, ed_b, ed_coll
-    L.add(sizeof(int) * B * 3 + sizeof(float) * B);                         // cn_*
+    L.add(sizeof(unsigned long long) * B + sizeof(int) * B * 3 + sizeof(float) * B);  // cn_key, cn_*
     L.add(sizeof(float) * ce_cap * D * 2);                                  // ce_from/to
     L.add(sizeof(int) * ce_cap * 2);                                        // ce_b, ce_coll
     L.add(sizeof(float) * (size_t)kPathCap * D);                            // path
@@ -498,7 +498,8 @@ int Engine::add(const mrpop_scene &scene, int A, const float *bases, const float
     v.ed_to = ed + (size_t)B * D;
     v.ed_b = (int *)(ed + (size_t)B * D * 2);
     v.ed_coll = (uint32_t *)(v.ed_b + B);
-    int *cn = (int *)take(sizeof(int) * B * 3 + sizeof(float) * B);
+    v.cn_key = (unsigned long long *)take(sizeof(unsigned long long) * B + sizeof(int) * B * 3 + sizeof(float) * B);
+    int *cn = (int *)(v.cn_key + B);
     v.cn_near = cn;
     v.cn_next = cn + B;
     v.cn_fail = (uint32_t *)(cn + 2 * B);
@@ -845,10 +846,10 @@ int Engine::launch_lane(Lane &L) {
             reset |= s->dv.round_reset != 0;
             s->dv.round_reset = 0;  // applied by this step's k_dr_begin
         }
-        // Large trees: the tree scans run a thread per expansion, each block
-        // over a slice of the trees, with about 2048 blocks in all and slices
-        // of at least expand_slice_rows rows.  The slice count only changes
-        // the speed.
+        // Large trees: the tree scans of the expansions and connections run
+        // a thread per expansion, each block over a slice of the trees, with
+        // about 2048 blocks in all and slices of at least expand_slice_rows
+        // rows.  The slice count only changes the speed.
         int splits = 0;
         if (c_->expand_split_nodes >= 0 && tree_n >= c_->expand_split_nodes) {
             const int groups = (max_b + 63) / 64 * n;
@@ -870,7 +871,7 @@ int Engine::launch_lane(Lane &L) {
                 plan::launch_dr_expand(ds, d, n, max_b, splits, st);
                 if (int rc = launch_staged(L, s1)) return rc;
                 plan::launch_dr_commit(ds, d, n, st);
-                plan::launch_dr_connect(ds, d, n, max_b, st);
+                plan::launch_dr_connect(ds, d, n, max_b, splits, st);
                 if (int rc = launch_staged(L, s2)) return rc;
                 plan::launch_dr_finish(ds, d, n, st);
             }

@@ -135,6 +135,7 @@ struct SlotDev {
     int *ed_b;                // [B]
     uint32_t *ed_coll;        // [B]
     // connection
+    unsigned long long *cn_key;  // [B] split connection scan: (squared distance bits << 32 | index) of the nearest node
     int *cn_near;             // [B]
     int *cn_next;             // [B]
     float *cn_d;              // [B]
@@ -213,7 +214,9 @@ void launch_dr_begin(SlotDev *d, const int *ids, int n, hipStream_t stream);
 // expansion.  Both give the same results.
 void launch_dr_expand(SlotDev *d, const int *ids, int n, int max_b, int splits, hipStream_t stream);
 void launch_dr_commit(SlotDev *d, const int *ids, int n, hipStream_t stream);
-void launch_dr_connect(SlotDev *d, const int *ids, int n, int max_b, hipStream_t stream);
+// splits > 0: the nearest-node scans run a thread per expansion over
+// `splits` slices of the other tree first (as the five-launch expansion)
+void launch_dr_connect(SlotDev *d, const int *ids, int n, int max_b, int splits, hipStream_t stream);
 void launch_dr_finish(SlotDev *d, const int *ids, int n, hipStream_t stream);
 void launch_sp_edges(SlotDev *d, const int *ids, int n, int stage, int bspline_step, float range, float min_change,
                      hipStream_t stream);

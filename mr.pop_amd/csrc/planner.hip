@@ -1200,6 +1200,7 @@ __global__ __launch_bounds__(256) void k_dr_commit(SlotDev *slots, const int *id
     __syncthreads();
     for (int b = tid; b < s.B; b += 256) {
         if (s.ex_state[b] == 2) s.rw_key[(size_t)s.ex_tree[b] * s.tr_cap + s.ex_exist[b]] = ~0ull;
+        s.cn_key[b] = ~0ull;  // the split connection scan merges into it
     }
     // insert: prefix over b per tree
     __shared__ int sh0[256], sh1[256];
@@ -1248,10 +1249,85 @@ __global__ __launch_bounds__(256) void k_dr_commit(SlotDev *slots, const int *id
     }
 }
 
+// Thread-per-expansion form of the connection scan below, for large trees:
+// a block holds 64 expansions of one slot and one of `splits` contiguous
+// slices of the trees; rows are staged a chunk at a time and every thread
+// compares them with its own inserted node in index order, keeping the
+// least squared distance that passes the cost bound (lowest index among
+// equals).  Slices merge by atomicMin on (distance bits, index), so the
+// result is the wave scan's.  A partial sum that already reaches the best
+// distance cannot pass (the sum only grows), so its row is left early.
+template <int D>
+__device__ void connect_scan_block(SlotDev &s, int b, int split, int splits, float *stage) {
+    const int lane = threadIdx.x;
+    const int idx = b < s.B ? s.ex_index[b] : -1;
+    const bool live = b < s.B && s.ex_state[b] == 1 && idx >= 0;
+    const int t = live ? s.ex_tree[b] : 0;
+    const int o_me = live ? 1 - t : -1;
+    float q[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = live ? tree_q(s, t, idx)[d] : 0.f;
+    const float ub = live ? *s.best_cost - s.tr_cost[(size_t)t * s.tr_cap + idx] : 0.f;
+    float bd = FLT_MAX;
+    int bi = 0x7fffffff;
+    for (int o = 0; o < 2; ++o) {
+        if (!__syncthreads_or(o_me == o)) continue;
+        const int no = s.tr_n[o];
+        const int per = ((no + kScanRows - 1) / kScanRows + splits - 1) / splits * kScanRows;
+        const int lo = split * per, hi = min(no, lo + per);
+        const float *rows = tree_q(s, o, 0);
+        for (int base = lo; base < hi; base += kScanRows) {
+            const int c = min(kScanRows, hi - base);
+            __syncthreads();  // the previous chunk has been read
+            stage_rows(rows, base, c, D, D, stage, lane);
+            __syncthreads();
+            if (o_me != o) continue;
+            for (int r = 0; r < c; ++r) {
+                const float *row = stage + r * D;
+                float acc = 0.f;
+                bool out = false;
+#pragma unroll
+                for (int a = 0; a < D / 7; ++a) {
+                    if (!out) {
+#pragma unroll
+                        for (int k = 0; k < 7; ++k) {
+                            const float x = row[7 * a + k] - q[7 * a + k];
+                            acc = __builtin_fmaf(x, x, acc);
+                        }
+                        if (a + 1 < D / 7) out = acc >= bd;
+                    }
+                }
+                if (!out && acc < bd && (!s.optimize || sqrtf(acc) < ub)) {
+                    bd = acc;
+                    bi = base + r;
+                }
+            }
+        }
+    }
+    if (live && bi != 0x7fffffff)
+        atomicMin(s.cn_key + b, ((unsigned long long)__float_as_uint(bd) << 32) | (unsigned)bi);
+}
+
+__global__ __launch_bounds__(64) void k_cn_scan(SlotDev *slots, const int *ids, int splits) {
+    SlotDev &s = slots[ids[blockIdx.y]];
+    const int group = blockIdx.x / splits, split = blockIdx.x % splits;
+    const int b0 = group * 64;
+    if (b0 >= s.B || !searching(s)) return;
+    __shared__ float stage[kScanRows * 35];
+    const int b = b0 + threadIdx.x;
+    switch (s.D) {
+    case 14: connect_scan_block<14>(s, b, split, splits, stage); break;
+    case 21: connect_scan_block<21>(s, b, split, splits, stage); break;
+    case 28: connect_scan_block<28>(s, b, split, splits, stage); break;
+    default: connect_scan_block<35>(s, b, split, splits, stage); break;
+    }
+}
+
 // Nearest node of the other tree for every inserted node, and the straight
 // connection split into n_ext = floor(d / range) + 1 chunks of G samples
-// (pop_planner.cu:624-703).  One wave per expansion.
-__global__ __launch_bounds__(64) void k_dr_connect(SlotDev *slots, const int *ids) {
+// (pop_planner.cu:624-703).  One wave per expansion; keyed: the nearest node
+// comes from k_cn_scan (cn_key) instead of the wave's own scan.
+__global__ __launch_bounds__(64) void k_dr_connect(SlotDev *slots, const int *ids, int keyed) {
     SlotDev &s = slots[ids[blockIdx.y]];
     const int b = blockIdx.x;
     if (b >= s.B || !searching(s)) return;
@@ -1264,26 +1340,34 @@ __global__ __launch_bounds__(64) void k_dr_connect(SlotDev *slots, const int *id
     __shared__ float stage[kScanRows * 35];
     if (lane < D) q[lane] = tree_q(s, t, idx)[lane];
     __syncthreads();
-    const float best = *s.best_cost;
-    const float ub = best - s.tr_cost[(size_t)t * s.tr_cap + idx];
-    const int no = s.tr_n[o];
-    const float *rows = tree_q(s, o, 0);
     float bd = FLT_MAX;
     int bi = 0x7fffffff;
-    for (int base = 0; base < no; base += kScanRows) {
-        const int c = min(kScanRows, no - base);
-        __syncthreads();
-        stage_rows(rows, base, c, D, S, stage, lane);
-        __syncthreads();
-        if (lane < c) {
-            const float d = sqdist(stage + lane * S, q, D);
-            if (d < bd && (!s.optimize || sqrtf(d) < ub)) {
-                bd = d;
-                bi = base + lane;
+    if (keyed) {
+        const unsigned long long k = s.cn_key[b];
+        if (k != ~0ull) {
+            bd = __uint_as_float((unsigned)(k >> 32));
+            bi = (int)(unsigned)k;
+        }
+    } else {
+        const float best = *s.best_cost;
+        const float ub = best - s.tr_cost[(size_t)t * s.tr_cap + idx];
+        const int no = s.tr_n[o];
+        const float *rows = tree_q(s, o, 0);
+        for (int base = 0; base < no; base += kScanRows) {
+            const int c = min(kScanRows, no - base);
+            __syncthreads();
+            stage_rows(rows, base, c, D, S, stage, lane);
+            __syncthreads();
+            if (lane < c) {
+                const float d = sqdist(stage + lane * S, q, D);
+                if (d < bd && (!s.optimize || sqrtf(d) < ub)) {
+                    bd = d;
+                    bi = base + lane;
+                }
             }
         }
+        wave_argmin(bd, bi);
     }
-    wave_argmin(bd, bi);
     if (bi == 0x7fffffff) return;
     const float d = sqrtf(bd);
     const int next = (int)(d / s.range) + 1;
@@ -1730,8 +1814,12 @@ void launch_dr_expand(SlotDev *d, const int *ids, int n, int max_b, int splits, 
 void launch_dr_commit(SlotDev *d, const int *ids, int n, hipStream_t stream) {
     MRPOP_L(k_dr_commit, dim3(1, n), dim3(256), d, ids);
 }
-void launch_dr_connect(SlotDev *d, const int *ids, int n, int max_b, hipStream_t stream) {
-    MRPOP_L(k_dr_connect, dim3(max_b, n), dim3(64), d, ids);
+void launch_dr_connect(SlotDev *d, const int *ids, int n, int max_b, int splits, hipStream_t stream) {
+    if (splits > 0) {
+        const int groups = (max_b + 63) / 64;
+        MRPOP_L(k_cn_scan, dim3(groups * splits, n), dim3(64), d, ids, splits);
+    }
+    MRPOP_L(k_dr_connect, dim3(max_b, n), dim3(64), d, ids, splits > 0 ? 1 : 0);
 }
 void launch_dr_finish(SlotDev *d, const int *ids, int n, hipStream_t stream) {
     MRPOP_L(k_dr_finish, dim3(1, n), dim3(256), d, ids);
