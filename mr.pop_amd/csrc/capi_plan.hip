@@ -458,9 +458,10 @@ extern "C" int mrpop_diag_set_lane_deadline(mrpop_ctx *c, double seconds) {
 }
 
 // Diagnostics (not part of the public ABI): the tree size from which dRRT
-// expansions run in the five-launch form (0: always, < 0: never) and the
-// least tree rows a block of its scans takes.  Both forms give the same
-// results; tests compare them.
+// expansions run in the five-launch form and connections scan the other
+// tree a thread per expansion (0: always, < 0: never), and the least tree
+// rows a block of those scans takes.  Both forms give the same results;
+// tests compare them.
 extern "C" int mrpop_diag_set_expand_split(mrpop_ctx *c, int min_nodes, int slice_rows) {
     if (!c || slice_rows < 1) return MRPOP_ERR_INVALID;
     ((Context *)c)->expand_split_nodes = min_nodes;
