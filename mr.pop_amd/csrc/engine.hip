@@ -1177,14 +1177,26 @@ int Engine::step() {
     // for its first path, the optimise passes of solved problems join a
     // search lane's launch on one step in kOptimiseShare, so a hard problem's
     // rounds are not slowed by every other problem's optimisation.
-    bool seekers = false;
-    for (Slot *s : slots_) seekers |= s->phase == Phase::Search && s->seeking;
+    // When fewer problems seek than optimise or simplify, the seekers get
+    // search lane 1 to themselves and the others run every step on the last
+    // lane, so the rounds of the hard problems that decide a batch's end are
+    // not held back by the others' work; otherwise the lanes split the
+    // problems by slot id.
+    int n_seek = 0, n_rest = 0;
+    for (Slot *s : slots_) {
+        if (s->phase == Phase::Search && s->seeking) ++n_seek;
+        else if (s->phase == Phase::Search || s->phase == Phase::Simplify) ++n_rest;
+    }
+    const bool by_role = n_seek > 0 && n_seek < n_rest;
     for (int k = 0; k < kSearchLanes; ++k) {
         Lane &S = lanes_[1 + k];
         if (S.busy) continue;
-        const bool optimisers = !seekers || (search_steps_[k]++ % kOptimiseShare) == 0;
+        const bool optimisers = by_role || n_seek == 0 || (search_steps_[k]++ % kOptimiseShare) == 0;
         for (Slot *s : slots_) {
-            if (s->lane >= 0 || s->id % kSearchLanes != k) continue;
+            if (s->lane >= 0) continue;
+            const bool seeker = s->phase == Phase::Search && s->seeking;
+            const int lane = by_role ? (seeker ? 0 : kSearchLanes - 1) : s->id % kSearchLanes;
+            if (lane != k) continue;
             if (s->phase == Phase::Search && (s->seeking || optimisers)) S.dr.push_back(s);
             else if (s->phase == Phase::Simplify) S.sp.push_back(s);
         }
