@@ -216,9 +216,11 @@ class Engine {
     std::vector<std::pair<int, void *>> graveyard_;  // (lane, block) of slots released while their lane was stuck
     int cap_ = 0;
     // Lane 0 runs roadmap rounds; lanes 1..kSearchLanes run dRRT rounds and
-    // simplification, a slot on lane 1 + id % kSearchLanes.  A round is a
-    // chain of small dependent launches that leave most of the GPU idle, so
-    // the search lanes' chains run side by side on their own streams.  Two
+    // simplification, a slot on lane 1 + id % kSearchLanes (or, while fewer
+    // problems seek a first solution than optimise, seekers on lane 1 and the
+    // rest on the last lane; Engine::step).  A round is a chain of small
+    // dependent launches that leave most of the GPU idle, so the search
+    // lanes' chains run side by side on their own streams.  Two
     // search lanes: the 64-problem mixed batch plans 18.5 -> 21.3 problems/s,
     // the reference's 50-query fixture batches 9.3 -> 11.3 (4 arms) and 9.2 ->
     // 11.2 (5 arms); three ran slower than two (17.6 / 9.5 / 9.1: with the
