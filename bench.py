@@ -291,6 +291,24 @@ def same_scene_bench(budget_s=30.0):
     return out
 
 
+def fixture_batch(ctx, name):
+    """The 50 queries of one reference fixture scene in one batch at the
+    harness settings (nnc 300, G 64, range 2.0, rrtc_iter 3, optimize_iters
+    1500), after a batch of its first four queries (first launches)."""
+    from mrpop import _lib, geometry, scenes
+    A = 4 if name == "panda_four" else 5
+    probs = scenes.load_problem_file(REPO / "tests" / "golden" / f"{name}_problems.json", geometry.default_bases(A))
+    st = _lib.settings(num_new_configs=300, granularity=64, range=2.0, rrtc_iter=3, optimize_iters=1500)
+    op = _lib.solve_options_default(time_budget_s=30.0)
+    _lib.solve_batch(ctx, probs[:4], [st] * 4, [op] * 4)
+    t0 = time.perf_counter()
+    sols = _lib.solve_batch(ctx, probs, [st] * len(probs), [op] * len(probs))
+    wall = time.perf_counter() - t0
+    out = _stats(sols, [s.ttfs_s for s in sols if s.solved], wall, 30.0)
+    out.update(scene=f"{name} fixture scene", rounds=int(sum(s.info["rounds"] for s in sols)))
+    return out
+
+
 def _ttfs_pcts(ttfs, prefix):
     return {f"{prefix}_p50_s": float(np.percentile(ttfs, 50)) if len(ttfs) else None,
             f"{prefix}_p99_s": float(np.percentile(ttfs, 99)) if len(ttfs) else None}
@@ -456,6 +474,9 @@ def main():
                     help="skip the two-arm plain line and the same-scene fixture lines (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 256 MiB three times and exit: the reference kernel that scales the HBM counters")
+    ap.add_argument("--fixture-batch", choices=("panda_four", "panda_five"),
+                    help="plan the 50 queries of one reference fixture scene in one batch at the harness settings "
+                         "and exit (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: problems are independent and this build runs on a single MI355X; "
@@ -477,6 +498,9 @@ def main():
         print(json.dumps({"copy_bytes": n * 4, "copies": 3}), flush=True)
         return
     ctx = _lib.Context(0)
+    if args.fixture_batch:
+        print(json.dumps(fixture_batch(ctx, args.fixture_batch)), flush=True)
+        return
     checker = _lib.gpu_checker(ctx)
     if args.plan_only:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
