@@ -52,6 +52,20 @@ __device__ __forceinline__ float sqdist(const float *a, const float *b, int D) {
     return s;
 }
 
+// Copies a row of D <= 35 floats with every load issued before the first
+// store: a plain copy loop waits for each element's load before its store
+// (the compiler cannot tell that the rows do not overlap), one memory round
+// trip per element.
+__device__ __forceinline__ void copy_row(float *dst, const float *src, int D) {
+    float v[35];
+#pragma unroll
+    for (int d = 0; d < 35; ++d)
+        if (d < D) v[d] = src[d];
+#pragma unroll
+    for (int d = 0; d < 35; ++d)
+        if (d < D) dst[d] = v[d];
+}
+
 __device__ __forceinline__ float *tree_q(const SlotDev &s, int t, int i) {
     return s.tr_q + ((size_t)t * s.tr_cap + i) * s.D;
 }
@@ -747,12 +761,8 @@ __global__ __launch_bounds__(64) void k_dr_claim(SlotDev *slots, const int *ids)
     if (!first) return;
     const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
     const int D = s.D;
-    const float *from = tree_q(s, s.ex_tree[b], s.ex_near[b]);
-    const float *to = s.ex_q + (size_t)b * D;
-    for (int d = 0; d < D; ++d) {
-        s.ed_from[(size_t)slot * D + d] = from[d];
-        s.ed_to[(size_t)slot * D + d] = to[d];
-    }
+    copy_row(s.ed_from + (size_t)slot * D, tree_q(s, s.ex_tree[b], s.ex_near[b]), D);
+    copy_row(s.ed_to + (size_t)slot * D, s.ex_q + (size_t)b * D, D);
     s.ed_b[slot] = b;
     s.ed_coll[slot] = 0u;  // the checker only sets bits
 }
@@ -1232,9 +1242,7 @@ __global__ __launch_bounds__(256) void k_dr_commit(SlotDev *slots, const int *id
             continue;
         }
         s.ex_index[b] = idx;
-        float *dst = tree_q(s, t, idx);
-        const float *src = s.ex_q + (size_t)b * s.D;
-        for (int d = 0; d < s.D; ++d) dst[d] = src[d];
+        copy_row(tree_q(s, t, idx), s.ex_q + (size_t)b * s.D, s.D);
         for (int a = 0; a < s.A; ++a) s.tr_rid[((size_t)t * s.tr_cap + idx) * s.A + a] = s.ex_rid[b * s.A + a];
         s.tr_parent[(size_t)t * s.tr_cap + idx] = s.ex_near[b];
         s.tr_cost[(size_t)t * s.tr_cap + idx] = s.ex_cost[b];
@@ -1511,8 +1519,7 @@ __global__ __launch_bounds__(256) void k_dr_finish(SlotDev *slots, const int *id
         int cur = start_node[1], k = chain_n[1] - 1;
         while (true) {
             const int par = s.tr_parent[(size_t)s.tr_cap + cur];
-            const float *src = tree_q(s, 1, cur);
-            for (int d = 0; d < D; ++d) s.path[(size_t)k * D + d] = src[d];
+            copy_row(s.path + (size_t)k * D, tree_q(s, 1, cur), D);
             if (par == cur) break;
             cost += sqrtf(sqdist(tree_q(s, 1, cur), tree_q(s, 1, par), D));
             cur = par;
@@ -1523,8 +1530,7 @@ __global__ __launch_bounds__(256) void k_dr_finish(SlotDev *slots, const int *id
         int cur = start_node[0], k = chain_n[1];
         while (true) {
             const int par = s.tr_parent[cur];
-            const float *src = tree_q(s, 0, cur);  // the start tree's only root is node 0
-            for (int d = 0; d < D; ++d) s.path[(size_t)k * D + d] = src[d];
+            copy_row(s.path + (size_t)k * D, tree_q(s, 0, cur), D);  // the start tree's only root is node 0
             if (par == cur) break;
             cost += sqrtf(sqdist(tree_q(s, 0, cur), tree_q(s, 0, par), D));
             cur = par;
@@ -1713,7 +1719,7 @@ __global__ void k_sp_shortcut(SlotDev *slots, const int *ids) {
         int child = N - 1, parent = par[child];
         while (parent < N && parent != -1) {
             for (int idx = parent + 1; idx < child; ++idx)
-                for (int d = 0; d < D; ++d) s.path[(size_t)idx * D + d] = s.path[(size_t)parent * D + d];
+                copy_row(s.path + (size_t)idx * D, s.path + (size_t)parent * D, D);
             child = parent;
             parent = par[child];
         }
