@@ -904,6 +904,10 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
                     if (__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) < lim * lim) need |= 1u << t;
                 }
             }
+            // rows 6..10 are five: row 5 of that half would repeat row 10,
+            // whose hit bit row 4 already gives (its candidates are kept
+            // only for the half of rows 0..5)
+            if (i0 != 0) need &= 0x1fu;
             if (!active) need = 0;
             const unsigned long long rows_any = __ballot(need != 0u);
             if (!rows_any) {
