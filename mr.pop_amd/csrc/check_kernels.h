@@ -925,30 +925,44 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
             unsigned mask = 0;
             unsigned long long clo = 0;
             unsigned chi = 0;
-#pragma unroll 1
-            for (int t = 0; t < 6; ++t) {
-                if (!__ballot((need >> t) & 1u)) continue;
-                const int i = min(i0 + t, kCoarseSpheres - 1);  // rows 6..10 are 5: repeat row 10
-                const float ax = sm.cs[i][0][a * S + slot], ay = sm.cs[i][1][a * S + slot], az = sm.cs[i][2][a * S + slot];
-                const float rad = kTab.coarse_r[i], brad = kTab.block_r[i];
-                bool hit = false;
-                unsigned cand = 0;
-                static_for<0, kCoarseSpheres>([&](auto jc) {
-                    constexpr int j = decltype(jc)::value;
-                    constexpr float rj = kCoarseSphere[j][3], Rj = kCoarseSphere[j][3] + kBlockInflate[j];
-                    const float dx = ax - bx[j], dy = ay - by[j], dz = az - bz[j];
-                    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                    const float rs = rad + rj;
-                    const float Rs = brad + Rj;
-                    hit |= d2 - rs * rs < 0.f;
-                    if (d2 < Rs * Rs) cand |= 1u << j;
+            // Rows 0..5 (i0 = 0) or 6..10 (i0 = 6) of arm a against the 11
+            // coarse spheres of arm b, one code path per half so that each
+            // row's radius is known at compile time: both thresholds of a
+            // pair, (r_i + r_j)^2 and the block balls' (R_i + R_j)^2, are the
+            // same float values the runtime sums and products give, folded
+            // into literals.
+            auto rows = [&](auto i0c) {
+                constexpr int I0 = decltype(i0c)::value;
+                static_for<0, (I0 == 0 ? 6 : 5)>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int i = I0 + t;
+                    if (!__ballot((need >> t) & 1u)) return;
+                    const float ax = sm.cs[i][0][a * S + slot], ay = sm.cs[i][1][a * S + slot],
+                                az = sm.cs[i][2][a * S + slot];
+                    constexpr float rad = kCoarseSphere[i][3], brad = kCoarseSphere[i][3] + kBlockInflate[i];
+                    bool hit = false;
+                    unsigned cand = 0;
+                    static_for<0, kCoarseSpheres>([&](auto jc) {
+                        constexpr int j = decltype(jc)::value;
+                        constexpr float rj = kCoarseSphere[j][3], Rj = kCoarseSphere[j][3] + kBlockInflate[j];
+                        constexpr float rs = rad + rj, Rs = brad + Rj;
+                        constexpr float rs2 = rs * rs, Rs2 = Rs * Rs;
+                        const float dx = ax - bx[j], dy = ay - by[j], dz = az - bz[j];
+                        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                        hit |= d2 < rs2;  // d2 - (r_i + r_j)^2 < 0
+                        if (d2 < Rs2) cand |= 1u << j;
+                    });
+                    if (hit) mask |= 1u << (i < 7 ? i : 7);
+                    if constexpr (t < 5)
+                        clo |= (unsigned long long)cand << (11 * t);
+                    else
+                        chi = cand;  // row 5 of the half of rows 0..5
                 });
-                if (hit) mask |= 1u << (i < 7 ? i : 7);
-                if (t < 5)
-                    clo |= (unsigned long long)cand << (11 * t);
-                else if (i0 == 0)
-                    chi = cand;
-            }
+            };
+            if (i0 == 0)
+                rows(IntC<0>{});
+            else
+                rows(IntC<6>{});
             sm.cand_lo[m][lane] = active ? clo : 0ull;
             sm.cand_hi[m][lane] = active ? chi : 0u;
             if (mask && item) atomicOr(&sm.inter_flags[a * S + slot], mask);
