@@ -541,6 +541,7 @@ template <int KH, int NT> struct TileLds {
     unsigned cand_hi[KH][NT];            // row 5: bit k
     unsigned edge_done[NT];              // per edge of the tile: a sample collides
     unsigned fine[NT];                   // per item: its fine tests found a collision
+    int hp_next;                         // next half-pair of the fine arm-arm stage to hand out
 };
 
 // Samples per edge in a launch of the given pass.  Pass 0 checks every
@@ -785,6 +786,7 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
     sm.result[lane] = selfbits;  // intra-arm part of the self bits
     sm.edge_done[lane] = 0u;
     __syncthreads();
+    if (lane == 0) sm.hp_next = 0;  // read only after barrier (2)
 
     // Lean kernels: the coarse environment tests, with this wave's centres in
     // LDS.  (a) Lanes 0..29 take the hull (box) of one coordinate of one of
@@ -1003,16 +1005,37 @@ __device__ __forceinline__ void check_tile(TileLds<KH, NT> &sm, const WorldDesc 
     }
     if (active && finehit) done[le] = 1u;
 
-    // ---- C. fine arm-arm pairs for candidate blocks of this lane's half-pairs
+    // ---- C. fine arm-arm pairs for candidate blocks of the half-pairs.
+    // With one wave per arm (64 samples per tile) the tile's A (A - 1)
+    // half-pairs go to the waves as they come free, each wave taking a
+    // half-pair for all 64 samples: a fixed share kept the waves holding
+    // close arm pairs busy while the others waited at barrier (3).  Other
+    // tilings keep the fixed share: each item takes its arm's A - 1.
     if constexpr (kInter) {
+        constexpr bool kShare = S == 64;
+        int m_next = 0;
 #pragma unroll 1
-        for (int m = 0; m < A - 1; ++m) {
-            if (early_out && active && done[le]) break;
+        while (true) {
+            int h;
+            if constexpr (kShare) {
+                int g = 0;
+                if ((lane & 63) == 0) g = atomicAdd(&sm.hp_next, 1);
+                h = __shfl(g, 0, 64);
+                if (h >= A * (A - 1)) break;
+                if (early_out && active && done[le]) continue;
+            } else {
+                if (m_next >= A - 1 || (early_out && active && done[le])) break;
+                h = arm + m_next * A;
+                ++m_next;
+            }
+            // half-pair h = arm w's m-th: its candidates sit in item (w, slot)
+            const int m = h / A, owner = h - m * A;
+            const int src = kShare ? owner * S + slot : lane;
             int a, b, i0;
-            half_pair<A>(arm + m * A, a, b, i0);
+            half_pair<A>(h, a, b, i0);
             const unsigned gate = sm.result[a * S + slot] | sm.inter_flags[a * S + slot];  // arm a's self bits
-            unsigned long long clo = sm.cand_lo[m][lane];
-            unsigned chi = sm.cand_hi[m][lane];
+            unsigned long long clo = sm.cand_lo[m][src];
+            unsigned chi = sm.cand_hi[m][src];
             // keep rows whose block's joint is self-flagged on arm a
 #pragma unroll
             for (int t = 0; t < 5; ++t) {
