@@ -85,17 +85,32 @@ __device__ __forceinline__ float *tree_q(const SlotDev &s, int t, int i) {
 constexpr int kScanRows = 64;
 __device__ __forceinline__ int wave_stride(int D) { return D | 1; }
 __device__ __forceinline__ void stage_rows(const float *rows, int base, int c, int D, int S, float *stage, int lane) {
+    // c <= kScanRows rows of D <= 35 floats: at most kPerLane floats a lane.
+    // Every load of the chunk is issued before the first store, so a chunk
+    // waits for one memory latency, not one per 64 floats.
+    constexpr int kPerLane = kScanRows * 35 / 64;
     const float *src = rows + (size_t)base * D;
     const int nf = c * D;
+    float v[kPerLane];
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+        const int x = lane + 64 * k;
+        if (x < nf) v[k] = src[x];
+    }
     if (S == D) {
-        for (int x = lane; x < nf; x += 64) stage[x] = src[x];
+#pragma unroll
+        for (int k = 0; k < kPerLane; ++k) {
+            const int x = lane + 64 * k;
+            if (x < nf) stage[x] = v[k];
+        }
         return;
     }
     // element x = r * D + col goes to r * S + col; x advances by 64 = dr * D + dc
     const int dr = 64 / D, dc = 64 - dr * D;
     int r = lane / D, col = lane - r * D;
-    for (int x = lane; x < nf; x += 64) {
-        stage[r * S + col] = src[x];
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+        if (lane + 64 * k < nf) stage[r * S + col] = v[k];
         r += dr;
         col += dc;
         if (col >= D) {
